@@ -1,0 +1,119 @@
+/*
+ * scalapack_amd.h -- C-ABI of the MI355X (gfx950) dense LU solve library.
+ *
+ * Part 1 is the drop-in: the Fortran-callable ScaLAPACK entry points of the
+ * LU solve path, with the reference's exact calling sequence (every argument
+ * by reference, 9-integer descriptors, INFO semantics, gfortran/flang
+ * "Add_" name mangling as in the reference's SLmake.inc.example:21 and
+ * CMAKE/FortranMangling.cmake:42).  A Fortran program that calls PDGESV /
+ * PDGETRF / PDGETRS links against libscalapack_amd.so instead of (or ahead
+ * of) libscalapack and runs them on the GPU for a 1 x 1 BLACS grid.  Arrays
+ * are host memory, as in the reference; they are staged to HBM inside the
+ * call.  Character arguments carry the hidden length Fortran compilers
+ * append; it is accepted and ignored, as the reference's C PBLAS does.
+ *
+ * Part 2 is a device-resident interface (pointers into HBM) for callers that
+ * keep the matrix on the GPU; the benchmark and the GPU tests use it.
+ *
+ * The 1 x 1 BLACS context routines the drop-in needs (BLACS_GRIDINFO etc.)
+ * come from libsla_blacs1x1.so (sla_blacs1x1.h) unless a real BLACS is linked
+ * ahead of it.
+ */
+#ifndef SCALAPACK_AMD_H
+#define SCALAPACK_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Part 1: drop-in ScaLAPACK entry points (Fortran ABI, 32-bit INTEGER).     */
+/* ------------------------------------------------------------------------ */
+
+/* PDGESV: solve sub(A) * X = sub(B).  Replaces SRC/pdgesv.f:1-2
+ * (checks :185-225, PDGETRF :229, PDGETRS :236). */
+void pdgesv_(const int *n, const int *nrhs, double *a, const int *ia, const int *ja,
+             const int *desca, int *ipiv, double *b, const int *ib, const int *jb,
+             const int *descb, int *info);
+
+/* PDGETRF: LU factorization with partial pivoting, sub(A) = P * L * U.
+ * Replaces SRC/pdgetrf.f:1 (checks :169-199, quick return :201-206,
+ * blocked loop :217-295).  IPIV(i) = global row interchanged with row i. */
+void pdgetrf_(const int *m, const int *n, double *a, const int *ia, const int *ja,
+              const int *desca, int *ipiv, int *info);
+
+/* PDGETRS: solve with the factors of PDGETRF.  Replaces SRC/pdgetrs.f:1-2
+ * (checks :184-237, PDLAPIV :255, PDTRSM :260,265).  TRANS = 'N' only;
+ * 'T' and 'C' (pdgetrs.f:273-284, not on the PDGESV path) return INFO = -1
+ * with a message. */
+void pdgetrs_(const char *trans, const int *n, const int *nrhs, const double *a, const int *ia,
+              const int *ja, const int *desca, const int *ipiv, double *b, const int *ib,
+              const int *jb, const int *descb, int *info, size_t trans_len);
+
+/* PDGETF2: unblocked panel factorization, N + MOD(JA-1, NB) <= NB.
+ * Replaces SRC/pdgetf2.f:1 (checks :170-193: an illegal argument prints the
+ * PXERBLA message and aborts, as BLACS_ABORT does there). */
+void pdgetf2_(const int *m, const int *n, double *a, const int *ia, const int *ja,
+              const int *desca, int *ipiv, int *info);
+
+/* PDLASWP: row ('R') or column ('C') interchanges K1..K2, forward ('F') or
+ * backward ('B').  Replaces SRC/pdlaswp.f:1-2 (:159-203). */
+void pdlaswp_(const char *direc, const char *rowcol, const int *n, double *a, const int *ia,
+              const int *ja, const int *desca, const int *k1, const int *k2, const int *ipiv,
+              size_t direc_len, size_t rowcol_len);
+
+/* ------------------------------------------------------------------------ */
+/* Part 2: device-resident interface.                                        */
+/* ------------------------------------------------------------------------ */
+
+typedef struct sla_handle_s *sla_handle_t;
+
+/* Return codes of the sla_* functions: 0 success, SLA_ERR_* on failure
+ * (the message is in sla_last_error()).  INFO-style results (a zero pivot)
+ * are returned through an `info` argument, never as an error. */
+#define SLA_OK 0
+#define SLA_ERR_ARG (-1)
+#define SLA_ERR_RUNTIME (-2)
+
+/* Creates a handle bound to the current HIP device. */
+int sla_create(sla_handle_t *h);
+int sla_destroy(sla_handle_t h);
+/* Work of the handle goes to `stream` (a hipStream_t; NULL restores the
+ * handle's own stream).  Calls return after the stream has drained. */
+int sla_set_stream(sla_handle_t h, void *stream);
+/* Tuning knobs: "nb" (block column width), "leaf_w" (panel leaf width,
+ * <= 32), "panel_wgs" (workgroups per panel leaf), "profile" (0/1). */
+int sla_set_param(sla_handle_t h, const char *name, int value);
+int sla_get_param(sla_handle_t h, const char *name, int *value);
+const char *sla_last_error(void);
+
+/* Per-kernel-class timing collected while "profile" is 1: classes
+ * 0 panel, 1 laswp, 2 trsm, 3 gemm, 4 trsv.  ms = summed kernel time,
+ * work = summed algorithmic flops (bytes for laswp), launches = count. */
+int sla_get_stats(sla_handle_t h, int cls, double *ms, double *work, long long *launches);
+int sla_reset_stats(sla_handle_t h);
+
+/* FP64 LU of the m x n column-major matrix dA (leading dimension lda) in
+ * HBM.  dipiv: device int[min(m,n)], 1-based pivot rows (PDGETRF's IPIV for
+ * IA = JA = 1).  *info as PDGETRF's INFO (> 0: exactly zero pivot). */
+int sla_dgetrf(sla_handle_t h, int m, int n, double *dA, int lda, int *dipiv, int *info);
+/* Solve A X = B with the factors of sla_dgetrf (TRANS = 'N'). */
+int sla_dgetrs(sla_handle_t h, int n, int nrhs, const double *dA, int lda, const int *dipiv,
+               double *dB, int ldb);
+int sla_dgesv(sla_handle_t h, int n, int nrhs, double *dA, int lda, int *dipiv, double *dB,
+              int ldb, int *info);
+
+/* The trailing-update kernel alone: C[m x n] -= A[m x k] * B[k x n] (FP64,
+ * column-major, device pointers) -- PDGEMM's local dgemm_('N','N', ..., -1,
+ * L21, U12, +1, A22) of SRC/pdgetrf.f:288 (PBLAS/SRC/PTOOLS/PB_CpgemmAB.c:
+ * 333-352).  For kernel-level checks and measurement. */
+int sla_dgemm_sub(sla_handle_t h, int m, int n, int k, const double *dA, int lda,
+                  const double *dB, int ldb, double *dC, int ldc);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SCALAPACK_AMD_H */

@@ -1,0 +1,599 @@
+// capi.hip -- the C-ABI of libscalapack_amd.so (include/scalapack_amd.h).
+//
+// The Fortran drop-in entry points validate their arguments with the
+// reference's rules and INFO codes (TOOLS/chk1mat.f:1-100; SRC/pdgetrf.f:
+// 173-199; SRC/pdgetrs.f:188-237; SRC/pdgesv.f:178-225; SRC/pdgetf2.f:170-193),
+// print the PXERBLA message (PBLAS/SRC/PTZBLAS/pxerbla.f:55-58) for an illegal
+// argument, and run the factorization / solve on the GPU.  On a grid other
+// than 1 x 1 the call is forwarded to the next definition of the same symbol
+// in the process (a real ScaLAPACK linked after this library); without one it
+// reports the illegal context.
+#include "sla_common.h"
+#include "sla_handle.h"
+#include "sla_kernels.h"
+
+#include "scalapack_amd.h"
+#include "sla_blacs1x1.h"
+
+#include <dlfcn.h>
+
+#include <algorithm>
+#include <climits>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_last_error;
+
+// Fortran descriptor entries (1-based, TOOLS/descinit.f).
+enum { DTYPE_ = 1, CTXT_ = 2, M_ = 3, N_ = 4, MB_ = 5, NB_ = 6, RSRC_ = 7, CSRC_ = 8, LLD_ = 9 };
+inline int D(const int *desc, int e) { return desc[e - 1]; }
+
+struct Grid {
+    int nprow, npcol, myrow, mycol;
+};
+Grid gridinfo(int ctxt)
+{
+    Grid g;
+    blacs_gridinfo_(&ctxt, &g.nprow, &g.npcol, &g.myrow, &g.mycol);
+    return g;
+}
+
+int numroc(int n, int nb, int iproc, int isrc, int nprocs)
+{
+    return numroc_(&n, &nb, &iproc, &isrc, &nprocs);
+}
+
+// CHK1MAT (TOOLS/chk1mat.f), with its DESCMULT encoding of descriptor entries.
+void chk1mat(int ma, int mapos0, int na, int napos0, int ia, int ja, const int *desca,
+             int descapos0, int &info)
+{
+    // The grid is that of this descriptor's own context (chk1mat.f).
+    const Grid g = gridinfo(D(desca, CTXT_));
+    const int DESCMULT = 100, BIGNUM = DESCMULT * DESCMULT;
+    if (info >= 0) info = BIGNUM;
+    else if (info < -DESCMULT) info = -info;
+    else info = -info * DESCMULT;
+    const int mapos = mapos0 * DESCMULT, napos = napos0 * DESCMULT;
+    const int iapos = (descapos0 - 2) * DESCMULT, japos = (descapos0 - 1) * DESCMULT;
+    const int descapos = descapos0 * DESCMULT;
+    if (D(desca, DTYPE_) != 1) info = std::min(info, descapos + DTYPE_);
+    else if (ma < 0) info = std::min(info, mapos);
+    else if (na < 0) info = std::min(info, napos);
+    else if (ia < 1) info = std::min(info, iapos);
+    else if (ja < 1) info = std::min(info, japos);
+    else if (D(desca, MB_) < 1) info = std::min(info, descapos + MB_);
+    else if (D(desca, NB_) < 1) info = std::min(info, descapos + NB_);
+    else if (D(desca, RSRC_) < 0 || D(desca, RSRC_) >= g.nprow) info = std::min(info, descapos + RSRC_);
+    else if (D(desca, CSRC_) < 0 || D(desca, CSRC_) >= g.npcol) info = std::min(info, descapos + CSRC_);
+    else if (D(desca, LLD_) < 1) info = std::min(info, descapos + LLD_);
+    else if (D(desca, LLD_) < numroc(D(desca, M_), D(desca, MB_), g.myrow, D(desca, RSRC_), g.nprow)) {
+        if (numroc(D(desca, N_), D(desca, NB_), g.mycol, D(desca, CSRC_), g.npcol) > 0)
+            info = std::min(info, descapos + LLD_);
+    }
+    if (ma == 0 || na == 0) {
+        if (D(desca, M_) < 0) info = std::min(info, descapos + M_);
+        if (D(desca, N_) < 0) info = std::min(info, descapos + N_);
+    } else {
+        if (D(desca, M_) < 1) info = std::min(info, descapos + M_);
+        else if (D(desca, N_) < 1) info = std::min(info, descapos + N_);
+        else {
+            if (ia > D(desca, M_)) info = std::min(info, iapos);
+            else if (ja > D(desca, N_)) info = std::min(info, japos);
+            else {
+                if (ia + ma - 1 > D(desca, M_)) info = std::min(info, mapos);
+                if (ja + na - 1 > D(desca, N_)) info = std::min(info, napos);
+            }
+        }
+    }
+    if (info == BIGNUM) info = 0;
+    else if (info % DESCMULT == 0) info = -info / DESCMULT;
+    else info = -info;
+}
+
+void pxerbla(const Grid &g, const char *name, int pos)
+{
+    std::printf("{%5d,%5d}:  On entry to %s parameter number %4d had an illegal value\n",
+                g.myrow, g.mycol, name, pos);
+    std::fflush(stdout);
+}
+
+int indxg2p(int g, int nb, int isrc, int nprocs) { return (isrc + (g - 1) / nb) % nprocs; }
+
+// --- process-wide default handle per device ---------------------------------
+std::mutex g_mu;
+std::map<int, std::unique_ptr<sla::Handle>> g_handles;
+
+sla::Handle &default_handle()
+{
+    int dev = 0;
+    SLA_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto &p = g_handles[dev];
+    if (!p) p.reset(new sla::Handle());
+    return *p;
+}
+
+// Device copies of host arrays for the Fortran entry points.
+struct DevBuf {
+    void *p = nullptr;
+    explicit DevBuf(size_t bytes) { SLA_HIP(hipMalloc(&p, bytes ? bytes : 8)); }
+    ~DevBuf() { if (p) hipFree(p); }
+    template <typename T> T *as() { return static_cast<T *>(p); }
+};
+
+void to_device(double *dst, int ldd, const double *src, int lds, int m, int n)
+{
+    SLA_HIP(hipMemcpy2D(dst, (size_t)ldd * 8, src, (size_t)lds * 8, (size_t)m * 8, n,
+                        hipMemcpyHostToDevice));
+}
+void to_host(double *dst, int ldd, const double *src, int lds, int m, int n)
+{
+    SLA_HIP(hipMemcpy2D(dst, (size_t)ldd * 8, src, (size_t)lds * 8, (size_t)m * 8, n,
+                        hipMemcpyDeviceToHost));
+}
+int pad_ld(int m) { return std::max(64, (m + 63) / 64 * 64); }
+
+template <typename Fn>
+Fn next_symbol(const char *name)
+{
+    return reinterpret_cast<Fn>(dlsym(RTLD_NEXT, name));
+}
+
+[[noreturn]] void fatal(const char *routine, const std::exception &e)
+{
+    std::fprintf(stderr, "%s: GPU execution failed: %s\n", routine, e.what());
+    std::fflush(stderr);
+    std::abort();
+}
+
+// Factor sub(A) (host, local array a with LLD) on the device.  Leaves the
+// device copy and pivots in dA/dipiv for a following solve.
+int getrf_host(sla::Handle &h, int m, int n, double *a, int ia, int ja, int lld, int *ipiv,
+               DevBuf &dA, int ldd, DevBuf &dipiv)
+{
+    double *asub = a + (size_t)(ia - 1) + (size_t)(ja - 1) * lld;
+    to_device(dA.as<double>(), ldd, asub, lld, m, n);
+    const int info = sla::dgetrf_device(h, m, n, dA.as<double>(), ldd, dipiv.as<int>());
+    to_host(asub, lld, dA.as<double>(), ldd, m, n);
+    const int mn = std::min(m, n);
+    std::vector<int> piv(mn);
+    SLA_HIP(hipMemcpy(piv.data(), dipiv.p, sizeof(int) * mn, hipMemcpyDeviceToHost));
+    // IPIV holds global rows, indexed by the local row (SRC/pdgetrf.f:118-121).
+    for (int i = 0; i < mn; i++) ipiv[(ia - 1) + i] = piv[i] + (ia - 1);
+    return info;
+}
+
+}  // namespace
+
+// ============================================================================
+// Part 1: Fortran drop-in
+// ============================================================================
+extern "C" {
+
+void pdgetrf_(const int *m, const int *n, double *a, const int *ia, const int *ja,
+              const int *desca, int *ipiv, int *info)
+{
+    const int ictxt = D(desca, CTXT_);
+    const Grid g = gridinfo(ictxt);
+    *info = 0;
+    if (g.nprow == -1) {
+        *info = -(600 + CTXT_);
+    } else {
+        chk1mat(*m, 1, *n, 2, *ia, *ja, desca, 6, *info);
+        if (*info == 0) {
+            const int iroff = (*ia - 1) % D(desca, MB_), icoff = (*ja - 1) % D(desca, NB_);
+            if (iroff != 0) *info = -4;
+            else if (icoff != 0) *info = -5;
+            else if (D(desca, MB_) != D(desca, NB_)) *info = -(600 + NB_);
+        }
+    }
+    if (*info != 0) {
+        pxerbla(g, "PDGETRF", -*info);
+        return;
+    }
+    if (D(desca, M_) == 1) {  // pdgetrf.f:201-203
+        ipiv[0] = 1;
+        return;
+    }
+    if (*m == 0 || *n == 0) return;
+    if (g.nprow * g.npcol != 1) {
+        auto next = next_symbol<void (*)(const int *, const int *, double *, const int *,
+                                         const int *, const int *, int *, int *)>("pdgetrf_");
+        if (next) return next(m, n, a, ia, ja, desca, ipiv, info);
+        *info = -(600 + CTXT_);
+        pxerbla(g, "PDGETRF", -*info);
+        return;
+    }
+    try {
+        sla::Handle &h = default_handle();
+        const int ldd = pad_ld(*m);
+        DevBuf dA((size_t)ldd * *n * 8), dp((size_t)std::min(*m, *n) * 4);
+        *info = getrf_host(h, *m, *n, a, *ia, *ja, D(desca, LLD_), ipiv, dA, ldd, dp);
+    } catch (const std::exception &e) {
+        fatal("PDGETRF", e);
+    }
+}
+
+void pdgetf2_(const int *m, const int *n, double *a, const int *ia, const int *ja,
+              const int *desca, int *ipiv, int *info)
+{
+    const int ictxt = D(desca, CTXT_);
+    const Grid g = gridinfo(ictxt);
+    *info = 0;
+    if (g.nprow == -1) {
+        *info = -(600 + CTXT_);
+    } else {
+        chk1mat(*m, 1, *n, 2, *ia, *ja, desca, 6, *info);
+        if (*info == 0) {
+            const int iroff = (*ia - 1) % D(desca, MB_), icoff = (*ja - 1) % D(desca, NB_);
+            if (*n + icoff > D(desca, NB_)) *info = -2;
+            else if (iroff != 0) *info = -4;
+            else if (icoff != 0) *info = -5;
+            else if (D(desca, MB_) != D(desca, NB_)) *info = -(600 + NB_);
+        }
+    }
+    if (*info != 0) {
+        pxerbla(g, "PDGETF2", -*info);
+        int one = 1;
+        blacs_abort_(&ictxt, &one);  // pdgetf2.f:190-193
+        return;
+    }
+    if (*m == 0 || *n == 0) return;
+    if (g.nprow * g.npcol != 1) {
+        auto next = next_symbol<void (*)(const int *, const int *, double *, const int *,
+                                         const int *, const int *, int *, int *)>("pdgetf2_");
+        if (next) return next(m, n, a, ia, ja, desca, ipiv, info);
+        *info = -(600 + CTXT_);
+        pxerbla(g, "PDGETF2", -*info);
+        return;
+    }
+    try {
+        sla::Handle &h = default_handle();
+        const int ldd = pad_ld(*m);
+        DevBuf dA((size_t)ldd * *n * 8), dp((size_t)std::min(*m, *n) * 4);
+        *info = getrf_host(h, *m, *n, a, *ia, *ja, D(desca, LLD_), ipiv, dA, ldd, dp);
+    } catch (const std::exception &e) {
+        fatal("PDGETF2", e);
+    }
+}
+
+static int getrs_checks(char trans, int n, int nrhs, int ia, int ja, const int *desca, int ib,
+                        int jb, const int *descb, const Grid &g)
+{
+    int info = 0;
+    if (g.nprow == -1) return -(700 + CTXT_);
+    const bool notran = (trans == 'N' || trans == 'n');
+    chk1mat(n, 2, n, 2, ia, ja, desca, 7, info);
+    chk1mat(n, 2, nrhs, 3, ib, jb, descb, 12, info);
+    if (info == 0) {
+        const int iarow = indxg2p(ia, D(desca, MB_), D(desca, RSRC_), g.nprow);
+        const int ibrow = indxg2p(ib, D(descb, MB_), D(descb, RSRC_), g.nprow);
+        const int iroffa = (ia - 1) % D(desca, MB_), icoffa = (ja - 1) % D(desca, NB_);
+        const int iroffb = (ib - 1) % D(descb, MB_);
+        if (!notran && !(trans == 'T' || trans == 't') && !(trans == 'C' || trans == 'c')) info = -1;
+        else if (iroffa != 0) info = -5;
+        else if (icoffa != 0) info = -6;
+        else if (D(desca, MB_) != D(desca, NB_)) info = -(700 + NB_);
+        else if (iroffb != 0 || ibrow != iarow) info = -10;
+        else if (D(descb, MB_) != D(desca, NB_)) info = -(1200 + NB_);
+        else if (D(desca, CTXT_) != D(descb, CTXT_)) info = -(1200 + CTXT_);
+    }
+    return info;
+}
+
+void pdgetrs_(const char *trans, const int *n, const int *nrhs, const double *a, const int *ia,
+              const int *ja, const int *desca, const int *ipiv, double *b, const int *ib,
+              const int *jb, const int *descb, int *info, size_t)
+{
+    const Grid g = gridinfo(D(desca, CTXT_));
+    *info = getrs_checks(*trans, *n, *nrhs, *ia, *ja, desca, *ib, *jb, descb, g);
+    if (*info != 0) {
+        pxerbla(g, "PDGETRS", -*info);
+        return;
+    }
+    if (*n == 0 || *nrhs == 0) return;
+    if (g.nprow * g.npcol != 1) {
+        auto next = next_symbol<void (*)(const char *, const int *, const int *, const double *,
+                                         const int *, const int *, const int *, const int *,
+                                         double *, const int *, const int *, const int *, int *,
+                                         size_t)>("pdgetrs_");
+        if (next) return next(trans, n, nrhs, a, ia, ja, desca, ipiv, b, ib, jb, descb, info, 1);
+        *info = -(700 + CTXT_);
+        pxerbla(g, "PDGETRS", -*info);
+        return;
+    }
+    if (!(*trans == 'N' || *trans == 'n')) {
+        std::printf("PDGETRS: TRANS = '%c' is not provided by this library (only 'N')\n", *trans);
+        *info = -1;
+        return;
+    }
+    try {
+        sla::Handle &h = default_handle();
+        const int N = *n, NR = *nrhs, ldd = pad_ld(N), lldb = D(descb, LLD_);
+        DevBuf dA((size_t)ldd * N * 8), dB((size_t)ldd * NR * 8), dp((size_t)N * 4);
+        to_device(dA.as<double>(), ldd, a + (size_t)(*ia - 1) + (size_t)(*ja - 1) * D(desca, LLD_),
+                  D(desca, LLD_), N, N);
+        double *bsub = b + (size_t)(*ib - 1) + (size_t)(*jb - 1) * lldb;
+        to_device(dB.as<double>(), ldd, bsub, lldb, N, NR);
+        std::vector<int> piv(N);
+        for (int i = 0; i < N; i++) piv[i] = ipiv[(*ia - 1) + i] - (*ia - 1);  // pdlapv2.f:229
+        SLA_HIP(hipMemcpy(dp.p, piv.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+        sla::dgetrs_device(h, N, NR, dA.as<double>(), ldd, dp.as<int>(), dB.as<double>(), ldd);
+        to_host(bsub, lldb, dB.as<double>(), ldd, N, NR);
+    } catch (const std::exception &e) {
+        fatal("PDGETRS", e);
+    }
+}
+
+void pdgesv_(const int *n, const int *nrhs, double *a, const int *ia, const int *ja,
+             const int *desca, int *ipiv, double *b, const int *ib, const int *jb,
+             const int *descb, int *info)
+{
+    const Grid g = gridinfo(D(desca, CTXT_));
+    *info = 0;
+    if (g.nprow == -1) {
+        *info = -(600 + CTXT_);
+    } else {
+        chk1mat(*n, 1, *n, 1, *ia, *ja, desca, 6, *info);
+        chk1mat(*n, 1, *nrhs, 2, *ib, *jb, descb, 11, *info);
+        if (*info == 0) {
+            const int iarow = indxg2p(*ia, D(desca, MB_), D(desca, RSRC_), g.nprow);
+            const int ibrow = indxg2p(*ib, D(descb, MB_), D(descb, RSRC_), g.nprow);
+            const int iroffa = (*ia - 1) % D(desca, MB_), icoffa = (*ja - 1) % D(desca, NB_);
+            const int iroffb = (*ib - 1) % D(descb, MB_);
+            if (iroffa != 0) *info = -4;
+            else if (icoffa != 0) *info = -5;
+            else if (D(desca, MB_) != D(desca, NB_)) *info = -(600 + NB_);
+            else if (ibrow != iarow || icoffa != iroffb) *info = -9;
+            else if (D(descb, MB_) != D(desca, NB_)) *info = -(1100 + NB_);
+            else if (D(desca, CTXT_) != D(descb, CTXT_)) *info = -(1100 + CTXT_);
+        }
+    }
+    if (*info != 0) {
+        pxerbla(g, "PDGESV", -*info);
+        return;
+    }
+    if (g.nprow * g.npcol != 1) {
+        auto next = next_symbol<void (*)(const int *, const int *, double *, const int *,
+                                         const int *, const int *, int *, double *, const int *,
+                                         const int *, const int *, int *)>("pdgesv_");
+        if (next) return next(n, nrhs, a, ia, ja, desca, ipiv, b, ib, jb, descb, info);
+        *info = -(600 + CTXT_);
+        pxerbla(g, "PDGESV", -*info);
+        return;
+    }
+    if (*n == 0) return;
+    if (D(desca, M_) == 1) {  // PDGETRF's quick return (pdgetrf.f:201-203)
+        ipiv[0] = 1;
+    }
+    try {
+        sla::Handle &h = default_handle();
+        const int N = *n, NR = *nrhs, ldd = pad_ld(N), lldb = D(descb, LLD_);
+        DevBuf dA((size_t)ldd * N * 8), dp((size_t)N * 4), dB((size_t)ldd * std::max(NR, 1) * 8);
+        if (D(desca, M_) != 1)
+            *info = getrf_host(h, N, N, a, *ia, *ja, D(desca, LLD_), ipiv, dA, ldd, dp);
+        else
+            to_device(dA.as<double>(), ldd, a + (size_t)(*ia - 1) + (size_t)(*ja - 1) * D(desca, LLD_),
+                      D(desca, LLD_), N, N);
+        if (*info == 0 && NR > 0) {  // pdgesv.f:231
+            if (D(desca, M_) == 1) {
+                int one = 1;
+                SLA_HIP(hipMemcpy(dp.p, &one, sizeof one, hipMemcpyHostToDevice));
+            }
+            double *bsub = b + (size_t)(*ib - 1) + (size_t)(*jb - 1) * lldb;
+            to_device(dB.as<double>(), ldd, bsub, lldb, N, NR);
+            sla::dgetrs_device(h, N, NR, dA.as<double>(), ldd, dp.as<int>(), dB.as<double>(), ldd);
+            to_host(bsub, lldb, dB.as<double>(), ldd, N, NR);
+        }
+    } catch (const std::exception &e) {
+        fatal("PDGESV", e);
+    }
+}
+
+void pdlaswp_(const char *direc, const char *rowcol, const int *n, double *a, const int *ia,
+              const int *ja, const int *desca, const int *k1, const int *k2, const int *ipiv,
+              size_t, size_t)
+{
+    if (*n == 0) return;
+    const Grid g = gridinfo(D(desca, CTXT_));
+    if (g.nprow * g.npcol != 1) {
+        auto next = next_symbol<void (*)(const char *, const char *, const int *, double *,
+                                         const int *, const int *, const int *, const int *,
+                                         const int *, const int *, size_t, size_t)>("pdlaswp_");
+        if (next) return next(direc, rowcol, n, a, ia, ja, desca, k1, k2, ipiv, 1, 1);
+        pxerbla(g, "PDLASWP", 7);
+        return;
+    }
+    const bool rows = (*rowcol == 'R' || *rowcol == 'r');
+    const bool fwd = (*direc == 'F' || *direc == 'f');
+    const int lld = D(desca, LLD_);
+    try {
+        sla::Handle &h = default_handle();
+        if (rows) {
+            // Rows K1..K2 interchange with rows IPIV(K1..K2) of columns JA..JA+N-1
+            // (pdlaswp.f:163-185); the touched rows are 1..max(K2, max IPIV).
+            int hi = *k2;
+            for (int i = *k1; i <= *k2; i++) hi = std::max(hi, ipiv[i - 1]);
+            DevBuf dA((size_t)hi * *n * 8), dp((size_t)hi * 4);
+            double *acol = a + (size_t)(*ja - 1) * lld;
+            to_device(dA.as<double>(), hi, acol, lld, hi, *n);
+            SLA_HIP(hipMemcpy(dp.p, ipiv, sizeof(int) * *k2, hipMemcpyHostToDevice));
+            sla::laswp_rows_device(h, dA.as<double>(), hi, *n, *k1 - 1, *k2, dp.as<int>(), fwd);
+            to_host(acol, lld, dA.as<double>(), hi, hi, *n);
+        } else {
+            // Columns K1..K2 interchange with columns IPIV(K1..K2) over rows
+            // IA..IA+N-1 (pdlaswp.f:186-203).
+            int hi = *k2;
+            for (int i = *k1; i <= *k2; i++) hi = std::max(hi, ipiv[i - 1]);
+            DevBuf dA((size_t)*n * hi * 8), dp((size_t)hi * 4);
+            double *arow = a + (size_t)(*ia - 1);
+            to_device(dA.as<double>(), *n, arow, lld, *n, hi);
+            SLA_HIP(hipMemcpy(dp.p, ipiv, sizeof(int) * *k2, hipMemcpyHostToDevice));
+            sla::laswp_cols_device(h, dA.as<double>(), *n, *n, *k1 - 1, *k2, dp.as<int>(), fwd);
+            to_host(arow, lld, dA.as<double>(), *n, *n, hi);
+        }
+    } catch (const std::exception &e) {
+        fatal("PDLASWP", e);
+    }
+}
+
+// ============================================================================
+// Part 2: device-resident interface
+// ============================================================================
+const char *sla_last_error(void) { return g_last_error.c_str(); }
+
+#define SLA_NEED_HANDLE(h)                                   \
+    do {                                                     \
+        if (!(h)) {                                          \
+            g_last_error = "sla: null handle";               \
+            return SLA_ERR_ARG;                              \
+        }                                                    \
+    } while (0)
+
+#define SLA_TRY(body)                                  \
+    try {                                              \
+        body;                                          \
+        return SLA_OK;                                 \
+    } catch (const std::exception &e) {                \
+        g_last_error = e.what();                       \
+        return SLA_ERR_RUNTIME;                        \
+    }
+
+int sla_create(sla_handle_t *h)
+{
+    SLA_TRY(*h = reinterpret_cast<sla_handle_t>(new sla::Handle()))
+}
+
+int sla_destroy(sla_handle_t h)
+{
+    SLA_NEED_HANDLE(h);
+    SLA_TRY(delete reinterpret_cast<sla::Handle *>(h))
+}
+
+int sla_set_stream(sla_handle_t h, void *stream)
+{
+    SLA_NEED_HANDLE(h);
+    auto *H = reinterpret_cast<sla::Handle *>(h);
+    SLA_TRY(H->stream = stream ? static_cast<hipStream_t>(stream) : H->own_stream)
+}
+
+int sla_set_param(sla_handle_t h, const char *name, int value)
+{
+    auto *H = reinterpret_cast<sla::Handle *>(h);
+    const std::string k(name ? name : "");
+    static const char *known[] = {"nb", "leaf_w", "panel_wgs", "profile"};
+    if (std::find_if(std::begin(known), std::end(known), [&](const char *x) { return k == x; }) ==
+        std::end(known)) {
+        g_last_error = "sla_set_param: unknown parameter " + k;
+        return SLA_ERR_ARG;
+    }
+    SLA_NEED_HANDLE(h);
+    if (k == "nb" && value >= 1 && value <= 1024) H->nb = value;
+    else if (k == "leaf_w" && value >= 1 && value <= 32) H->leaf_w = value;
+    else if (k == "panel_wgs" && value >= 1 && value <= 256) H->panel_wgs = value;
+    else if (k == "profile") H->profile = value != 0;
+    else {
+        g_last_error = "sla_set_param: value out of range for " + k;
+        return SLA_ERR_ARG;
+    }
+    return SLA_OK;
+}
+
+int sla_get_param(sla_handle_t h, const char *name, int *value)
+{
+    SLA_NEED_HANDLE(h);
+    auto *H = reinterpret_cast<sla::Handle *>(h);
+    const std::string k(name ? name : "");
+    if (k == "nb") *value = H->nb;
+    else if (k == "leaf_w") *value = H->leaf_w;
+    else if (k == "panel_wgs") *value = H->panel_wgs;
+    else if (k == "profile") *value = H->profile ? 1 : 0;
+    else {
+        g_last_error = "sla_get_param: unknown parameter " + k;
+        return SLA_ERR_ARG;
+    }
+    return SLA_OK;
+}
+
+int sla_get_stats(sla_handle_t h, int cls, double *ms, double *work, long long *launches)
+{
+    SLA_NEED_HANDLE(h);
+    auto *H = reinterpret_cast<sla::Handle *>(h);
+    if (cls < 0 || cls >= sla::CAT_COUNT) {
+        g_last_error = "sla_get_stats: class out of range";
+        return SLA_ERR_ARG;
+    }
+    *ms = H->stats[cls].ms;
+    *work = H->stats[cls].work;
+    *launches = H->stats[cls].launches;
+    return SLA_OK;
+}
+
+int sla_reset_stats(sla_handle_t h)
+{
+    SLA_NEED_HANDLE(h);
+    SLA_TRY(reinterpret_cast<sla::Handle *>(h)->reset_stats())
+}
+
+int sla_dgetrf(sla_handle_t h, int m, int n, double *dA, int lda, int *dipiv, int *info)
+{
+    SLA_NEED_HANDLE(h);
+    if (m < 0 || n < 0 || lda < std::max(1, m)) {
+        g_last_error = "sla_dgetrf: illegal m, n or lda";
+        return SLA_ERR_ARG;
+    }
+    SLA_TRY(*info = sla::dgetrf_device(*reinterpret_cast<sla::Handle *>(h), m, n, dA, lda, dipiv))
+}
+
+int sla_dgetrs(sla_handle_t h, int n, int nrhs, const double *dA, int lda, const int *dipiv,
+               double *dB, int ldb)
+{
+    SLA_NEED_HANDLE(h);
+    if (n < 0 || nrhs < 0 || lda < std::max(1, n) || ldb < std::max(1, n)) {
+        g_last_error = "sla_dgetrs: illegal n, nrhs, lda or ldb";
+        return SLA_ERR_ARG;
+    }
+    SLA_TRY(sla::dgetrs_device(*reinterpret_cast<sla::Handle *>(h), n, nrhs, dA, lda, dipiv, dB, ldb))
+}
+
+int sla_dgemm_sub(sla_handle_t h, int m, int n, int k, const double *dA, int lda,
+                  const double *dB, int ldb, double *dC, int ldc)
+{
+    SLA_NEED_HANDLE(h);
+    if (m < 0 || n < 0 || k < 0 || lda < std::max(1, m) || ldb < std::max(1, k) ||
+        ldc < std::max(1, m)) {
+        g_last_error = "sla_dgemm_sub: illegal dimensions";
+        return SLA_ERR_ARG;
+    }
+    SLA_TRY({
+        auto &H = *reinterpret_cast<sla::Handle *>(h);
+        H.timing_begin(sla::CAT_GEMM);
+        sla::dgemm_nn_sub(m, n, k, dA, lda, dB, ldb, dC, ldc, H.stream);
+        H.timing_end(2.0 * m * n * k);
+        SLA_HIP(hipStreamSynchronize(H.stream));
+        H.timing_collect();
+    })
+}
+
+int sla_dgesv(sla_handle_t h, int n, int nrhs, double *dA, int lda, int *dipiv, double *dB,
+              int ldb, int *info)
+{
+    SLA_NEED_HANDLE(h);
+    if (n < 0 || nrhs < 0 || lda < std::max(1, n) || ldb < std::max(1, n)) {
+        g_last_error = "sla_dgesv: illegal n, nrhs, lda or ldb";
+        return SLA_ERR_ARG;
+    }
+    SLA_TRY({
+        auto &H = *reinterpret_cast<sla::Handle *>(h);
+        *info = sla::dgetrf_device(H, n, n, dA, lda, dipiv);
+        if (*info == 0) sla::dgetrs_device(H, n, nrhs, dA, lda, dipiv, dB, ldb);
+    })
+}
+
+}  // extern "C"

@@ -1,0 +1,215 @@
+// gemm_f64.hip -- the trailing (Schur-complement) update of the LU
+// factorization, A22 <- A22 - L21 * U12, as an FP64 MFMA GEMM for gfx950.
+//
+// This is the one local dgemm_ PDGEMM issues on a 1 x 1 grid
+// (SRC/pdgetrf.f:288-290 -> PBLAS/SRC/pdgemm_.c:476 ->
+//  PBLAS/SRC/PTOOLS/PB_CpgemmAB.c:333-352: dgemm_('N','N', M-J-JB+1,
+//  N-J-JB+1, JB, -1, L21, LLD, U12, LLD, +1, A22, LLD)), and it also serves
+// the updates inside the recursive panel and the block triangular solves.
+//
+// Tiling: a 256-thread workgroup (4 wave64s) owns a 128 x 128 tile of C and
+// walks K in steps of 16 through double-buffered LDS tiles.  Each wave owns a
+// 64 x 64 sub-tile as 4 x 4 accumulators of v_mfma_f64_16x16x4f64 (16 x 16
+// outputs, 4 doubles per lane).  Lane layout of that instruction:
+//   A operand : lane l holds A[l % 16][l / 16]         (16 rows x 4 k)
+//   B operand : lane l holds B[l / 16][l % 16]         (4 k x 16 cols)
+//   D         : lane l, element r holds D[4*(l/16)+r][l % 16]
+// so each lane's 4 results are 4 consecutive rows of one column: a 32-byte
+// contiguous piece of column-major C.
+#include "sla_common.h"
+#include "sla_kernels.h"
+
+namespace sla {
+
+namespace {
+
+constexpr int TM = 128, TN = 128, TK = 16;
+constexpr int BPAD = 1;  // odd pad on the k-major B tile staggers LDS banks
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+
+// Maps the linear workgroup id to a (tile_m, tile_n) pair so that the
+// workgroups one XCD receives (ids congruent mod 8 are dispatched to the same
+// XCD) sweep a compact band of tiles that share L21 rows and U12 columns.
+__device__ __forceinline__ void tile_coords(int tiles_m, int tiles_n, int &tm, int &tn)
+{
+    const int id = blockIdx.x;
+    const int total = tiles_m * tiles_n;
+    int lid = id;
+    if (total >= 64) {
+        // Round-robin dispatch sends id to XCD id % 8; give each XCD a
+        // contiguous range of the row-major tile order.
+        const int per = total / 8, rem = total % 8;
+        const int xcd = id % 8, k = id / 8;
+        const int start = xcd * per + (xcd < rem ? xcd : rem);
+        const int len = per + (xcd < rem ? 1 : 0);
+        lid = (k < len) ? start + k : id;  // every id maps to a distinct tile
+    }
+    // Column-band order: consecutive ids walk down M inside a band of 8 N
+    // tiles, so neighbours reuse the same U12 columns and L21 rows from L2.
+    const int band = 8;
+    const int bands_full = tiles_n / band;
+    const int in_full = bands_full * band * tiles_m;
+    if (lid < in_full) {
+        const int b = lid / (band * tiles_m), r = lid % (band * tiles_m);
+        tm = r % tiles_m;
+        tn = b * band + r / tiles_m;
+    } else {
+        const int r = lid - in_full, w = tiles_n - bands_full * band;
+        tm = r % tiles_m;
+        tn = bands_full * band + r / tiles_m;
+        (void)w;
+    }
+}
+
+__global__ __launch_bounds__(256, 1) void dgemm_nn_sub_kernel(
+    int m, int n, int k, const double *__restrict__ A, int lda,
+    const double *__restrict__ B, int ldb, double *__restrict__ C, int ldc)
+{
+    __shared__ double As[2][TK][TM];
+    __shared__ double Bs[2][TK][TN + BPAD];
+
+    const int tiles_m = ceil_div(m, TM), tiles_n = ceil_div(n, TN);
+    int tmi, tni;
+    tile_coords(tiles_m, tiles_n, tmi, tni);
+    const int bm = tmi * TM, bn = tni * TN;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+
+    // Global -> register mapping.  A tile (128 x 16): lanes run down a column
+    // of A (coalesced 512-byte rows per wave), 8 k per thread.
+    const int a_row = tid & 127, a_k0 = (tid >> 7) * 8;
+    // B tile (16 x 128): 16 lanes cover one column's 16 k (128 bytes).
+    const int b_k = tid & 15, b_n0 = tid >> 4;  // columns b_n0 + 16*i
+
+    const bool a_row_ok = (bm + a_row) < m;
+    const double *Ap = A + (size_t)(bm + a_row);
+    double ra[8], rb[8];
+
+    auto load_tile = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int kk = k0 + a_k0 + i;
+            ra[i] = (a_row_ok && kk < k) ? Ap[(size_t)kk * lda] : 0.0;
+        }
+        const int kk = k0 + b_k;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int nn = bn + b_n0 + 16 * i;
+            rb[i] = (nn < n && kk < k) ? B[kk + (size_t)nn * ldb] : 0.0;
+        }
+    };
+    auto store_tile = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) As[buf][a_k0 + i][a_row] = ra[i];
+#pragma unroll
+        for (int i = 0; i < 8; i++) Bs[buf][b_k][b_n0 + 16 * i] = rb[i];
+    };
+
+    d4_t acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+
+    const int nk = ceil_div(k, TK);
+    load_tile(0);
+    store_tile(0);
+    __syncthreads();
+
+    const int lr = lane & 15, lk = lane >> 4;
+    for (int t = 0; t < nk; t++) {
+        const int buf = t & 1;
+        if (t + 1 < nk) load_tile((t + 1) * TK);
+#pragma unroll
+        for (int s = 0; s < TK / 4; s++) {
+            double af[4], bf[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) af[i] = As[buf][s * 4 + lk][wm + i * 16 + lr];
+#pragma unroll
+            for (int j = 0; j < 4; j++) bf[j] = Bs[buf][s * 4 + lk][wn + j * 16 + lr];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+        }
+        if (t + 1 < nk) store_tile(buf ^ 1);
+        __syncthreads();
+    }
+
+    // Epilogue: C -= acc.  Lane owns rows 4*lk .. 4*lk+3 of column lr of each
+    // 16 x 16 block.
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int col = bn + wn + j * 16 + lr;
+        if (col >= n) continue;
+        double *Cc = C + (size_t)col * ldc;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int row0 = bm + wm + i * 16 + 4 * lk;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = row0 + r;
+                if (row < m) Cc[row] = Cc[row] - acc[i][j][r];
+            }
+        }
+    }
+}
+
+// C[m x n] -= A[m x k] * B[k x n] for n <= 16 (the right-hand sides of the
+// triangular solves).  HBM-bound: each thread streams rows of A once.
+template <int NMAX>
+__global__ __launch_bounds__(256) void dgemm_nn_sub_skinny_kernel(
+    int m, int n, int k, const double *__restrict__ A, int lda,
+    const double *__restrict__ B, int ldb, double *__restrict__ C, int ldc)
+{
+    extern __shared__ double Bsh[];  // k x n, column-major with ld k
+    for (int idx = threadIdx.x; idx < k * n; idx += blockDim.x) {
+        const int kk = idx % k, jj = idx / k;
+        Bsh[idx] = B[kk + (size_t)jj * ldb];
+    }
+    __syncthreads();
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= m) return;
+    double acc[NMAX];
+#pragma unroll
+    for (int j = 0; j < NMAX; j++) acc[j] = 0.0;
+    const double *Ar = A + row;
+    for (int kk = 0; kk < k; kk++) {
+        const double a = Ar[(size_t)kk * lda];
+#pragma unroll
+        for (int j = 0; j < NMAX; j++)
+            if (j < n) acc[j] = fma(a, Bsh[kk + j * k], acc[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < NMAX; j++)
+        if (j < n) C[row + (size_t)j * ldc] -= acc[j];
+}
+
+}  // namespace
+
+void dgemm_nn_sub(int m, int n, int k, const double *A, int lda, const double *B, int ldb,
+                  double *C, int ldc, hipStream_t s)
+{
+    if (m <= 0 || n <= 0 || k <= 0) return;
+    if (n <= 16 && (size_t)k * n * sizeof(double) <= 48 * 1024) {
+        const int grid = ceil_div(m, 256);
+        const size_t sh = (size_t)k * n * sizeof(double);
+        if (n <= 1)
+            hipLaunchKernelGGL(dgemm_nn_sub_skinny_kernel<1>, dim3(grid), dim3(256), sh, s, m, n, k, A, lda, B, ldb, C, ldc);
+        else if (n <= 4)
+            hipLaunchKernelGGL(dgemm_nn_sub_skinny_kernel<4>, dim3(grid), dim3(256), sh, s, m, n, k, A, lda, B, ldb, C, ldc);
+        else
+            hipLaunchKernelGGL(dgemm_nn_sub_skinny_kernel<16>, dim3(grid), dim3(256), sh, s, m, n, k, A, lda, B, ldb, C, ldc);
+        SLA_LAUNCH_CHECK();
+        return;
+    }
+    const int tiles = ceil_div(m, TM) * ceil_div(n, TN);
+    hipLaunchKernelGGL(dgemm_nn_sub_kernel, dim3(tiles), dim3(256), 0, s, m, n, k, A, lda, B, ldb,
+                       C, ldc);
+    SLA_LAUNCH_CHECK();
+}
+
+}  // namespace sla

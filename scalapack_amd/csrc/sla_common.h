@@ -1,0 +1,124 @@
+// sla_common.h -- shared definitions for the MI355X (gfx950) LU solve library.
+//
+// Layout conventions used by every kernel in this directory:
+//   * matrices are column-major with a leading dimension (the LLD of the
+//     ScaLAPACK descriptor on a 1 x 1 grid), element (i, j) at a[i + j*ld];
+//   * pivot vectors hold 1-based global row numbers, as PDGETRF returns them
+//     (SRC/pdgetrf.f:118-121);
+//   * every kernel runs on the stream it is given; nothing synchronises the
+//     device except the host entry points that must return data.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+namespace sla {
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+[[noreturn]] inline void hip_fail(hipError_t e, const char *what, const char *file, int line)
+{
+    char buf[512];
+    std::snprintf(buf, sizeof buf, "%s failed at %s:%d: %s", what, file, line,
+                  hipGetErrorString(e));
+    throw HipError(buf);
+}
+
+#define SLA_HIP(call)                                                          \
+    do {                                                                       \
+        hipError_t e_ = (call);                                                \
+        if (e_ != hipSuccess) ::sla::hip_fail(e_, #call, __FILE__, __LINE__);  \
+    } while (0)
+
+#define SLA_LAUNCH_CHECK() SLA_HIP(hipGetLastError())
+
+__host__ __device__ inline int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+// ---------------------------------------------------------------------------
+// Device-side helpers
+// ---------------------------------------------------------------------------
+
+// Loads and stores of data that another workgroup of the same launch reads
+// or writes.  Agent-scope relaxed atomics keep them out of the per-XCD
+// non-coherent cache path; ordering comes from grid_sync's fences.
+template <typename T>
+__device__ __forceinline__ T coherent_load(const T *p)
+{
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "word-sized only");
+    if constexpr (sizeof(T) == 8) {
+        uint64_t v = __hip_atomic_load(reinterpret_cast<const uint64_t *>(p),
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return __builtin_bit_cast(T, v);
+    } else {
+        uint32_t v = __hip_atomic_load(reinterpret_cast<const uint32_t *>(p),
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return __builtin_bit_cast(T, v);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void coherent_store(T *p, T v)
+{
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "word-sized only");
+    if constexpr (sizeof(T) == 8) {
+        __hip_atomic_store(reinterpret_cast<uint64_t *>(p), __builtin_bit_cast(uint64_t, v),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        __hip_atomic_store(reinterpret_cast<uint32_t *>(p), __builtin_bit_cast(uint32_t, v),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// Device-wide barrier for a launch whose workgroups are all resident.
+// `ctr` counts arrivals monotonically; the caller passes the count that marks
+// this barrier (base + k * gridDim.x).  The wait is bounded: after
+// `spin_limit` wall-clock ticks (100 MHz on gfx9) a workgroup raises *abort
+// and every waiter leaves, so a launch whose workgroups cannot all be resident
+// finishes with an error flag instead of hanging the device.  Returns false
+// once the launch has been aborted.
+__device__ __forceinline__ bool grid_sync(unsigned *ctr, unsigned target, int *abort_flag,
+                                          uint64_t spin_limit)
+{
+    __shared__ int s_ok;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");   // publish this workgroup's stores
+        __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t t0 = wall_clock64();
+        while (true) {
+            unsigned v = __hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((int)(v - target) >= 0) break;
+            if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                ok = 0;
+                break;
+            }
+            if (wall_clock64() - t0 > spin_limit) {
+                __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // see the other workgroups' stores
+        s_ok = ok;
+    }
+    __syncthreads();
+    return s_ok != 0;
+}
+
+// (|v|, row) ordering of the pivot search: larger magnitude wins, the lower
+// row wins a tie -- IDAMAX's "first index of the maximum" on the current row
+// order (PBLAS/SRC/pdamax_.c:421-425).
+__device__ __forceinline__ bool pivot_better(double v1, int r1, double v2, int r2)
+{
+    return v1 > v2 || (v1 == v2 && r1 < r2);
+}
+
+}  // namespace sla

@@ -1,0 +1,84 @@
+// sla_kernels.h -- host-side launchers of the HIP kernels (one stream each).
+#pragma once
+
+#include "sla_common.h"
+
+namespace sla {
+
+// ---- Schur-complement update: C[m x n] -= A[m x k] * B[k x n] (FP64) -----
+void dgemm_nn_sub(int m, int n, int k, const double *A, int lda, const double *B, int ldb,
+                  double *C, int ldc, hipStream_t s);
+
+// ---- panel leaf (PDGETF2 on <= 32 columns) --------------------------------
+template <typename T>
+struct PanelSlots {
+    double *cand_v;  // [2][gmax] candidate |pivot|
+    int *cand_r;     // [2][gmax] candidate row (leaf-relative)
+    T *rows;         // [2][gmax][32] candidate rows
+    T *rowj;         // [2][32] row j
+    int gmax;
+};
+
+template <typename T>
+struct PanelWork {
+    PanelSlots<T> slots;
+    unsigned *ctr;       // monotonic arrival counter of the device-wide barrier
+    int *abort_flag;     // set by a barrier that waited past spin_limit
+    int *zero_col;       // min over zero pivots of the 1-based column (INT_MAX: none)
+    uint64_t spin_limit; // wall-clock ticks a barrier may wait
+    int target_wgs;      // preferred number of workgroups per leaf launch
+};
+
+// Factors the m x wa leaf at A (global rows row0.., columns col0.. relative to
+// the factored matrix) in place; writes ipiv[row0 .. row0+wa-1] as 1-based
+// global rows.  Returns the number of barrier arrivals the launch adds to
+// *ctr (the caller's next `base`).
+template <typename T>
+unsigned panel_leaf(T *A, int lda, int m, int wa, int row0, int col0, int *ipiv,
+                    const PanelWork<T> &pw, unsigned base, hipStream_t s);
+
+// ---- row interchanges (PDLASWP) --------------------------------------------
+// Net effect of the forward interchanges ipiv[k1 .. k2-1] (0-based global row
+// indices of the pivoted rows, ipiv 1-based global) as a list of
+// (dst, src) global rows: after the interchanges row dst holds what row src
+// held before.  At most 2*(k2-k1) pairs; k2-k1 <= 512.
+void laswp_pairs(const int *ipiv, int k1, int k2, int *pairs, int *count, hipStream_t s);
+// Applies a pair list to columns [0, ncols) of A whose row 0 is global row
+// `roff`; `inverse` applies the inverse permutation (the same interchanges
+// replayed backward).
+template <typename T>
+void laswp_apply(T *A, int lda, int roff, int ncols, const int *pairs, const int *count,
+                 int max_pairs, hipStream_t s, bool inverse = false);
+// Column interchanges i <-> ipiv[i]-1, i = k1 .. k2-1, over rows [0, nrows).
+void laswp_cols(double *A, int lda, int nrows, int k1, int k2, const int *ipiv, bool forward,
+                hipStream_t s);
+
+// ---- triangular solves -----------------------------------------------------
+// B[m x n] <- L^{-1} B, L unit lower triangular m x m, m <= 256
+// (PDTRSM 'L','L','N','U').
+template <typename T>
+void trsm_llnu(int m, int n, const T *L, int ldl, T *B, int ldb, hipStream_t s);
+// One launch per triangle for n x nrhs right-hand sides (nrhs <= 16):
+// forward = X <- L^{-1} X (unit lower), else X <- U^{-1} X (upper, non-unit).
+struct TrsvWork {
+    int *ticket;         // workgroup ticket counter (reset before each launch)
+    int *flags;          // per 64-row block: == epoch once that block is solved
+    int epoch;           // host-side counter, one value per launch
+    int max_blocks;      // capacity of flags
+    int *abort_flag;
+    uint64_t spin_limit;
+};
+template <typename T>
+void trsv_persistent(bool forward, int n, int nrhs, const T *A, int lda, T *X, int ldx,
+                     TrsvWork &tw, hipStream_t s);
+
+// ---- small helpers ---------------------------------------------------------
+void fill_int(int *p, int n, int v, hipStream_t s);
+void fill_u32(unsigned *p, int n, unsigned v, hipStream_t s);
+void iota_int(int *p, int n, int base, hipStream_t s);
+// B[i, :] = Bsrc[perm[i], :] (row gather), n x nrhs
+template <typename T>
+void gather_rows(const T *src, int lds, T *dst, int ldd, const int *perm, int n, int nrhs,
+                 hipStream_t s);
+
+}  // namespace sla

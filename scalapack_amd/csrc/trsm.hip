@@ -1,0 +1,238 @@
+// trsm.hip -- triangular solves of the LU path.
+//
+// trsm_llnu: the block row of U in PDGETRF, U12 <- L11^{-1} A12
+//   (SRC/pdgetrf.f:280-281, PDTRSM('L','L','N','U'); on a 1 x 1 grid
+//   PBLAS/SRC/PTOOLS/PB_CptrsmAB.c:359-412 splits it into 32-row dtrsm_ +
+//   dgemm_ steps).  Each workgroup owns a strip of 16 columns of A12, keeps
+//   it in LDS, and walks L11 in 32-column blocks staged through LDS: solve the
+//   32 x 32 diagonal block, then update the rows below with one fma chain.
+//
+// trsv_persistent: the two solves of PDGETRS (SRC/pdgetrs.f:260-266) for a
+//   few right-hand sides.  Those are latency-bound, so one launch covers the
+//   whole triangle: block row b (64 rows) is owned by the workgroup that
+//   takes ticket b; it accumulates -L[b, c] X[c] for every finished block c
+//   (a device-wide flag per block, bounded wait), then solves its 64 x 64
+//   diagonal block in registers of one wave (lane = row, values broadcast by
+//   lane reads) and publishes X[b].  Tickets are taken in start order, so a
+//   workgroup only ever waits on workgroups that are already running.
+#include "sla_common.h"
+#include "sla_kernels.h"
+
+namespace sla {
+
+namespace {
+
+constexpr int TS_CW = 16, TS_KB = 32, TS_MAXM = 256;
+
+template <typename T>
+__global__ __launch_bounds__(256) void trsm_llnu_strip_kernel(int m, int n, const T *__restrict__ L,
+                                                              int ldl, T *__restrict__ B, int ldb)
+{
+    __shared__ T Bs[TS_MAXM][TS_CW + 1];
+    __shared__ T Ls[TS_MAXM][TS_KB + 1];
+    const int c0 = blockIdx.x * TS_CW, tid = threadIdx.x;
+    const int c = tid % TS_CW, rg = tid / TS_CW;  // 16 row groups
+    for (int idx = tid; idx < m * TS_CW; idx += 256) {
+        const int r = idx % m, cc = idx / m;
+        Bs[r][cc] = (c0 + cc < n) ? B[r + (size_t)(c0 + cc) * ldb] : T(0);
+    }
+    for (int kb = 0; kb < m; kb += TS_KB) {
+        const int kbb = min(TS_KB, m - kb), rows = m - kb;
+        __syncthreads();  // previous block's readers of Ls are done
+        for (int idx = tid; idx < rows * kbb; idx += 256) {
+            const int r = idx % rows, k = idx / rows;
+            Ls[r][k] = L[(size_t)(kb + r) + (size_t)(kb + k) * ldl];
+        }
+        __syncthreads();
+        for (int i = 0; i < kbb; i++) {
+            const T x = Bs[kb + i][c];
+            for (int r = i + 1 + rg; r < kbb; r += 16) Bs[kb + r][c] = fma(-Ls[r][i], x, Bs[kb + r][c]);
+            __syncthreads();
+        }
+        for (int r = kbb + rg; r < rows; r += 16) {
+            T acc = Bs[kb + r][c];
+            for (int k = 0; k < kbb; k++) acc = fma(-Ls[r][k], Bs[kb + k][c], acc);
+            Bs[kb + r][c] = acc;
+        }
+    }
+    __syncthreads();
+    for (int idx = tid; idx < m * TS_CW; idx += 256) {
+        const int r = idx % m, cc = idx / m;
+        if (c0 + cc < n) B[r + (size_t)(c0 + cc) * ldb] = Bs[r][cc];
+    }
+}
+
+// ---------------------------------------------------------------------------
+constexpr int TV_BS = 64, TV_NR = 16;
+
+template <typename T>
+__device__ __forceinline__ T lane_read(T v, int lane)
+{
+    if constexpr (sizeof(T) == 8) {
+        const uint64_t u = __builtin_bit_cast(uint64_t, v);
+        const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, lane);
+        const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), lane);
+        return __builtin_bit_cast(T, ((uint64_t)hi << 32) | lo);
+    } else {
+        return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(uint32_t, v), lane));
+    }
+}
+
+// FORWARD: X = L^{-1} X with L unit lower.  Otherwise X = U^{-1} X with U
+// upper, non-unit (division by the diagonal, as dtrsm_ does).  X is n x nrhs
+// (nrhs <= 16) with leading dimension ldx; the matrix is n x n.
+template <typename T, bool FORWARD>
+__global__ __launch_bounds__(256) void trsv_persistent_kernel(
+    int n, int nrhs, const T *__restrict__ A, int lda, T *X, int ldx, int *ticket, int *flags,
+    int epoch, int *abort_flag, uint64_t spin_limit)
+{
+    __shared__ int s_b, s_abort;
+    __shared__ T Xs[TV_BS][TV_NR];
+    __shared__ T Part[4][TV_BS][TV_NR + 1];
+    __shared__ T Ds[TV_BS][TV_BS + 1];  // diagonal block, row-major
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nblk = ceil_div(n, TV_BS);
+
+    if (tid == 0) s_b = atomicAdd(ticket, 1);
+    __syncthreads();
+    const int t = s_b;
+    if (t >= nblk) return;
+    const int b = FORWARD ? t : nblk - 1 - t;
+    const int rb = b * TV_BS, rows = min(TV_BS, n - rb);
+
+    // Diagonal block staged in LDS before any wait (it does not depend on
+    // other blocks).
+    for (int idx = tid; idx < TV_BS * TV_BS; idx += 256) {
+        const int r = idx % TV_BS, k = idx / TV_BS;
+        Ds[r][k] = (r < rows && k < rows) ? A[(size_t)(rb + r) + (size_t)(rb + k) * lda] : T(0);
+    }
+
+    T part[TV_NR];
+#pragma unroll
+    for (int j = 0; j < TV_NR; j++) part[j] = T(0);
+
+    // Off-diagonal contributions, in dependency order.
+    for (int s = 0; s < t; s++) {
+        const int c = FORWARD ? s : nblk - 1 - s;
+        const int cb = c * TV_BS, crows = min(TV_BS, n - cb);
+        if (tid == 0) {
+            int ab = 0;
+            const uint64_t t0 = wall_clock64();
+            while (__hip_atomic_load(&flags[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+                if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    ab = 1;
+                    break;
+                }
+                if (wall_clock64() - t0 > spin_limit) {
+                    __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ab = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            s_abort = ab;
+        }
+        __syncthreads();
+        if (s_abort) return;  // the whole workgroup leaves together
+        for (int idx = tid; idx < TV_BS * TV_NR; idx += 256) {
+            const int r = idx % TV_BS, j = idx / TV_BS;
+            Xs[r][j] = (r < crows && j < nrhs) ? coherent_load(&X[(size_t)(cb + r) + (size_t)j * ldx]) : T(0);
+        }
+        __syncthreads();
+        // wave w covers k in [16w, 16w+16) of the 64 x 64 block A[rb.., cb..]
+        if (lane < rows) {
+            const T *Ar = A + (size_t)(rb + lane) + (size_t)cb * lda;
+#pragma unroll 4
+            for (int kk = 0; kk < 16; kk++) {
+                const int k = wave * 16 + kk;
+                if (k < crows) {
+                    const T a = Ar[(size_t)k * lda];
+#pragma unroll
+                    for (int j = 0; j < TV_NR; j++) part[j] = fma(a, Xs[k][j], part[j]);
+                }
+            }
+        }
+        __syncthreads();  // Xs is reused by the next block
+    }
+#pragma unroll
+    for (int j = 0; j < TV_NR; j++) Part[wave][lane][j] = part[j];
+    __syncthreads();
+
+    if (wave == 0) {
+        T x[TV_NR];
+#pragma unroll
+        for (int j = 0; j < TV_NR; j++) {
+            const T rhs = (lane < rows && j < nrhs) ? X[(size_t)(rb + lane) + (size_t)j * ldx] : T(0);
+            const T s = ((Part[0][lane][j] + Part[1][lane][j]) + Part[2][lane][j]) + Part[3][lane][j];
+            x[j] = rhs - s;
+        }
+        if (FORWARD) {
+            for (int i = 0; i < rows; i++) {
+                const T l = Ds[lane][i];
+#pragma unroll
+                for (int j = 0; j < TV_NR; j++) {
+                    const T xi = lane_read(x[j], i);
+                    if (lane > i) x[j] = fma(-l, xi, x[j]);
+                }
+            }
+        } else {
+            for (int i = rows - 1; i >= 0; i--) {
+                const T u = Ds[lane][i];
+#pragma unroll
+                for (int j = 0; j < TV_NR; j++) {
+                    if (lane == i) x[j] = x[j] / u;
+                    const T xi = lane_read(x[j], i);
+                    if (lane < i) x[j] = fma(-u, xi, x[j]);
+                }
+            }
+        }
+        if (lane < rows) {
+#pragma unroll
+            for (int j = 0; j < TV_NR; j++)
+                if (j < nrhs) coherent_store(&X[(size_t)(rb + lane) + (size_t)j * ldx], x[j]);
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_store(&flags[b], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+}  // namespace
+
+template <typename T>
+void trsm_llnu(int m, int n, const T *L, int ldl, T *B, int ldb, hipStream_t s)
+{
+    if (m <= 0 || n <= 0) return;
+    if (m > TS_MAXM) throw std::logic_error("trsm_llnu: at most 256 rows per call");
+    hipLaunchKernelGGL(trsm_llnu_strip_kernel<T>, dim3(ceil_div(n, TS_CW)), dim3(256), 0, s, m, n,
+                       L, ldl, B, ldb);
+    SLA_LAUNCH_CHECK();
+}
+template void trsm_llnu<double>(int, int, const double *, int, double *, int, hipStream_t);
+template void trsm_llnu<float>(int, int, const float *, int, float *, int, hipStream_t);
+
+template <typename T>
+void trsv_persistent(bool forward, int n, int nrhs, const T *A, int lda, T *X, int ldx,
+                     TrsvWork &tw, hipStream_t s)
+{
+    if (n <= 0 || nrhs <= 0) return;
+    if (nrhs > TV_NR) throw std::logic_error("trsv_persistent: at most 16 right-hand sides per call");
+    const int nblk = ceil_div(n, TV_BS);
+    if (nblk > tw.max_blocks) throw std::logic_error("trsv_persistent: flag array too small");
+    fill_int(tw.ticket, 1, 0, s);
+    const int epoch = ++tw.epoch;
+    if (forward)
+        hipLaunchKernelGGL((trsv_persistent_kernel<T, true>), dim3(nblk), dim3(256), 0, s, n, nrhs,
+                           A, lda, X, ldx, tw.ticket, tw.flags, epoch, tw.abort_flag, tw.spin_limit);
+    else
+        hipLaunchKernelGGL((trsv_persistent_kernel<T, false>), dim3(nblk), dim3(256), 0, s, n, nrhs,
+                           A, lda, X, ldx, tw.ticket, tw.flags, epoch, tw.abort_flag, tw.spin_limit);
+    SLA_LAUNCH_CHECK();
+}
+template void trsv_persistent<double>(bool, int, int, const double *, int, double *, int, TrsvWork &, hipStream_t);
+template void trsv_persistent<float>(bool, int, int, const float *, int, float *, int, TrsvWork &, hipStream_t);
+
+}  // namespace sla

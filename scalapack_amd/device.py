@@ -1,0 +1,98 @@
+"""Device-resident interface (matrices already in HBM).
+
+Handle wraps the sla_* C-ABI of include/scalapack_amd.h.  Tensors are torch
+CUDA (HIP) tensors used as plain device memory: a column-major m x n matrix
+with leading dimension lda is a tensor whose storage holds lda * n values,
+e.g. the transpose view of a contiguous (n, lda) tensor.
+"""
+from __future__ import annotations
+
+import ctypes
+
+from ._lib import last_error, load
+
+STAT_CLASSES = ("panel", "laswp", "trsm", "gemm", "trsv")
+
+
+class SlaError(RuntimeError):
+    pass
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise SlaError(last_error())
+
+
+def _ptr(t) -> ctypes.c_void_p:
+    if isinstance(t, int):
+        return ctypes.c_void_p(t)
+    if not t.is_cuda:
+        raise TypeError("device interface: tensor must live on the GPU")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Handle:
+    """One library handle (streams, workspace, tuning) on the current device."""
+
+    def __init__(self, **params):
+        self._lib = load()
+        h = ctypes.c_void_p()
+        _check(self._lib.sla_create(ctypes.byref(h)))
+        self._h = h
+        for k, v in params.items():
+            self.set(k, v)
+
+    def close(self) -> None:
+        if self._h:
+            self._lib.sla_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set(self, name: str, value: int) -> None:
+        _check(self._lib.sla_set_param(self._h, name.encode(), int(value)))
+
+    def get(self, name: str) -> int:
+        v = ctypes.c_int()
+        _check(self._lib.sla_get_param(self._h, name.encode(), ctypes.byref(v)))
+        return v.value
+
+    def set_stream(self, stream) -> None:
+        """stream: a torch.cuda.Stream, a raw hipStream_t int, or None."""
+        raw = None if stream is None else (stream if isinstance(stream, int) else stream.cuda_stream)
+        _check(self._lib.sla_set_stream(self._h, ctypes.c_void_p(raw)))
+
+    def stats(self) -> dict[str, dict[str, float]]:
+        out = {}
+        for i, name in enumerate(STAT_CLASSES):
+            ms, work, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_longlong()
+            _check(self._lib.sla_get_stats(self._h, i, ctypes.byref(ms), ctypes.byref(work),
+                                           ctypes.byref(n)))
+            out[name] = {"ms": ms.value, "work": work.value, "launches": n.value}
+        return out
+
+    def reset_stats(self) -> None:
+        _check(self._lib.sla_reset_stats(self._h))
+
+    # ------------------------------------------------------------------ FP64
+    def dgetrf(self, m: int, n: int, a, lda: int, ipiv) -> int:
+        info = ctypes.c_int()
+        _check(self._lib.sla_dgetrf(self._h, m, n, _ptr(a), lda, _ptr(ipiv), ctypes.byref(info)))
+        return info.value
+
+    def dgetrs(self, n: int, nrhs: int, a, lda: int, ipiv, b, ldb: int) -> None:
+        _check(self._lib.sla_dgetrs(self._h, n, nrhs, _ptr(a), lda, _ptr(ipiv), _ptr(b), ldb))
+
+    def dgemm_sub(self, m: int, n: int, k: int, a, lda: int, b, ldb: int, c, ldc: int) -> None:
+        """C -= A B (the trailing-update kernel alone)."""
+        _check(self._lib.sla_dgemm_sub(self._h, m, n, k, _ptr(a), lda, _ptr(b), ldb, _ptr(c), ldc))
+
+    def dgesv(self, n: int, nrhs: int, a, lda: int, ipiv, b, ldb: int) -> int:
+        info = ctypes.c_int()
+        _check(self._lib.sla_dgesv(self._h, n, nrhs, _ptr(a), lda, _ptr(ipiv), _ptr(b), ldb,
+                                   ctypes.byref(info)))
+        return info.value

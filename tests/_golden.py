@@ -1,0 +1,53 @@
+"""Readers of the golden fixtures written by tests/golden/gen_golden.py from
+the reference's own PDGETRF/PDGETRS (see that script for the cases)."""
+from __future__ import annotations
+
+import json
+from fractions import Fraction
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent / "golden"
+
+
+def lu_dat() -> dict:
+    z = np.load(HERE / "lu_dat_cases.npz")
+    return {k: z[k] for k in z.files}
+
+
+def lu_dat_cases():
+    """(M, N, NB, NRHS, P, Q) of every stored LU.dat case."""
+    out = set()
+    for k in lu_dat():
+        p = k.split("_")
+        out.add((int(p[0][1:]), int(p[1][1:]), int(p[2][2:]), int(p[3][4:]), int(p[4][1:]),
+                 int(p[5][1:])))
+    return sorted(out)
+
+
+def lu_dat_key(M, N, NB, NRHS, P, Q) -> str:
+    return f"M{M}_N{N}_NB{NB}_NRHS{NRHS}_P{P}_Q{Q}"
+
+
+def c1() -> dict:
+    z = np.load(HERE / "c1_n1024_nb64.npz")
+    return {k: z[k] for k in z.files}
+
+
+def c2() -> dict | None:
+    p = HERE / "c2_n16384_nb256.npz"
+    if not p.exists():
+        return None
+    z = np.load(p)
+    return {k: z[k] for k in z.files}
+
+
+def example_6x6():
+    z = np.load(HERE / "example_6x6.npz")
+    x = [Fraction(int(n), int(d)) for n, d in zip(z["x_num"], z["x_den"])]
+    return np.asfortranarray(z["a"]), np.asfortranarray(z["b"]), x
+
+
+def argcheck_cases() -> list[dict]:
+    return json.loads((HERE / "argcheck_cases.json").read_text())

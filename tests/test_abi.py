@@ -1,0 +1,130 @@
+"""The drop-in boundary without a GPU: the C-ABI library loads, exports every
+symbol the public headers declare, its BLACS 1 x 1 / TOOLS helpers follow the
+reference's closed forms, and every illegal-argument INFO code equals the one
+the reference itself returns (tests/golden/argcheck_cases.json, written by
+running the reference's PDGETRF / PDGETRS / PDGESV)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import scalapack_amd
+from scalapack_amd import api
+from scalapack_amd._lib import declared_symbols
+
+
+def test_library_loads_and_exports_declared_symbols():
+    lib = scalapack_amd.load()
+    shim = scalapack_amd._lib.shim()
+    decl = declared_symbols()
+    assert len(decl["scalapack_amd.h"]) >= 15
+    for name in decl["scalapack_amd.h"]:
+        assert hasattr(lib, name), name
+    for name in decl["sla_blacs1x1.h"]:
+        assert hasattr(shim, name), name
+
+
+def test_reference_fortran_entry_points_are_declared():
+    decl = declared_symbols()["scalapack_amd.h"]
+    for name in ("pdgesv_", "pdgetrf_", "pdgetrs_", "pdgetf2_", "pdlaswp_"):
+        assert name in decl
+
+
+def test_blacs_1x1_context():
+    assert api.blacs_pinfo() == (0, 1)
+    ctx = api.blacs_gridinit(1, 1)
+    assert api.blacs_gridinfo(ctx) == (1, 1, 0, 0)
+    api.blacs_gridexit(ctx)
+    assert api.blacs_gridinfo(ctx) == (-1, -1, -1, -1)
+    bad = api.blacs_gridinit(2, 2)   # needs 4 processes
+    assert api.blacs_gridinfo(bad)[0] == -1
+
+
+def numroc_closed(n, nb, iproc, isrc, nprocs):
+    # count the entries of 0..n-1 owned by iproc, by brute force
+    return sum(1 for g in range(n) if (isrc + g // nb) % nprocs == iproc)
+
+
+@pytest.mark.parametrize("n,nb,np_", [(10, 3, 1), (10, 3, 2), (17, 4, 3), (1024, 64, 2), (5, 8, 4)])
+def test_numroc_matches_enumeration(n, nb, np_):
+    for p in range(np_):
+        for src in range(np_):
+            assert api.numroc(n, nb, p, src, np_) == numroc_closed(n, nb, p, src, np_)
+
+
+def test_descinit():
+    ctx = api.blacs_gridinit(1, 1)
+    d, info = api.descinit(1024, 1024, 64, 64, 0, 0, ctx, 1024)
+    assert info == 0
+    assert d.tolist() == [1, ctx, 1024, 1024, 64, 64, 0, 0, 1024]
+    d, info = api.descinit(100, 5, 8, 8, 0, 0, ctx, 50)   # LLD < M
+    assert info == -9
+
+
+def _run_argcheck(case, ctx):
+    def fix(d):
+        d = np.array(d, np.int32)
+        d[1] = ctx + 77 if d[1] == -999 else ctx
+        return d
+    a = np.asfortranarray(np.eye(8))
+    b = np.asfortranarray(np.zeros((8, 2)))
+    ipiv = np.arange(1, 33, dtype=np.int32)
+    r = case["routine"]
+    if r == "pdgetrf":
+        m, n, ia, ja = case["args"]
+        return api.pdgetrf(m, n, a, ia, ja, fix(case["desca"]), ipiv)
+    n, nrhs, ia, ja = case["args"]
+    ib, jb = case["bargs"]
+    if r == "pdgetrs":
+        return api.pdgetrs(case["trans"], n, nrhs, a, ia, ja, fix(case["desca"]), ipiv, b, ib,
+                           jb, fix(case["descb"]))
+    return api.pdgesv(n, nrhs, a, ia, ja, fix(case["desca"]), ipiv, b, ib, jb,
+                      fix(case["descb"]))
+
+
+def _cpu_safe(case):
+    """Cases whose INFO is decided before any device work."""
+    if case["info"] < 0:
+        return True
+    args = case["args"]
+    if case["routine"] == "pdgetrf":
+        return args[0] == 0 or args[1] == 0 or case["desca"][2] == 1
+    return args[0] == 0
+
+
+def test_illegal_argument_info_matches_reference(golden, capfd):
+    ctx = api.blacs_gridinit(1, 1)
+    checked = 0
+    for case in golden.argcheck_cases():
+        if not _cpu_safe(case):
+            continue
+        got = _run_argcheck(case, ctx)
+        assert got == case["info"], case
+        checked += 1
+    assert checked >= 35
+    out = capfd.readouterr().out
+    assert "On entry to PDGETRF parameter number  606 had an illegal value" in out
+
+
+def test_pdgetrs_transpose_is_reported():
+    ctx = api.blacs_gridinit(1, 1)
+    d, _ = api.descinit(4, 4, 2, 2, 0, 0, ctx, 4)
+    db, _ = api.descinit(4, 1, 2, 2, 0, 0, ctx, 4)
+    a = np.asfortranarray(np.eye(4))
+    b = np.asfortranarray(np.ones((4, 1)))
+    assert api.pdgetrs("T", 4, 1, a, 1, 1, d, np.arange(1, 5, dtype=np.int32), b, 1, 1, db) == -1
+
+
+def test_python_mirror_rejects_c_ordered_arrays():
+    ctx = api.blacs_gridinit(1, 1)
+    d, _ = api.descinit(4, 4, 2, 2, 0, 0, ctx, 4)
+    with pytest.raises(TypeError):
+        api.pdgetrf(4, 4, np.eye(4)[:, ::-1].copy(order="C"), 1, 1, d, np.zeros(6, np.int32))
+
+
+def test_device_handle_params_need_no_gpu_for_validation():
+    lib = scalapack_amd.load()
+    # an unknown parameter name is rejected by the C-ABI before any device work
+    rc = lib.sla_set_param(ctypes.c_void_p(0), b"no_such_knob", 1)
+    assert rc != 0
+    assert "unknown parameter" in scalapack_amd._lib.last_error()

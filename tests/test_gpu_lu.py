@@ -1,0 +1,266 @@
+"""GPU parity of the FP64 LU path, through the C-ABI.
+
+Against the reference's golden fixtures (IPIV bit-exact, X and residuals to
+the LU tester's thresholds) and against the oracle on the same seeded inputs.
+Tolerances: the solution X within 1e-10 relative of the reference's (FP64
+LU of well-conditioned PDMATGEN matrices); residuals as the reference's
+tester: SRESID <= 1.0 (TESTING/traditional/LU.dat:17) and < 16 at the
+benchmark sizes; FRESID < 1.0.  Row interchanges are data movement and are
+compared bit for bit.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _note(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def dev():
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    from scalapack_amd.device import Handle
+    h = Handle()
+    yield h
+    h.close()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from scalapack_amd import api
+    return api.blacs_gridinit(1, 1)
+
+
+def to_dev(a: np.ndarray):
+    """Column-major numpy matrix -> device tensor with the same layout
+    (storage = columns; lda = rows)."""
+    return torch.from_numpy(np.ascontiguousarray(np.asfortranarray(a).T)).cuda()
+
+
+def from_dev(t) -> np.ndarray:
+    return np.asfortranarray(t.cpu().numpy().T)
+
+
+# ------------------------------------------------------------------ kernels
+@pytest.mark.parametrize("m,n,k", [(128, 128, 16), (300, 200, 77), (1000, 1000, 256),
+                                   (4096, 257, 128), (77, 5, 64), (1500, 16, 256), (33, 1, 300)])
+def test_gemm_sub_parity(dev, m, n, k):
+    _note(f"gemm m={m} n={n} k={k}")
+    rng = np.random.default_rng(m * 7 + n * 3 + k)
+    A = np.asfortranarray(rng.uniform(-1, 1, (m, k)))
+    B = np.asfortranarray(rng.uniform(-1, 1, (k, n)))
+    C = np.asfortranarray(rng.uniform(-1, 1, (m, n)))
+    dA, dB, dC = to_dev(A), to_dev(B), to_dev(C)
+    dev.dgemm_sub(m, n, k, dA, m, dB, k, dC, m)
+    torch.cuda.synchronize()
+    got = from_dev(dC)
+    ref = C - A @ B
+    bound = 2.0 * k * 2.0 ** -52 * (np.abs(A) @ np.abs(B) + np.abs(C))
+    assert np.all(np.abs(got - ref) <= bound + 1e-300)
+
+
+# ------------------------------------------------------------ host drop-in
+def test_c1_pdgesv_matches_reference(oracle, golden, ctx):
+    from scalapack_amd import api
+    _note("C1 pdgesv N=1024 NB=64")
+    g = golden.c1()
+    n, nb = 1024, 64
+    a = oracle.matgen(n, n, 100)
+    a0 = a.copy(order="F")
+    b = np.asfortranarray(np.array([[sum(a0[i, :].tolist())] for i in range(n)]))
+    b0 = b.copy(order="F")
+    desca, _ = api.descinit(n, n, nb, nb, 0, 0, ctx, n)
+    descb, _ = api.descinit(n, 1, nb, nb, 0, 0, ctx, n)
+    ipiv = np.zeros(n + nb, np.int32)
+    info = api.pdgesv(n, 1, a, 1, 1, desca, ipiv, b, 1, 1, descb)
+    assert info == 0
+    assert np.array_equal(ipiv[:n], g["P1_Q1_ipiv"])
+    assert oracle.sresid(a0, b, b0) <= 1.0
+    assert np.allclose(b, g["P1_Q1_x"], rtol=1e-10, atol=1e-10)
+    assert oracle.fresid(a0, a, ipiv[:n]) < 1.0
+
+
+def test_lu_dat_cases_match_reference(oracle, golden, ctx):
+    from scalapack_amd import api
+    g = golden.lu_dat()
+    seen = set()
+    for (M, N, NB, NRHS, P, Q) in golden.lu_dat_cases():
+        if (P, Q) != (1, 1):
+            continue
+        key = golden.lu_dat_key(M, N, NB, NRHS, P, Q)
+        _note(f"LU.dat {key}")
+        a = oracle.matgen(M, N, 100)
+        a0 = a.copy(order="F")
+        desca, _ = api.descinit(M, N, NB, NB, 0, 0, ctx, M)
+        ipiv = np.zeros(M + NB, np.int32)
+        info = api.pdgetrf(M, N, a, 1, 1, desca, ipiv)
+        mn = min(M, N)
+        assert info == g[key + "_info"][0], key
+        assert np.array_equal(ipiv[:mn], g[key + "_ipiv"]), key
+        assert oracle.fresid(a0, a, ipiv[:mn]) < 1.0, key
+        if M == N:
+            b = oracle.matgen(N, NRHS, 200)
+            b0 = b.copy(order="F")
+            descb, _ = api.descinit(N, NRHS, NB, NB, 0, 0, ctx, N)
+            assert api.pdgetrs("N", N, NRHS, a, 1, 1, desca, ipiv, b, 1, 1, descb) == 0
+            xr = g[key + "_x"]
+            assert np.allclose(b, xr, rtol=1e-10, atol=1e-10 * np.abs(xr).max()), key
+            assert oracle.sresid(a0, b, b0) <= 1.0, key
+        seen.add((M, N, NB))
+    assert len(seen) == 12
+
+
+def test_argcheck_legal_cases_on_gpu(golden, ctx):
+    """The legal calls of the argument-check table (identity A): same INFO as
+    the reference."""
+    from test_abi import _cpu_safe, _run_argcheck
+    n = 0
+    for case in golden.argcheck_cases():
+        if _cpu_safe(case):
+            continue
+        _note(f"argcheck {case['routine']} {case['args']}")
+        assert _run_argcheck(case, ctx) == case["info"], case
+        n += 1
+    assert n >= 3
+
+
+def test_example_6x6_pdgesv(golden, ctx):
+    from scalapack_amd import api
+    a, b, xs = golden.example_6x6()
+    desca, _ = api.descinit(6, 6, 2, 2, 0, 0, ctx, 6)
+    descb, _ = api.descinit(6, 1, 2, 2, 0, 0, ctx, 6)
+    ipiv = np.zeros(8, np.int32)
+    assert api.pdgesv(6, 1, a, 1, 1, desca, ipiv, b, 1, 1, descb) == 0
+    assert np.allclose(b[:, 0], [float(v) for v in xs], rtol=1e-13)
+
+
+def test_zero_pivot_info_matches_oracle(oracle, ctx):
+    from scalapack_amd import api
+    n = 6
+    a = oracle.matgen(n, n, 7)
+    a[:, 3] = 0.0
+    a[:, 4] = a[:, 1]
+    ao = a.copy(order="F")
+    piv_o, info_o = oracle.pdgetrf(ao, 2)
+    desca, _ = api.descinit(n, n, 2, 2, 0, 0, ctx, n)
+    ipiv = np.zeros(n + 2, np.int32)
+    info = api.pdgetrf(n, n, a, 1, 1, desca, ipiv)
+    assert info == info_o == 4
+    assert np.array_equal(ipiv[:n], piv_o)
+
+
+@pytest.mark.parametrize("M,N,NB,IA", [(700, 300, 64, 1), (300, 700, 64, 1), (1000, 1000, 128, 129),
+                                       (513, 513, 256, 257), (2000, 1000, 512, 1)])
+def test_rectangular_and_offset_submatrices(oracle, ctx, M, N, NB, IA):
+    from scalapack_amd import api
+    _note(f"pdgetrf M={M} N={N} NB={NB} IA=JA={IA}")
+    LM, LN = M + IA - 1 + 5, N + IA - 1
+    a = oracle.matgen(LM, LN, 11)
+    ao = a.copy(order="F")
+    piv_o, info_o = oracle.pdgetrf(ao, NB, m=M, n=N, ia=IA, ja=IA)
+    desca, _ = api.descinit(LM, LN, NB, NB, 0, 0, ctx, LM)
+    ipiv = np.zeros(LM + NB, np.int32)
+    a_before = a.copy(order="F")
+    info = api.pdgetrf(M, N, a, IA, IA, desca, ipiv)
+    mn = min(M, N)
+    assert info == info_o == 0
+    assert np.array_equal(ipiv[IA - 1: IA - 1 + mn], piv_o)
+    sub0 = a_before[IA - 1: IA - 1 + M, IA - 1: IA - 1 + N]
+    sub = a[IA - 1: IA - 1 + M, IA - 1: IA - 1 + N]
+    assert oracle.fresid(sub0, sub, piv_o - (IA - 1)) < 1.0
+    # entries outside sub(A) are untouched
+    mask = np.ones_like(a, dtype=bool)
+    mask[IA - 1: IA - 1 + M, IA - 1: IA - 1 + N] = False
+    assert np.array_equal(a[mask], a_before[mask])
+
+
+@pytest.mark.parametrize("M", [33, 257, 5000, 70000])
+def test_panel_leaf_heights(oracle, dev, M):
+    """A 32-column panel: one leaf kernel over 1 .. 137 workgroups."""
+    _note(f"panel M={M} N=32")
+    a = oracle.matgen(M, 32, 5)
+    ao = a.copy(order="F")
+    piv_o, _ = oracle.pdgetrf(ao, 32)
+    dA = to_dev(a)
+    dp = torch.zeros(32, dtype=torch.int32, device="cuda")
+    assert dev.dgetrf(M, 32, dA, M, dp) == 0
+    assert np.array_equal(dp.cpu().numpy(), piv_o)
+    lu = from_dev(dA)
+    assert np.allclose(lu, ao, rtol=1e-12, atol=1e-12)
+
+
+def test_pdlaswp_rows_and_columns(oracle, ctx):
+    from scalapack_amd import api
+    rng = np.random.default_rng(4)
+    M, N = 40, 30
+    a = np.asfortranarray(rng.standard_normal((M, N)))
+    desca, _ = api.descinit(M, N, 4, 4, 0, 0, ctx, M)
+    ipiv = np.array([rng.integers(i + 1, M + 1) for i in range(M)], np.int32)
+    for direc in ("F", "B"):
+        g = a.copy(order="F")
+        oracle.pdlaswp(direc, g, 3, 20, 2, 17, ipiv)
+        t = a.copy(order="F")
+        api.pdlaswp(direc, "R", 20, t, 1, 3, desca, 2, 17, ipiv)
+        assert np.array_equal(t, g), direc
+    cp = np.array([rng.integers(i + 1, N + 1) for i in range(N)], np.int32)
+    t = a.copy(order="F")
+    api.pdlaswp("F", "C", 25, t, 4, 1, desca, 3, 12, cp)
+    g = a.copy()
+    for i in range(2, 12):
+        p = cp[i] - 1
+        g[3:28, [i, p]] = g[3:28, [p, i]]
+    assert np.array_equal(t, g)
+
+
+def test_getrs_standalone_many_rhs(oracle, ctx):
+    """PDGETRS on factors it did not compute (pivots rebuilt from IPIV) with
+    more right-hand sides than one solve launch takes."""
+    from scalapack_amd import api
+    n, nb, nrhs = 777, 64, 20
+    a = oracle.matgen(n, n, 21)
+    a0 = a.copy(order="F")
+    piv, info = oracle.pdgetrf(a, nb)
+    b = oracle.matgen(n, nrhs, 22)
+    b0 = b.copy(order="F")
+    x_o = oracle.pdgetrs(a, piv, b, nb)
+    desca, _ = api.descinit(n, n, nb, nb, 0, 0, ctx, n)
+    descb, _ = api.descinit(n, nrhs, nb, nb, 0, 0, ctx, n)
+    ipiv = np.zeros(n + nb, np.int32)
+    ipiv[:n] = piv
+    assert api.pdgetrs("N", n, nrhs, a, 1, 1, desca, ipiv, b, 1, 1, descb) == 0
+    assert np.allclose(b, x_o, rtol=1e-11, atol=1e-11 * np.abs(x_o).max())
+    assert oracle.sresid(a0, b, b0) <= 1.0
+
+
+# --------------------------------------------------------- device, config 2
+@pytest.mark.parametrize("nb", [128, 256, 512])
+def test_c2_device_matches_reference(oracle, golden, dev, nb):
+    g = golden.c2()
+    if g is None:
+        pytest.skip("config 2 fixture not generated")
+    n, nrhs = 16384, 16
+    _note(f"C2 device N={n} NB={nb}")
+    a = oracle.matgen(n, n, 100)
+    B = np.asfortranarray(np.random.default_rng(200).standard_normal((n, nrhs)))
+    dA, dB = to_dev(a), to_dev(B)
+    dp = torch.zeros(n, dtype=torch.int32, device="cuda")
+    dev.set("nb", nb)
+    info = dev.dgesv(n, nrhs, dA, n, dp, dB, n)
+    dev.set("nb", 256)
+    assert info == 0
+    piv = dp.cpu().numpy()
+    mism = np.nonzero(piv != g["ipiv"])[0]
+    assert mism.size == 0, f"IPIV differs from the reference at {mism[:10]}"
+    x = from_dev(dB)
+    s = oracle.sresid(a, x, B)
+    _note(f"C2 NB={nb} sresid={s:.3e}")
+    assert s < 16.0
+    xr = g["x"]
+    assert np.abs(x - xr).max() <= 1e-8 * np.abs(xr).max()
