@@ -419,7 +419,13 @@ void pdlaswp_(const char *direc, const char *rowcol, const int *n, double *a, co
             // Rows K1..K2 interchange with rows IPIV(K1..K2) of columns JA..JA+N-1
             // (pdlaswp.f:163-185); the touched rows are 1..max(K2, max IPIV).
             int hi = *k2;
-            for (int i = *k1; i <= *k2; i++) hi = std::max(hi, ipiv[i - 1]);
+            for (int i = *k1; i <= *k2; i++) {
+                if (ipiv[i - 1] < 1) {
+                    pxerbla(g, "PDLASWP", 10);
+                    return;
+                }
+                hi = std::max(hi, ipiv[i - 1]);
+            }
             DevBuf dA((size_t)hi * *n * 8), dp((size_t)hi * 4);
             double *acol = a + (size_t)(*ja - 1) * lld;
             to_device(dA.as<double>(), hi, acol, lld, hi, *n);
@@ -430,7 +436,13 @@ void pdlaswp_(const char *direc, const char *rowcol, const int *n, double *a, co
             // Columns K1..K2 interchange with columns IPIV(K1..K2) over rows
             // IA..IA+N-1 (pdlaswp.f:186-203).
             int hi = *k2;
-            for (int i = *k1; i <= *k2; i++) hi = std::max(hi, ipiv[i - 1]);
+            for (int i = *k1; i <= *k2; i++) {
+                if (ipiv[i - 1] < 1) {
+                    pxerbla(g, "PDLASWP", 10);
+                    return;
+                }
+                hi = std::max(hi, ipiv[i - 1]);
+            }
             DevBuf dA((size_t)*n * hi * 8), dp((size_t)hi * 4);
             double *arow = a + (size_t)(*ia - 1);
             to_device(dA.as<double>(), *n, arow, lld, *n, hi);

@@ -13,9 +13,10 @@
 // outputs, 4 doubles per lane).  Lane layout of that instruction:
 //   A operand : lane l holds A[l % 16][l / 16]         (16 rows x 4 k)
 //   B operand : lane l holds B[l / 16][l % 16]         (4 k x 16 cols)
-//   D         : lane l, element r holds D[4*(l/16)+r][l % 16]
-// so each lane's 4 results are 4 consecutive rows of one column: a 32-byte
-// contiguous piece of column-major C.
+//   D         : lane l, element r holds D[l/16 + 4*r][l % 16]
+// (checked on gfx950 with a one-hot probe), so for a fixed r the 64 lanes
+// cover 4 consecutive rows of 16 columns: 32 contiguous bytes per column of
+// column-major C.
 #include "sla_common.h"
 #include "sla_kernels.h"
 
@@ -139,8 +140,8 @@ __global__ __launch_bounds__(256, 1) void dgemm_nn_sub_kernel(
         __syncthreads();
     }
 
-    // Epilogue: C -= acc.  Lane owns rows 4*lk .. 4*lk+3 of column lr of each
-    // 16 x 16 block.
+    // Epilogue: C -= acc.  Lane owns rows lk, lk+4, lk+8, lk+12 of column lr
+    // of each 16 x 16 block.
 #pragma unroll
     for (int j = 0; j < 4; j++) {
         const int col = bn + wn + j * 16 + lr;
@@ -148,10 +149,10 @@ __global__ __launch_bounds__(256, 1) void dgemm_nn_sub_kernel(
         double *Cc = C + (size_t)col * ldc;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const int row0 = bm + wm + i * 16 + 4 * lk;
+            const int row0 = bm + wm + i * 16 + lk;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int row = row0 + r;
+                const int row = row0 + 4 * r;
                 if (row < m) Cc[row] = Cc[row] - acc[i][j][r];
             }
         }

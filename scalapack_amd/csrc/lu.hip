@@ -58,6 +58,7 @@ Handle::Handle()
     zero_col = ctl + 2;
     ticket = ctl + 3;
     pair_count = ctl + 4;
+    bad_pivot = ctl + 5;
 
     constexpr int GMAX = 256;
     slot_cand_v = alloc_dev<double>(2 * GMAX);
@@ -205,7 +206,7 @@ struct Lu {
     void make_pairs(int k1, int k2)
     {
         h.timing_begin(CAT_LASWP);
-        laswp_pairs(ipiv, k1, k2, h.pairs, h.pair_count, h.stream);
+        laswp_pairs(ipiv, k1, k2, M, h.pairs, h.pair_count, h.bad_pivot, h.stream);
         h.timing_end(0.0);
     }
 
@@ -288,7 +289,7 @@ int dgetrf_device(Handle &h, int m, int n, double *dA, int lda, int *dipiv)
     const int mn = std::min(m, n);
     h.ensure_perm(m);
     h.ctr_base = 0;
-    fill_int(h.ctl, 3, 0, h.stream);         // ctr, abort flag, zero column
+    fill_int(h.ctl, 6, 0, h.stream);         // counter, flags
     fill_int(h.zero_col, 1, INT_MAX, h.stream);
     iota_int(h.perm, m, 0, h.stream);
     Lu<double> lu{h, dA, lda, m, n, dipiv, h.panel_work<double>()};
@@ -313,7 +314,7 @@ static void perm_from_ipiv(Handle &h, int n, const int *dipiv)
     iota_int(h.perm, n, 0, h.stream);
     for (int k = 0; k < n; k += 512) {
         const int k2 = std::min(n, k + 512);
-        laswp_pairs(dipiv, k, k2, h.pairs, h.pair_count, h.stream);
+        laswp_pairs(dipiv, k, k2, n, h.pairs, h.pair_count, h.bad_pivot, h.stream);
         laswp_apply<int>(h.perm, n, 0, 1, h.pairs, h.pair_count, 2 * (k2 - k), h.stream);
     }
 }
@@ -322,6 +323,7 @@ int dgetrs_device(Handle &h, int n, int nrhs, const double *dA, int lda, const i
                   double *dB, int ldb)
 {
     if (n <= 0 || nrhs <= 0) return 0;
+    fill_int(h.bad_pivot, 1, 0, h.stream);
     if (!(h.perm_key_a == dA && h.perm_key_ipiv == dipiv && h.perm_m == n))
         perm_from_ipiv(h, n, dipiv);
     const int chunk = std::min(nrhs, 16);
@@ -345,11 +347,12 @@ int dgetrs_device(Handle &h, int n, int nrhs, const double *dA, int lda, const i
                                  (size_t)n * sizeof(double), nr, hipMemcpyDeviceToDevice, h.stream));
     }
     h.trsv_epoch = tw.epoch;
-    int ab = 0;
-    SLA_HIP(hipMemcpyAsync(&ab, h.abort_flag, sizeof ab, hipMemcpyDeviceToHost, h.stream));
+    int flags[6];
+    SLA_HIP(hipMemcpyAsync(flags, h.ctl, sizeof flags, hipMemcpyDeviceToHost, h.stream));
     SLA_HIP(hipStreamSynchronize(h.stream));
     h.timing_collect();
-    if (ab) throw std::runtime_error("dgetrs: a dependency wait timed out (triangular solve)");
+    if (flags[5]) throw std::runtime_error("dgetrs: IPIV holds a row outside 1..N");
+    if (flags[1]) throw std::runtime_error("dgetrs: a dependency wait timed out (triangular solve)");
     return 0;
 }
 
@@ -359,12 +362,17 @@ void laswp_rows_device(Handle &h, double *dA, int lda, int ncols, int k1, int k2
     std::vector<std::pair<int, int>> chunks;
     for (int k = k1; k < k2; k += 512) chunks.push_back({k, std::min(k2, k + 512)});
     if (!forward) std::reverse(chunks.begin(), chunks.end());
+    fill_int(h.bad_pivot, 1, 0, h.stream);
     for (auto &c : chunks) {
-        laswp_pairs(dipiv, c.first, c.second, h.pairs, h.pair_count, h.stream);
+        laswp_pairs(dipiv, c.first, c.second, lda, h.pairs, h.pair_count, h.bad_pivot,
+                    h.stream);
         laswp_apply<double>(dA, lda, 0, ncols, h.pairs, h.pair_count, 2 * (c.second - c.first),
                             h.stream, !forward);
     }
+    int bad = 0;
+    SLA_HIP(hipMemcpyAsync(&bad, h.bad_pivot, sizeof bad, hipMemcpyDeviceToHost, h.stream));
     SLA_HIP(hipStreamSynchronize(h.stream));
+    if (bad) throw std::runtime_error("pdlaswp: IPIV holds a row outside the matrix");
 }
 
 void laswp_cols_device(Handle &h, double *dA, int lda, int nrows, int k1, int k2,

@@ -32,10 +32,12 @@ struct Handle {
     uint64_t spin_limit = 0;
 
     // device control block: [0] barrier counter, [1] abort flag,
-    // [2] first zero-pivot column (INT_MAX: none), [3] ticket, [4] pair count
+    // [2] first zero-pivot column (INT_MAX: none), [3] ticket, [4] pair count,
+    // [5] pivot-row-out-of-range flag
     int *ctl = nullptr;
     unsigned *ctr = nullptr;
     int *abort_flag = nullptr, *zero_col = nullptr, *ticket = nullptr, *pair_count = nullptr;
+    int *bad_pivot = nullptr;
     unsigned ctr_base = 0;
 
     double *slot_cand_v = nullptr;

@@ -41,8 +41,10 @@ unsigned panel_leaf(T *A, int lda, int m, int wa, int row0, int col0, int *ipiv,
 // Net effect of the forward interchanges ipiv[k1 .. k2-1] (0-based global row
 // indices of the pivoted rows, ipiv 1-based global) as a list of
 // (dst, src) global rows: after the interchanges row dst holds what row src
-// held before.  At most 2*(k2-k1) pairs; k2-k1 <= 512.
-void laswp_pairs(const int *ipiv, int k1, int k2, int *pairs, int *count, hipStream_t s);
+// held before.  At most 2*(k2-k1) pairs; k2-k1 <= 512.  A pivot row outside
+// [1, nrows] sets *bad and yields an empty list.
+void laswp_pairs(const int *ipiv, int k1, int k2, int nrows, int *pairs, int *count, int *bad,
+                 hipStream_t s);
 // Applies a pair list to columns [0, ncols) of A whose row 0 is global row
 // `roff`; `inverse` applies the inverse permutation (the same interchanges
 // replayed backward).

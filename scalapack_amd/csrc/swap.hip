@@ -19,23 +19,40 @@ namespace {
 constexpr int PAIRS_MAX_PIV = 512;
 
 // One workgroup.  Slots 0..n-1 stand for rows k1..k2-1; slot n+i stands for
-// pivot row ipiv[k1+i] >= k2 at its first occurrence.  The interchanges are
-// replayed on slot contents (row ids), then each slot s emits the pair
-// (row_of(s), content(s)).
+// pivot row ipiv[k1+i] outside [k1, k2) at its first occurrence (factorization
+// pivots are >= their row, a caller's IPIV may hold any row).  The
+// interchanges are replayed on slot contents (row ids), then each slot s
+// emits the pair (row_of(s), content(s)).  A pivot row outside [0, nrows)
+// raises *bad and the block emits no pairs.
 __global__ __launch_bounds__(512) void laswp_pairs_kernel(const int *__restrict__ ipiv, int k1,
-                                                          int k2, int *__restrict__ pairs,
-                                                          int *__restrict__ count)
+                                                          int k2, int nrows,
+                                                          int *__restrict__ pairs,
+                                                          int *__restrict__ count, int *bad)
 {
     __shared__ int tgt[PAIRS_MAX_PIV];
     __shared__ int first[PAIRS_MAX_PIV];
     __shared__ int content[2 * PAIRS_MAX_PIV];
+    __shared__ int s_bad;
     const int n = k2 - k1, tid = threadIdx.x;
-    for (int i = tid; i < n; i += blockDim.x) tgt[i] = ipiv[k1 + i] - 1;
+    if (tid == 0) s_bad = 0;
     __syncthreads();
+    for (int i = tid; i < n; i += blockDim.x) {
+        const int t = ipiv[k1 + i] - 1;
+        tgt[i] = t;
+        if (t < 0 || t >= nrows) s_bad = 1;
+    }
+    __syncthreads();
+    if (s_bad) {
+        if (tid == 0) {
+            *count = 0;
+            atomicExch(bad, 1);
+        }
+        return;
+    }
     for (int i = tid; i < n; i += blockDim.x) {
         const int t = tgt[i];
         int f = -1;
-        if (t >= k2) {
+        if (t < k1 || t >= k2) {
             f = i;
             for (int i2 = 0; i2 < i; i2++)
                 if (tgt[i2] == t) { f = i2; break; }
@@ -51,7 +68,7 @@ __global__ __launch_bounds__(512) void laswp_pairs_kernel(const int *__restrict_
     if (tid == 0) {
         for (int i = 0; i < n; i++) {
             const int t = tgt[i];
-            const int s = (t < k2) ? (t - k1) : (n + first[i]);
+            const int s = (t >= k1 && t < k2) ? (t - k1) : (n + first[i]);
             if (s != i) {
                 const int x = content[i];
                 content[i] = content[s];
@@ -146,11 +163,13 @@ __global__ void gather_rows_kernel(const T *__restrict__ src, int lds, T *__rest
 
 }  // namespace
 
-void laswp_pairs(const int *ipiv, int k1, int k2, int *pairs, int *count, hipStream_t s)
+void laswp_pairs(const int *ipiv, int k1, int k2, int nrows, int *pairs, int *count, int *bad,
+                 hipStream_t s)
 {
     if (k2 - k1 > PAIRS_MAX_PIV || k2 <= k1)
         throw std::logic_error("laswp_pairs: pivot block must hold 1..512 pivots");
-    hipLaunchKernelGGL(laswp_pairs_kernel, dim3(1), dim3(512), 0, s, ipiv, k1, k2, pairs, count);
+    hipLaunchKernelGGL(laswp_pairs_kernel, dim3(1), dim3(512), 0, s, ipiv, k1, k2, nrows, pairs,
+                       count, bad);
     SLA_LAUNCH_CHECK();
 }
 

@@ -66,7 +66,14 @@ def _run_argcheck(case, ctx):
         d = np.array(d, np.int32)
         d[1] = ctx + 77 if d[1] == -999 else ctx
         return d
-    a = np.asfortranarray(np.eye(8))
+    # identity A laid out with the descriptor's LLD, as the reference driver
+    # builds it (oracle/ref_driver.c, argcheck mode)
+    lld = max(1, case["desca"][8])
+    a = np.zeros(max(64, lld * 32))
+    for i in range(32):
+        if i * (lld + 1) < a.size:
+            a[i * (lld + 1)] = 1.0
+    a = np.asfortranarray(a.reshape(-1, lld).T) if lld > 1 else np.asfortranarray(a[None, :])
     b = np.asfortranarray(np.zeros((8, 2)))
     ipiv = np.arange(1, 33, dtype=np.int32)
     r = case["routine"]
