@@ -63,7 +63,7 @@ __device__ __forceinline__ void tile_coords(int tiles_m, int tiles_n, int &tm, i
     }
 }
 
-__global__ __launch_bounds__(256, 1) void dgemm_nn_sub_kernel(
+__global__ __launch_bounds__(256, 2) void dgemm_nn_sub_kernel(
     int m, int n, int k, const double *__restrict__ A, int lda,
     const double *__restrict__ B, int ldb, double *__restrict__ C, int ldc)
 {
@@ -159,6 +159,85 @@ __global__ __launch_bounds__(256, 1) void dgemm_nn_sub_kernel(
     }
 }
 
+// The interior kernel: no LDS and no workgroup barrier.  Each wave owns a
+// 64 x 64 tile of C and feeds v_mfma_f64_16x16x4f64 straight from HBM/L2 into
+// VGPRs, three 16-deep K steps in flight.  Two index permutations make every
+// lane's operands 32 contiguous bytes (one 32-byte load each):
+//   rows : MFMA block i, operand row lr  <->  C row wm + 4*lr + i
+//   K    : operand k-lane lk, step s     <->  k = k0 + 4*lk + s
+// Both A and B use the same K permutation, so the sum is unchanged; the
+// epilogue maps D element r of lane l (block row l/16 + 4r) back to C row
+// wm + 4*(l/16 + 4r) + i, four consecutive rows for i = 0..3.
+// Requires m, n multiples of 128, k a multiple of 16, 32-byte aligned column
+// starts (lda, ldb, ldc multiples of 4 and aligned base pointers).
+typedef double dv4_t __attribute__((ext_vector_type(4)));
+constexpr int DEPTH = 3;
+
+__global__ __launch_bounds__(256, 1) void dgemm_nn_sub_direct_kernel(
+    int m, int n, int k, const double *__restrict__ A, int lda,
+    const double *__restrict__ B, int ldb, double *__restrict__ C, int ldc)
+{
+    const int tiles_m = m / 128, tiles_n = n / 128;
+    int tmi, tni;
+    tile_coords(tiles_m, tiles_n, tmi, tni);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = tmi * 128 + (wave & 1) * 64, wn = tni * 128 + (wave >> 1) * 64;
+    const int lr = lane & 15, lk = lane >> 4;
+    const double *Ap = A + (wm + 4 * lr) + (size_t)(4 * lk) * lda;
+    const double *Bp = B + (size_t)(wn + lr) * ldb + 4 * lk;
+    const size_t bstep = (size_t)16 * ldb;
+
+    dv4_t a[DEPTH][4], b[DEPTH][4];
+    auto load = [&](int t, int q) {
+        const size_t k0 = (size_t)16 * t;
+#pragma unroll
+        for (int s = 0; s < 4; s++) a[q][s] = *(const dv4_t *)(Ap + (k0 + s) * lda);
+#pragma unroll
+        for (int j = 0; j < 4; j++) b[q][j] = *(const dv4_t *)(Bp + j * bstep + k0);
+    };
+    d4_t acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+
+    const int nk = k / 16;
+#pragma unroll
+    for (int q = 0; q < DEPTH - 1; q++)
+        if (q < nk) load(q, q);
+    for (int t = 0; t < nk; t += DEPTH) {
+#pragma unroll
+        for (int q = 0; q < DEPTH; q++) {
+            if (t + q < nk) {
+                const int nt = t + q + DEPTH - 1;
+                if (nt < nk) load(nt, (q + DEPTH - 1) % DEPTH);
+#pragma unroll
+                for (int s = 0; s < 4; s++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+#pragma unroll
+                        for (int j = 0; j < 4; j++)
+                            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[q][s][i], b[q][j][s],
+                                                                            acc[i][j], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        double *Cc = C + (size_t)(wn + 16 * j + lr) * ldc;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            dv4_t *cp = (dv4_t *)(Cc + wm + 4 * (lk + 4 * r));
+            dv4_t c = *cp;
+            c[0] -= acc[0][j][r];
+            c[1] -= acc[1][j][r];
+            c[2] -= acc[2][j][r];
+            c[3] -= acc[3][j][r];
+            *cp = c;
+        }
+    }
+}
+
 // C[m x n] -= A[m x k] * B[k x n] for n <= 16 (the right-hand sides of the
 // triangular solves).  HBM-bound: each thread streams rows of A once.
 template <int NMAX>
@@ -205,6 +284,22 @@ void dgemm_nn_sub(int m, int n, int k, const double *A, int lda, const double *B
         else
             hipLaunchKernelGGL(dgemm_nn_sub_skinny_kernel<16>, dim3(grid), dim3(256), sh, s, m, n, k, A, lda, B, ldb, C, ldc);
         SLA_LAUNCH_CHECK();
+        return;
+    }
+    auto aligned = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 31) == 0; };
+    const int mf = (m / 128) * 128, nf = (n / 128) * 128;
+    if (mf > 0 && nf > 0 && k % 16 == 0 && lda % 4 == 0 && ldb % 4 == 0 && ldc % 4 == 0 &&
+        aligned(A) && aligned(B) && aligned(C)) {
+        const int tiles = (mf / 128) * (nf / 128);
+        hipLaunchKernelGGL(dgemm_nn_sub_direct_kernel, dim3(tiles), dim3(256), 0, s, mf, nf, k,
+                           A, lda, B, ldb, C, ldc);
+        SLA_LAUNCH_CHECK();
+        // ragged edges: the rows below the interior (all columns) and the
+        // columns right of it
+        if (m > mf) dgemm_nn_sub(m - mf, n, k, A + mf, lda, B, ldb, C + mf, ldc, s);
+        if (n > nf)
+            dgemm_nn_sub(mf, n - nf, k, A, lda, B + (size_t)nf * ldb, ldb, C + (size_t)nf * ldc,
+                         ldc, s);
         return;
     }
     const int tiles = ceil_div(m, TM) * ceil_div(n, TN);

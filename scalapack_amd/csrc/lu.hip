@@ -43,7 +43,11 @@ Handle::Handle()
 {
     SLA_HIP(hipGetDevice(&device));
     SLA_HIP(hipStreamCreateWithFlags(&own_stream, hipStreamNonBlocking));
+    SLA_HIP(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
     stream = own_stream;
+    SLA_HIP(hipEventCreateWithFlags(&ev_next, hipEventDisableTiming));
+    SLA_HIP(hipEventCreateWithFlags(&ev_panel, hipEventDisableTiming));
+    SLA_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
 
     int clk_khz = 100000;
     if (hipDeviceGetAttribute(&clk_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess ||
@@ -67,7 +71,8 @@ Handle::Handle()
     slot_rowj = alloc_dev<double>(2 * 32);
     slot_gmax = GMAX;
 
-    pairs = alloc_dev<int>(4 * 512);
+    pairs = alloc_dev<int>(2 * 2048);
+    pair_counts = alloc_dev<int>(2);
 
     trsv_max_blocks = 1 << 16;
     trsv_flags = alloc_dev<int>(trsv_max_blocks);
@@ -80,10 +85,15 @@ Handle::~Handle()
 {
     hipStreamSynchronize(stream);
     for (void *p : {(void *)ctl, (void *)slot_cand_v, (void *)slot_cand_r, (void *)slot_rows,
-                    (void *)slot_rowj, (void *)pairs, (void *)trsv_flags, (void *)perm,
-                    (void *)work})
+                    (void *)slot_rowj, (void *)pairs, (void *)pair_counts, (void *)trsv_flags,
+                    (void *)perm, (void *)work})
         if (p) hipFree(p);
     for (auto &e : ev_pool) hipEventDestroy(e);
+    hipEventDestroy(ev_next);
+    hipEventDestroy(ev_panel);
+    hipEventDestroy(ev_join);
+    hipStreamSynchronize(stream2);
+    hipStreamDestroy(stream2);
     hipStreamDestroy(own_stream);
 }
 
@@ -134,9 +144,10 @@ void Handle::ensure_work(size_t bytes)
 }
 
 // --- per-kernel timing -----------------------------------------------------
-void Handle::timing_begin(int cat)
+void Handle::timing_begin(int cat, hipStream_t s)
 {
     if (!profile) return;
+    if (!s) s = stream;
     hipEvent_t e0, e1;
     if (ev_free.size() < 2) {
         SLA_HIP(hipEventCreate(&e0));
@@ -149,22 +160,30 @@ void Handle::timing_begin(int cat)
         e1 = ev_free.back();
         ev_free.pop_back();
     }
-    SLA_HIP(hipEventRecord(e0, stream));
-    pending.push_back({cat, 0.0, e0, e1});
+    SLA_HIP(hipEventRecord(e0, s));
+    pending.push_back({cat, 0.0, e0, e1, s});
 }
 
-void Handle::timing_end(double work_units)
+void Handle::timing_end(double work_units, hipStream_t s)
 {
     if (!profile) return;
-    Pending &p = pending.back();
-    p.work = work_units;
-    SLA_HIP(hipEventRecord(p.e1, stream));
+    if (!s) s = stream;
+    // the open interval of this stream
+    for (auto it = pending.rbegin(); it != pending.rend(); ++it) {
+        if (it->s == s && !it->closed) {
+            it->work = work_units;
+            it->closed = true;
+            SLA_HIP(hipEventRecord(it->e1, s));
+            return;
+        }
+    }
 }
 
 void Handle::timing_collect()
 {
     if (!profile) return;
     SLA_HIP(hipStreamSynchronize(stream));
+    SLA_HIP(hipStreamSynchronize(stream2));
     for (auto &p : pending) {
         float ms = 0.f;
         SLA_HIP(hipEventElapsedTime(&ms, p.e0, p.e1));
@@ -195,90 +214,126 @@ struct Lu {
 
     T *at(int i, int j) const { return A + (size_t)i + (size_t)j * lda; }
 
-    void swap_block(int k1, int k2, T *cols, int ncols)
+    // Pair list buffer q (two alternate between consecutive block columns).
+    int *pairs(int q) const { return h.pairs + q * 2048; }
+    int *pair_count(int q) const { return h.pair_counts + q; }
+
+    void make_pairs(int k1, int k2, int q, hipStream_t s)
+    {
+        h.timing_begin(CAT_LASWP, s);
+        laswp_pairs(ipiv, k1, k2, M, pairs(q), pair_count(q), h.bad_pivot, s);
+        h.timing_end(0.0, s);
+    }
+
+    void swap_block(int k1, int k2, int q, int c0, int ncols, hipStream_t s)
     {
         if (ncols <= 0) return;
-        h.timing_begin(CAT_LASWP);
-        laswp_apply<T>(cols, lda, 0, ncols, h.pairs, h.pair_count, 2 * (k2 - k1), h.stream);
-        h.timing_end(32.0 * (double)(k2 - k1) * ncols);
+        h.timing_begin(CAT_LASWP, s);
+        laswp_apply<T>(at(0, c0), lda, 0, ncols, pairs(q), pair_count(q), 2 * (k2 - k1), s);
+        h.timing_end(32.0 * (double)(k2 - k1) * ncols, s);
     }
 
-    void make_pairs(int k1, int k2)
-    {
-        h.timing_begin(CAT_LASWP);
-        laswp_pairs(ipiv, k1, k2, M, h.pairs, h.pair_count, h.bad_pivot, h.stream);
-        h.timing_end(0.0);
-    }
+    void gemm(int m, int n, int k, const T *a, const T *b, T *c, hipStream_t s);
 
-    void gemm(int m, int n, int k, const T *a, const T *b, T *c);
-    void trsm(int m, int n, const T *l, T *b)
+    // B[m x n] <- L^{-1} B for the unit lower m x m L at l: recursive halving
+    // down to <= 32-row diagonal solves, the off-diagonal products on the
+    // MFMA GEMM (the 32-row blocking mirrors PB_CptrsmAB.c:359-412).
+    void trsm(int m, int n, const T *l, T *b, hipStream_t s)
     {
         if (m <= 0 || n <= 0) return;
-        h.timing_begin(CAT_TRSM);
-        // trsm_llnu holds at most 256 rows of L; taller blocks are split
-        // into 256-row diagonal solves and a gemm update between them.
-        for (int k = 0; k < m; k += 256) {
-            const int kb = std::min(256, m - k);
-            trsm_llnu<T>(kb, n, l + (size_t)k + (size_t)k * lda, lda, b + k, lda, h.stream);
-            if (k + kb < m) gemm(m - k - kb, n, kb, l + (size_t)(k + kb) + (size_t)k * lda, b + k, b + k + kb);
+        if (m <= 32) {
+            h.timing_begin(CAT_TRSM, s);
+            trsm_llnu<T>(m, n, l, lda, b, lda, s);
+            h.timing_end((double)m * m * n, s);
+            return;
         }
-        h.timing_end((double)m * m * n);
+        const int m1 = ((m / 2 + 31) / 32) * 32;
+        trsm(m1, n, l, b, s);
+        gemm(m - m1, n, m1, l + m1, b, b + m1, s);
+        trsm(m - m1, n, l + (size_t)m1 + (size_t)m1 * lda, b + m1, s);
     }
 
-    // Recursive panel on rows r0.., columns c0 .. c0+n-1 (global indices).
-    void panel(int r0, int c0, int n)
+    // Recursive panel on rows r0.., columns c0 .. c0+n-1 (global indices) of
+    // the block column [pc0, pc1).  Every leaf applies its interchanges to
+    // the whole block column, so no interchange step is needed in between.
+    void panel(int r0, int c0, int n, int pc0, int pc1, hipStream_t s)
     {
         const int m = M - r0;
         if (n <= h.leaf_w) {
-            h.timing_begin(CAT_PANEL);
-            h.ctr_base += panel_leaf<T>(at(r0, c0), lda, m, n, r0, c0, ipiv, pw, h.ctr_base, h.stream);
-            h.timing_end((double)m * n * n);
+            h.timing_begin(CAT_PANEL, s);
+            h.ctr_base += panel_leaf<T>(at(r0, c0), lda, m, n, c0 - pc0, pc1 - c0 - n, r0, c0,
+                                        ipiv, pw, h.ctr_base, s);
+            h.timing_end((double)m * n * n, s);
             return;
         }
         int n1 = (n / 2 / h.leaf_w) * h.leaf_w;
         if (n1 < h.leaf_w) n1 = h.leaf_w;
-        panel(r0, c0, n1);
-        make_pairs(r0, r0 + n1);
-        swap_block(r0, r0 + n1, at(0, c0 + n1), n - n1);
-        trsm(n1, n - n1, at(r0, c0), at(r0, c0 + n1));
-        gemm(m - n1, n - n1, n1, at(r0 + n1, c0), at(r0, c0 + n1), at(r0 + n1, c0 + n1));
-        panel(r0 + n1, c0 + n1, n - n1);
-        make_pairs(r0 + n1, r0 + n);
-        swap_block(r0 + n1, r0 + n, at(0, c0), n1);
+        panel(r0, c0, n1, pc0, pc1, s);
+        trsm(n1, n - n1, at(r0, c0), at(r0, c0 + n1), s);
+        gemm(m - n1, n - n1, n1, at(r0 + n1, c0), at(r0, c0 + n1), at(r0 + n1, c0 + n1), s);
+        panel(r0 + n1, c0 + n1, n - n1, pc0, pc1, s);
     }
 
+    // Right-looking blocked LU (SRC/pdgetrf.f:227-295) with a look-ahead of
+    // one block column.  Main stream S0, step of block column p = [j, jn):
+    //   apply p's interchanges to the previous block column and to the
+    //   columns right of p, and to the kept row permutation;
+    //   U12 and the Schur update for the NEXT block column only -> event;
+    //   U12 and the Schur update for the rest of the trailing matrix.
+    // Panel stream S1, once the next block column is updated: factor it,
+    // build its pair list, apply its interchanges to the columns left of the
+    // block column before it (nothing reads those any more).  S0 waits for S1
+    // before its next step.  Interchanges of one column range therefore
+    // always arrive in panel order.
     int run()
     {
-        const int mn = std::min(M, N);
-        for (int j = 0; j < mn; j += h.nb) {
-            const int jb = std::min(h.nb, mn - j);
-            panel(j, j, jb);
-            // The panel's pivots on the columns left and right of it and on
-            // the row permutation kept for the solve (PDLASWP, pdgetrf.f:268,275).
-            for (int k = j; k < j + jb; k += 512) {
-                const int k2 = std::min(j + jb, k + 512);
-                make_pairs(k, k2);
-                swap_block(k, k2, at(0, 0), j);
-                swap_block(k, k2, at(0, j + jb), N - j - jb);
-                laswp_apply<int>(h.perm, M, 0, 1, h.pairs, h.pair_count, 2 * (k2 - k), h.stream);
+        const int mn = std::min(M, N), nb = h.nb;
+        if (nb > 512) throw std::logic_error("dgetrf: NB above 512");
+        hipStream_t s0 = h.stream, s1 = h.stream2;
+        int jprev = 0;
+        {
+            const int jb = std::min(nb, mn);
+            panel(0, 0, jb, 0, jb, s0);
+            make_pairs(0, jb, 0, s0);
+        }
+        for (int j = 0, p = 0; j < mn; j += nb, p++) {
+            const int jb = std::min(nb, mn - j), jn = j + jb, q = p & 1;
+            swap_block(j, jn, q, jprev, j - jprev, s0);
+            swap_block(j, jn, q, jn, N - jn, s0);
+            laswp_apply<int>(h.perm, M, 0, 1, pairs(q), pair_count(q), 2 * jb, s0);
+            if (jn >= N) break;
+            const bool has_next = jn < mn;
+            const int jb2 = has_next ? std::min(nb, mn - jn) : 0;
+            if (has_next) {
+                trsm(jb, jb2, at(j, j), at(j, jn), s0);
+                gemm(M - jn, jb2, jb, at(jn, j), at(j, jn), at(jn, jn), s0);
+                SLA_HIP(hipEventRecord(h.ev_next, s0));
+                SLA_HIP(hipStreamWaitEvent(s1, h.ev_next, 0));
+                panel(jn, jn, jb2, jn, jn + jb2, s1);
+                make_pairs(jn, jn + jb2, q ^ 1, s1);
+                swap_block(jn, jn + jb2, q ^ 1, 0, j, s1);
+                SLA_HIP(hipEventRecord(h.ev_panel, s1));
             }
-            if (j + jb < N) {
-                trsm(jb, N - j - jb, at(j, j), at(j, j + jb));
-                if (j + jb < M)
-                    gemm(M - j - jb, N - j - jb, jb, at(j + jb, j), at(j, j + jb), at(j + jb, j + jb));
+            const int c = jn + jb2, rest = N - c;
+            if (rest > 0) {
+                trsm(jb, rest, at(j, j), at(j, c), s0);
+                if (jn < M) gemm(M - jn, rest, jb, at(jn, j), at(j, c), at(jn, c), s0);
             }
+            if (has_next) SLA_HIP(hipStreamWaitEvent(s0, h.ev_panel, 0));
+            jprev = j;
         }
         return 0;
     }
 };
 
 template <>
-void Lu<double>::gemm(int m, int n, int k, const double *a, const double *b, double *c)
+void Lu<double>::gemm(int m, int n, int k, const double *a, const double *b, double *c,
+                      hipStream_t s)
 {
     if (m <= 0 || n <= 0 || k <= 0) return;
-    h.timing_begin(CAT_GEMM);
-    dgemm_nn_sub(m, n, k, a, lda, b, lda, c, lda, h.stream);
-    h.timing_end(2.0 * m * n * k);
+    h.timing_begin(CAT_GEMM, s);
+    dgemm_nn_sub(m, n, k, a, lda, b, lda, c, lda, s);
+    h.timing_end(2.0 * m * n * k, s);
 }
 
 }  // namespace
@@ -292,8 +347,13 @@ int dgetrf_device(Handle &h, int m, int n, double *dA, int lda, int *dipiv)
     fill_int(h.ctl, 6, 0, h.stream);         // counter, flags
     fill_int(h.zero_col, 1, INT_MAX, h.stream);
     iota_int(h.perm, m, 0, h.stream);
+    // the panel stream starts after everything queued so far on the main one
+    SLA_HIP(hipEventRecord(h.ev_join, h.stream));
+    SLA_HIP(hipStreamWaitEvent(h.stream2, h.ev_join, 0));
     Lu<double> lu{h, dA, lda, m, n, dipiv, h.panel_work<double>()};
     lu.run();
+    SLA_HIP(hipEventRecord(h.ev_join, h.stream2));
+    SLA_HIP(hipStreamWaitEvent(h.stream, h.ev_join, 0));
     int flags[3];
     SLA_HIP(hipMemcpyAsync(flags, h.ctl, sizeof flags, hipMemcpyDeviceToHost, h.stream));
     SLA_HIP(hipStreamSynchronize(h.stream));

@@ -12,7 +12,11 @@
 //   3. one device-wide barrier;
 //   4. every workgroup reduces the G candidates in the same total order, so
 //      all agree on the pivot row p without a second barrier;
-//   5. the owners of rows j and p exchange them in registers;
+//   5. the owners of rows j and p exchange them in registers, and the
+//      interchange is applied to the rest of the block column outside the
+//      leaf (PDGETF2 swaps the full JB-wide panel rows, pdgetf2.f:219-220),
+//      each of those columns always by the same thread, so successive
+//      interchanges of one column stay in order;
 //   6. rows below j are scaled by the reciprocal 1/pivot (the reference's
 //      ONE / GMAX multiply, pdgetf2.f:224) and updated a -= l * u with one fma.
 // A zero pivot skips the interchange and the scaling and records the first
@@ -26,23 +30,27 @@ namespace sla {
 
 namespace {
 
-constexpr int PT = 256;  // threads per workgroup
+constexpr int PT = 512;          // threads per workgroup (8 waves)
+constexpr int NWV = PT / 64;
+constexpr int LEAF_GMAX = 128;   // workgroups per leaf launch at most
 
 template <typename T, int W, int RPT>
 __global__ __launch_bounds__(PT) void panel_leaf_kernel(
-    T *__restrict__ A, int lda, int m, int wa, int row0, int col0, int *__restrict__ ipiv,
-    PanelSlots<T> slots, unsigned *ctr, unsigned base, int *abort_flag, int *zero_col,
-    uint64_t spin_limit)
+    T *__restrict__ A, int lda, int m, int wa, int lcols, int rcols, int row0, int col0,
+    int *__restrict__ ipiv, PanelSlots<T> slots, unsigned *ctr, unsigned base, int *abort_flag,
+    int *zero_col, uint64_t spin_limit)
 {
     const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     constexpr int R = RPT * PT;  // rows per workgroup
 
-    __shared__ double s_wv[PT / 64];
-    __shared__ int s_wr[PT / 64];
-    __shared__ int s_p;
-    __shared__ T s_piv[W];
+    __shared__ double s_wv[NWV];
+    __shared__ int s_wr[NWV];
+    __shared__ double s_cv[LEAF_GMAX];
+    __shared__ int s_cr[LEAF_GMAX];
+    __shared__ T s_rows[LEAF_GMAX][W];
     __shared__ T s_rowj[W];
+    __shared__ int s_p;
 
     T a[RPT][W];
 #pragma unroll
@@ -58,7 +66,8 @@ __global__ __launch_bounds__(PT) void panel_leaf_kernel(
         if (j < wa) {  // uniform across the grid
         const int par = j & 1;
 
-        // 1. candidate of this workgroup
+        // 1. candidate of this workgroup: per thread, per wave (shuffles),
+        //    then every thread folds the NWV wave results itself
         double bv = -1.0;
         int br = INT_MAX;
 #pragma unroll
@@ -77,34 +86,30 @@ __global__ __launch_bounds__(PT) void panel_leaf_kernel(
         }
         if (lane == 0) { s_wv[wave] = bv; s_wr[wave] = br; }
         __syncthreads();
-        if (tid == 0) {
-            double v = s_wv[0];
-            int r = s_wr[0];
+        double cv = s_wv[0];
+        int cr = s_wr[0];
 #pragma unroll
-            for (int w = 1; w < PT / 64; w++)
-                if (pivot_better(s_wv[w], s_wr[w], v, r)) { v = s_wv[w]; r = s_wr[w]; }
-            coherent_store(&slots.cand_v[par * slots.gmax + g], v);
-            coherent_store(&slots.cand_r[par * slots.gmax + g], r);
-            s_p = r;
-        }
-        __syncthreads();
+        for (int w = 1; w < NWV; w++)
+            if (pivot_better(s_wv[w], s_wr[w], cv, cr)) { cv = s_wv[w]; cr = s_wr[w]; }
 
-        // 2. publish the candidate row and (workgroup 0) row j
-        {
-            const int cr = s_p;
+        // 2. publish the candidate (value, row, the row's W entries) and,
+        //    from workgroup 0, row j
+        if (tid == 0) {
+            coherent_store(&slots.cand_v[par * slots.gmax + g], cv);
+            coherent_store(&slots.cand_r[par * slots.gmax + g], cr);
+        }
 #pragma unroll
-            for (int q = 0; q < RPT; q++) {
-                const int r = g * R + q * PT + tid;
-                if (r == cr) {
-                    T *dst = slots.rows + ((size_t)par * slots.gmax + g) * W;
+        for (int q = 0; q < RPT; q++) {
+            const int r = g * R + q * PT + tid;
+            if (r == cr) {
+                T *dst = slots.rows + ((size_t)par * slots.gmax + g) * W;
 #pragma unroll
-                    for (int c = 0; c < W; c++) coherent_store(&dst[c], a[q][c]);
-                }
-                if (r == j) {
-                    T *dst = slots.rowj + (size_t)par * W;
+                for (int c = 0; c < W; c++) coherent_store(&dst[c], a[q][c]);
+            }
+            if (r == j) {
+                T *dst = slots.rowj + (size_t)par * W;
 #pragma unroll
-                    for (int c = 0; c < W; c++) coherent_store(&dst[c], a[q][c]);
-                }
+                for (int c = 0; c < W; c++) coherent_store(&dst[c], a[q][c]);
             }
         }
 
@@ -112,15 +117,22 @@ __global__ __launch_bounds__(PT) void panel_leaf_kernel(
         if (!grid_sync(ctr, base + (unsigned)(j + 1) * (unsigned)G, abort_flag, spin_limit))
             return;
 
-        // 4. global pivot: identical reduction order in every workgroup
+        // 4. one remote round trip: all candidates, all candidate rows, row j
+        if (tid < G) {
+            s_cv[tid] = coherent_load(&slots.cand_v[par * slots.gmax + tid]);
+            s_cr[tid] = coherent_load(&slots.cand_r[par * slots.gmax + tid]);
+        }
+        for (int idx = tid; idx < G * W; idx += PT)
+            s_rows[idx / W][idx % W] = coherent_load(&slots.rows[(size_t)par * slots.gmax * W + idx]);
+        if (tid < W) s_rowj[tid] = coherent_load(&slots.rowj[(size_t)par * W + tid]);
+        __syncthreads();
+
+        // 5. global pivot: the same reduction order in every workgroup
         if (wave == 0) {
             double v = -1.0;
             int r = INT_MAX;
-            for (int x = lane; x < G; x += 64) {
-                const double cv = coherent_load(&slots.cand_v[par * slots.gmax + x]);
-                const int cr = coherent_load(&slots.cand_r[par * slots.gmax + x]);
-                if (pivot_better(cv, cr, v, r)) { v = cv; r = cr; }
-            }
+            for (int x = lane; x < G; x += 64)
+                if (pivot_better(s_cv[x], s_cr[x], v, r)) { v = s_cv[x]; r = s_cr[x]; }
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) {
                 const double ov = __shfl_xor(v, off);
@@ -131,16 +143,11 @@ __global__ __launch_bounds__(PT) void panel_leaf_kernel(
         }
         __syncthreads();
         const int p = s_p;
-        if (tid < W) {
-            const int gw = p / R;
-            s_piv[tid] = coherent_load(&slots.rows[((size_t)par * slots.gmax + gw) * W + tid]);
-            s_rowj[tid] = coherent_load(&slots.rowj[(size_t)par * W + tid]);
-        }
-        __syncthreads();
-        const T piv = s_piv[j];
+        const int gw = p / R;
+        const T piv = s_rows[gw][j];
         const bool nonzero = (piv != T(0));
 
-        // 5. interchange rows j and p (only if the pivot is nonzero)
+        // 6. interchange rows j and p (only if the pivot is nonzero)
         if (nonzero && p != j) {
 #pragma unroll
             for (int q = 0; q < RPT; q++) {
@@ -151,8 +158,19 @@ __global__ __launch_bounds__(PT) void panel_leaf_kernel(
                 }
                 if (r == j) {
 #pragma unroll
-                    for (int c = 0; c < W; c++) a[q][c] = s_piv[c];
+                    for (int c = 0; c < W; c++) a[q][c] = s_rows[gw][c];
                 }
+            }
+            // The same interchange on the block column's other columns: the
+            // lcols columns left of the leaf and the rcols right of it.
+            // Column e is always handled by thread e / G of workgroup e % G.
+            const int ne = lcols + rcols;
+            for (int e = g + G * tid; e < ne; e += G * PT) {
+                const int cc = (e < lcols) ? (e - lcols) : (wa + e - lcols);
+                T *col = A + (ptrdiff_t)cc * lda;
+                const T x = col[j], y = col[p];
+                col[j] = y;
+                col[p] = x;
             }
         }
         if (g == 0 && tid == 0) {
@@ -161,7 +179,7 @@ __global__ __launch_bounds__(PT) void panel_leaf_kernel(
             if (!nonzero) atomicMin(zero_col, col0 + j + 1);
         }
 
-        // 6. scale by the reciprocal and rank-1 update of the rows below j
+        // 7. scale by the reciprocal and rank-1 update of the rows below j
         const T rcp = nonzero ? T(1) / piv : T(0);
 #pragma unroll
         for (int q = 0; q < RPT; q++) {
@@ -173,7 +191,7 @@ __global__ __launch_bounds__(PT) void panel_leaf_kernel(
                     a[q][j] = l;
                 }
 #pragma unroll
-                for (int c = j + 1; c < W; c++) a[q][c] = fma(-l, s_piv[c], a[q][c]);
+                for (int c = j + 1; c < W; c++) a[q][c] = fma(-l, s_rows[gw][c], a[q][c]);
             }
         }
         }  // j < wa
@@ -191,15 +209,15 @@ __global__ __launch_bounds__(PT) void panel_leaf_kernel(
 }
 
 template <typename T, int W, int RPT>
-void launch_leaf_rpt(T *A, int lda, int m, int wa, int row0, int col0, int *ipiv,
-                     const PanelWork<T> &pw, unsigned base, int G, hipStream_t s)
+void launch_leaf_rpt(T *A, int lda, int m, int wa, int lcols, int rcols, int row0, int col0,
+                     int *ipiv, const PanelWork<T> &pw, unsigned base, int G, hipStream_t s)
 {
     // Only register-resident shapes are instantiated: RPT * W values of T in
     // at most 512 bytes per thread (128 VGPRs).
     if constexpr (RPT * W * sizeof(T) <= 512) {
         hipLaunchKernelGGL((panel_leaf_kernel<T, W, RPT>), dim3(G), dim3(PT), 0, s, A, lda, m,
-                           wa, row0, col0, ipiv, pw.slots, pw.ctr, base, pw.abort_flag,
-                           pw.zero_col, pw.spin_limit);
+                           wa, lcols, rcols, row0, col0, ipiv, pw.slots, pw.ctr, base,
+                           pw.abort_flag, pw.zero_col, pw.spin_limit);
         SLA_LAUNCH_CHECK();
     } else {
         throw std::logic_error("panel leaf: rows per thread exceed the register budget");
@@ -207,22 +225,25 @@ void launch_leaf_rpt(T *A, int lda, int m, int wa, int row0, int col0, int *ipiv
 }
 
 template <typename T, int W>
-void launch_leaf_w(T *A, int lda, int m, int wa, int row0, int col0, int *ipiv,
-                   const PanelWork<T> &pw, unsigned base, int G, int rpt, hipStream_t s)
+void launch_leaf_w(T *A, int lda, int m, int wa, int lcols, int rcols, int row0, int col0,
+                   int *ipiv, const PanelWork<T> &pw, unsigned base, int G, int rpt,
+                   hipStream_t s)
 {
+#define SLA_RPT(R) launch_leaf_rpt<T, W, R>(A, lda, m, wa, lcols, rcols, row0, col0, ipiv, pw, base, G, s)
     switch (rpt) {
-    case 1: launch_leaf_rpt<T, W, 1>(A, lda, m, wa, row0, col0, ipiv, pw, base, G, s); break;
-    case 2: launch_leaf_rpt<T, W, 2>(A, lda, m, wa, row0, col0, ipiv, pw, base, G, s); break;
-    case 4: launch_leaf_rpt<T, W, 4>(A, lda, m, wa, row0, col0, ipiv, pw, base, G, s); break;
-    default: launch_leaf_rpt<T, W, 8>(A, lda, m, wa, row0, col0, ipiv, pw, base, G, s); break;
+    case 1: SLA_RPT(1); break;
+    case 2: SLA_RPT(2); break;
+    case 4: SLA_RPT(4); break;
+    default: SLA_RPT(8); break;
     }
+#undef SLA_RPT
 }
 
 }  // namespace
 
 template <typename T>
-unsigned panel_leaf(T *A, int lda, int m, int wa, int row0, int col0, int *ipiv,
-                    const PanelWork<T> &pw, unsigned base, hipStream_t s)
+unsigned panel_leaf(T *A, int lda, int m, int wa, int lcols, int rcols, int row0, int col0,
+                    int *ipiv, const PanelWork<T> &pw, unsigned base, hipStream_t s)
 {
     // Rows per thread grows (1, 2, 4, 8) until the grid is within the target
     // workgroup count; the register budget caps rows/thread * W at 64
@@ -237,22 +258,22 @@ unsigned panel_leaf(T *A, int lda, int m, int wa, int row0, int col0, int *ipiv,
         rpt *= 2;
         G = ceil_div(m, PT * rpt);
     }
-    if (G > pw.slots.gmax)
-        throw std::runtime_error("panel leaf: panel too tall for the configured workgroup slots");
+    if (G > pw.slots.gmax || G > LEAF_GMAX)
+        throw std::runtime_error("panel leaf: panel too tall for the leaf's workgroup limit");
     if (wa <= 8)
-        launch_leaf_w<T, 8>(A, lda, m, wa, row0, col0, ipiv, pw, base, G, rpt, s);
+        launch_leaf_w<T, 8>(A, lda, m, wa, lcols, rcols, row0, col0, ipiv, pw, base, G, rpt, s);
     else if (wa <= 16)
-        launch_leaf_w<T, 16>(A, lda, m, wa, row0, col0, ipiv, pw, base, G, rpt, s);
+        launch_leaf_w<T, 16>(A, lda, m, wa, lcols, rcols, row0, col0, ipiv, pw, base, G, rpt, s);
     else if (wa <= 32)
-        launch_leaf_w<T, 32>(A, lda, m, wa, row0, col0, ipiv, pw, base, G, rpt, s);
+        launch_leaf_w<T, 32>(A, lda, m, wa, lcols, rcols, row0, col0, ipiv, pw, base, G, rpt, s);
     else
         throw std::runtime_error("panel leaf: width above 32");
     return (unsigned)wa * (unsigned)G;
 }
 
-template unsigned panel_leaf<double>(double *, int, int, int, int, int, int *,
+template unsigned panel_leaf<double>(double *, int, int, int, int, int, int, int, int *,
                                      const PanelWork<double> &, unsigned, hipStream_t);
-template unsigned panel_leaf<float>(float *, int, int, int, int, int, int *,
+template unsigned panel_leaf<float>(float *, int, int, int, int, int, int, int, int *,
                                     const PanelWork<float> &, unsigned, hipStream_t);
 
 }  // namespace sla

@@ -82,14 +82,31 @@ __device__ __forceinline__ void coherent_store(T *p, T v)
 // and every waiter leaves, so a launch whose workgroups cannot all be resident
 // finishes with an error flag instead of hanging the device.  Returns false
 // once the launch has been aborted.
+//
+// Wait until every vector-memory access this thread issued has completed
+// (s_waitcnt with all counters at zero).  A coherent_store is visible
+// device-wide once complete, so draining before a workgroup barrier and only
+// then raising a device-wide flag publishes the stores of every wave.  A
+// workgroup-scope fence alone does not wait for other waves' global stores.
+__device__ __forceinline__ void drain_memory() { __builtin_amdgcn_s_waitcnt(0); }
+
+// Memory ordering: everything the workgroups exchange across a barrier goes
+// through coherent_load / coherent_store (agent-scope atomics, coherent at
+// the device level without touching the per-XCD L2's other lines).  Every
+// thread drains its stores before the workgroup barrier, so the arrival
+// increment cannot overtake them; the fences only order this workgroup's
+// accesses around the counter -- workgroup scope.  An agent-scope fence would
+// write back and invalidate the whole L2 of the XCD at every barrier,
+// stalling a concurrently running GEMM.
 __device__ __forceinline__ bool grid_sync(unsigned *ctr, unsigned target, int *abort_flag,
                                           uint64_t spin_limit)
 {
     __shared__ int s_ok;
+    drain_memory();
     __syncthreads();
     if (threadIdx.x == 0) {
         int ok = 1;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");   // publish this workgroup's stores
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");   // publish this workgroup's stores
         __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const uint64_t t0 = wall_clock64();
         while (true) {
@@ -106,7 +123,7 @@ __device__ __forceinline__ bool grid_sync(unsigned *ctr, unsigned target, int *a
             }
             __builtin_amdgcn_s_sleep(1);
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // see the other workgroups' stores
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");   // see the other workgroups' stores
         s_ok = ok;
     }
     __syncthreads();

@@ -28,7 +28,9 @@ struct Handle {
 
     int device = 0;
     hipStream_t own_stream = nullptr;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;   // main stream (the caller's, or own_stream)
+    hipStream_t stream2 = nullptr;  // look-ahead panel stream (default priority)
+    hipEvent_t ev_next = nullptr, ev_panel = nullptr, ev_join = nullptr;
     uint64_t spin_limit = 0;
 
     // device control block: [0] barrier counter, [1] abort flag,
@@ -45,7 +47,8 @@ struct Handle {
     double *slot_rows = nullptr, *slot_rowj = nullptr;  // sized for doubles, reused for floats
     int slot_gmax = 0;
 
-    int *pairs = nullptr;
+    int *pairs = nullptr;        // two pair lists of 2048 ints
+    int *pair_counts = nullptr;  // their counts
     int *trsv_flags = nullptr;
     int trsv_max_blocks = 0, trsv_epoch = 0;
 
@@ -61,7 +64,7 @@ struct Handle {
     // tuning knobs
     int nb = 256;          // block column width of the outer factorization
     int leaf_w = 32;       // panel leaf width (columns per device-wide-barrier kernel)
-    int panel_wgs = 64;    // preferred workgroups per leaf launch
+    int panel_wgs = 32;    // preferred workgroups per leaf launch
 
     // timing
     bool profile = false;
@@ -70,11 +73,13 @@ struct Handle {
         int cat;
         double work;
         hipEvent_t e0, e1;
+        hipStream_t s;
+        bool closed = false;
     };
     std::vector<Pending> pending;
     std::vector<hipEvent_t> ev_pool, ev_free;
-    void timing_begin(int cat);
-    void timing_end(double work_units);
+    void timing_begin(int cat, hipStream_t s = nullptr);
+    void timing_end(double work_units, hipStream_t s = nullptr);
     void timing_collect();
     void reset_stats();
 

@@ -31,11 +31,13 @@ struct PanelWork {
 
 // Factors the m x wa leaf at A (global rows row0.., columns col0.. relative to
 // the factored matrix) in place; writes ipiv[row0 .. row0+wa-1] as 1-based
-// global rows.  Returns the number of barrier arrivals the launch adds to
-// *ctr (the caller's next `base`).
+// global rows and applies each interchange also to the lcols columns left
+// and rcols columns right of the leaf (the rest of its block column).
+// Returns the number of barrier arrivals the launch adds to *ctr (the
+// caller's next `base`).
 template <typename T>
-unsigned panel_leaf(T *A, int lda, int m, int wa, int row0, int col0, int *ipiv,
-                    const PanelWork<T> &pw, unsigned base, hipStream_t s);
+unsigned panel_leaf(T *A, int lda, int m, int wa, int lcols, int rcols, int row0, int col0,
+                    int *ipiv, const PanelWork<T> &pw, unsigned base, hipStream_t s);
 
 // ---- row interchanges (PDLASWP) --------------------------------------------
 // Net effect of the forward interchanges ipiv[k1 .. k2-1] (0-based global row

@@ -79,16 +79,17 @@ __device__ __forceinline__ T lane_read(T v, int lane)
 }
 
 // FORWARD: X = L^{-1} X with L unit lower.  Otherwise X = U^{-1} X with U
-// upper, non-unit (division by the diagonal, as dtrsm_ does).  X is n x nrhs
-// (nrhs <= 16) with leading dimension ldx; the matrix is n x n.
-template <typename T, bool FORWARD>
+// upper, non-unit, scaled by the reciprocals of its diagonal, computed before
+// any wait so they stay off the dependency chain.  X is n x nrhs
+// (nrhs <= NR <= 16) with leading dimension ldx; the matrix is n x n.
+template <typename T, bool FORWARD, int NR>
 __global__ __launch_bounds__(256) void trsv_persistent_kernel(
     int n, int nrhs, const T *__restrict__ A, int lda, T *X, int ldx, int *ticket, int *flags,
     int epoch, int *abort_flag, uint64_t spin_limit)
 {
     __shared__ int s_b, s_abort;
-    __shared__ T Xs[TV_BS][TV_NR];
-    __shared__ T Part[4][TV_BS][TV_NR + 1];
+    __shared__ T Xs[TV_BS][NR];
+    __shared__ T Part[4][TV_BS][NR + 1];
     __shared__ T Ds[TV_BS][TV_BS + 1];  // diagonal block, row-major
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nblk = ceil_div(n, TV_BS);
@@ -101,15 +102,17 @@ __global__ __launch_bounds__(256) void trsv_persistent_kernel(
     const int rb = b * TV_BS, rows = min(TV_BS, n - rb);
 
     // Diagonal block staged in LDS before any wait (it does not depend on
-    // other blocks).
+    // other blocks); wave 0 lane r keeps 1 / U(r, r).
     for (int idx = tid; idx < TV_BS * TV_BS; idx += 256) {
         const int r = idx % TV_BS, k = idx / TV_BS;
         Ds[r][k] = (r < rows && k < rows) ? A[(size_t)(rb + r) + (size_t)(rb + k) * lda] : T(0);
     }
+    T rdiag = T(1);
+    if (!FORWARD && wave == 0 && lane < rows) rdiag = T(1) / A[(size_t)(rb + lane) * (lda + 1)];
 
-    T part[TV_NR];
+    T part[NR];
 #pragma unroll
-    for (int j = 0; j < TV_NR; j++) part[j] = T(0);
+    for (int j = 0; j < NR; j++) part[j] = T(0);
 
     // Off-diagonal contributions, in dependency order.
     for (int s = 0; s < t; s++) {
@@ -130,12 +133,13 @@ __global__ __launch_bounds__(256) void trsv_persistent_kernel(
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            // X of block c is read with coherent loads only: workgroup scope
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
             s_abort = ab;
         }
         __syncthreads();
         if (s_abort) return;  // the whole workgroup leaves together
-        for (int idx = tid; idx < TV_BS * TV_NR; idx += 256) {
+        for (int idx = tid; idx < TV_BS * NR; idx += 256) {
             const int r = idx % TV_BS, j = idx / TV_BS;
             Xs[r][j] = (r < crows && j < nrhs) ? coherent_load(&X[(size_t)(cb + r) + (size_t)j * ldx]) : T(0);
         }
@@ -149,20 +153,20 @@ __global__ __launch_bounds__(256) void trsv_persistent_kernel(
                 if (k < crows) {
                     const T a = Ar[(size_t)k * lda];
 #pragma unroll
-                    for (int j = 0; j < TV_NR; j++) part[j] = fma(a, Xs[k][j], part[j]);
+                    for (int j = 0; j < NR; j++) part[j] = fma(a, Xs[k][j], part[j]);
                 }
             }
         }
         __syncthreads();  // Xs is reused by the next block
     }
 #pragma unroll
-    for (int j = 0; j < TV_NR; j++) Part[wave][lane][j] = part[j];
+    for (int j = 0; j < NR; j++) Part[wave][lane][j] = part[j];
     __syncthreads();
 
     if (wave == 0) {
-        T x[TV_NR];
+        T x[NR];
 #pragma unroll
-        for (int j = 0; j < TV_NR; j++) {
+        for (int j = 0; j < NR; j++) {
             const T rhs = (lane < rows && j < nrhs) ? X[(size_t)(rb + lane) + (size_t)j * ldx] : T(0);
             const T s = ((Part[0][lane][j] + Part[1][lane][j]) + Part[2][lane][j]) + Part[3][lane][j];
             x[j] = rhs - s;
@@ -171,7 +175,7 @@ __global__ __launch_bounds__(256) void trsv_persistent_kernel(
             for (int i = 0; i < rows; i++) {
                 const T l = Ds[lane][i];
 #pragma unroll
-                for (int j = 0; j < TV_NR; j++) {
+                for (int j = 0; j < NR; j++) {
                     const T xi = lane_read(x[j], i);
                     if (lane > i) x[j] = fma(-l, xi, x[j]);
                 }
@@ -180,8 +184,8 @@ __global__ __launch_bounds__(256) void trsv_persistent_kernel(
             for (int i = rows - 1; i >= 0; i--) {
                 const T u = Ds[lane][i];
 #pragma unroll
-                for (int j = 0; j < TV_NR; j++) {
-                    if (lane == i) x[j] = x[j] / u;
+                for (int j = 0; j < NR; j++) {
+                    if (lane == i) x[j] = x[j] * rdiag;
                     const T xi = lane_read(x[j], i);
                     if (lane < i) x[j] = fma(-u, xi, x[j]);
                 }
@@ -189,13 +193,15 @@ __global__ __launch_bounds__(256) void trsv_persistent_kernel(
         }
         if (lane < rows) {
 #pragma unroll
-            for (int j = 0; j < TV_NR; j++)
+            for (int j = 0; j < NR; j++)
                 if (j < nrhs) coherent_store(&X[(size_t)(rb + lane) + (size_t)j * ldx], x[j]);
         }
     }
+    drain_memory();  // X[b]'s coherent stores complete before the flag rises
     __syncthreads();
     if (tid == 0) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        // X[b] was written with coherent stores; order them before the flag
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __hip_atomic_store(&flags[b], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
@@ -224,12 +230,22 @@ void trsv_persistent(bool forward, int n, int nrhs, const T *A, int lda, T *X, i
     if (nblk > tw.max_blocks) throw std::logic_error("trsv_persistent: flag array too small");
     fill_int(tw.ticket, 1, 0, s);
     const int epoch = ++tw.epoch;
-    if (forward)
-        hipLaunchKernelGGL((trsv_persistent_kernel<T, true>), dim3(nblk), dim3(256), 0, s, n, nrhs,
-                           A, lda, X, ldx, tw.ticket, tw.flags, epoch, tw.abort_flag, tw.spin_limit);
-    else
-        hipLaunchKernelGGL((trsv_persistent_kernel<T, false>), dim3(nblk), dim3(256), 0, s, n, nrhs,
-                           A, lda, X, ldx, tw.ticket, tw.flags, epoch, tw.abort_flag, tw.spin_limit);
+#define SLA_TRSV(FW, NRV)                                                                       \
+    hipLaunchKernelGGL((trsv_persistent_kernel<T, FW, NRV>), dim3(nblk), dim3(256), 0, s, n,    \
+                       nrhs, A, lda, X, ldx, tw.ticket, tw.flags, epoch, tw.abort_flag,        \
+                       tw.spin_limit)
+#define SLA_TRSV_NR(FW)                           \
+    do {                                          \
+        if (nrhs <= 1) SLA_TRSV(FW, 1);           \
+        else if (nrhs <= 2) SLA_TRSV(FW, 2);      \
+        else if (nrhs <= 4) SLA_TRSV(FW, 4);      \
+        else if (nrhs <= 8) SLA_TRSV(FW, 8);      \
+        else SLA_TRSV(FW, 16);                    \
+    } while (0)
+    if (forward) SLA_TRSV_NR(true);
+    else SLA_TRSV_NR(false);
+#undef SLA_TRSV_NR
+#undef SLA_TRSV
     SLA_LAUNCH_CHECK();
 }
 template void trsv_persistent<double>(bool, int, int, const double *, int, double *, int, TrsvWork &, hipStream_t);
