@@ -1,21 +1,31 @@
 """Benchmark of the dense LU solve path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 32768] [--nb 256]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 32768]
+                    [--mode mixed|fp64] [--kappas 1e2,1e4,1e6,1e8] [--nb 512]
 
-One step = one solve A x = b of the N x N system already resident in HBM:
-PDGETRF + PDGETRS (sla_dgesv), ending at FP64 backward error (the tester's
-scaled residual ||b - A x||_inf / (||A||_inf ||x||_inf N eps) < 16 is
-checked after the timed steps).  The factorization overwrites A, so every
-step restores A from a pristine copy first; the restore is outside the timed
-region, which brackets exactly the solve with device synchronisation.
+One step = one solve A x = b of an N x N system already resident in HBM,
+ending at FP64 backward error: the LU testers' scaled residual
+||b - A x||_inf / (||A||_inf ||x||_inf N eps) < 16 (eps = 2^-53) is checked
+for every timed solve.  The solve never reads a cached result: every step
+starts from the pristine A (restored outside the timed region where the
+solve overwrites it) and does the whole factorization + solve (+ refinement).
 
-value = (2/3 N^3) / (mean seconds per solve) / 1e12   [TFLOPS-equivalent]
+--mode mixed (default; the headline): sla_dxgesv -- an FP32-storage LU whose
+    trailing updates run on FP32 MFMA, refined with FP64 residuals to FP64
+    backward error (classical refinement, continuing as GMRES-IR on a stall;
+    GMRES-IR from the start for kappa >= 1e6).  Matrices: A = U diag(sigma)
+    V^T, U and V the Q factors of seeded Gaussian N x N matrices, sigma
+    geometric from 1 to 1/kappa, one per kappa; b = A x_true, x_true seeded
+    Gaussian.  value = total 2/3 N^3 over the kappa set / total solve time.
+    The plain FP64 LU (sla_dgesv) is timed on the first matrix beside it.
+--mode fp64: the plain FP64 PDGETRF + PDGETRS drop-in path (sla_dgesv) on A
+    i.i.d. uniform[-1, 1), b = A * ones.
 
 The line also carries
-  roofline      the trailing-update GEMM (the dominant kernel): algorithmic
-                FP64 flops of all its launches / their summed duration (HIP
-                events around each launch on the library's stream, recorded
-                during the timed steps) against the MI355X FP64 matrix peak;
+  roofline      the dominant kernel of the timed solves (most summed time):
+                algorithmic work of all its launches / their summed duration
+                (HIP events around each launch on the stream it runs on)
+                against the MI355X peak of its unit;
   cpu_baseline  the reference ScaLAPACK PDGETRF + PDGETRS (oracle/_ref,
                 built from the reference sources, MKL BLAS) on the host cores
                 for a bounded N = 16384 sample, same rate definition.
@@ -32,7 +42,10 @@ import time
 from pathlib import Path
 
 ROOT = Path(__file__).resolve().parent
-FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 matrix peak
+PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0, "fp8": 5000.0}  # dense matrix TFLOP/s
+HBM_PEAK_GBS = 8000.0
+EPS = 2.0 ** -53
+METRIC = "2N^3/3 TFLOPS-equiv at FP64 backward error: mixed-prec LU+IR vs FP64 LU, N=32k"
 
 
 def parse():
@@ -42,9 +55,16 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=32768)
     ap.add_argument("--nrhs", type=int, default=1)
-    ap.add_argument("--nb", type=int, default=256)
+    ap.add_argument("--nb", type=int, default=512)
+    ap.add_argument("--mode", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--prec", choices=["fp32", "bf16", "fp8"], default="fp32",
+                    help="factorization precision of the mixed solve")
+    ap.add_argument("--kappas", default="1e2,1e4,1e6,1e8")
+    ap.add_argument("--gmres-from", type=float, default=1e6,
+                    help="kappa from which the mixed solve starts with GMRES-IR")
     ap.add_argument("--cpu-n", type=int, default=16384, help="CPU baseline sample size")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-fp64", action="store_true", help="mixed mode: skip the FP64 comparison")
     return ap.parse_args()
 
 
@@ -69,10 +89,118 @@ def cpu_baseline(n: int) -> dict | None:
     t = r["t_getrf"] + r["t_getrs"]
     return {"value": round((2.0 / 3.0) * n ** 3 / t / 1e12, 4), "unit": "TFLOPS",
             "cores": cores, "kind": "reference",
-            "sample": f"reference PDGETRF+PDGETRS (oracle/_ref, MKL, {cores} threads, 1x1 grid) "
-                      f"on N={n}, NB=256, NRHS=1, PDMATGEN seed 100, b=A*ones; "
+            "sample": f"reference PDGETRF+PDGETRS in FP64 (oracle/_ref, MKL, {cores} threads, "
+                      f"1x1 grid) on N={n}, NB=256, NRHS=1, PDMATGEN seed 100, b=A*ones; "
                       f"t_getrf={r['t_getrf']:.2f}s t_getrs={r['t_getrs']:.3f}s "
                       f"scaled residual {r['sresid']:.2e}"}
+
+
+def scaled_residual(torch, AT, B0, X) -> float:
+    """AT: storage (n, n) of column-major A (AT = A^T); B0, X: (nrhs, n)."""
+    n = AT.shape[0]
+    R = B0.T - AT.T @ X.T
+    anorm = AT.abs().sum(dim=0).max().item()  # ||A||_inf: max row sum
+    return (R.abs().amax(dim=0) / (X.T.abs().amax(dim=0) * anorm * EPS * n)).max().item()
+
+
+def svd_matrices(torch, n: int, kappas, seed: int = 100):
+    """Storage of A = U diag(sigma) V^T (column-major: the (n, n) tensor
+    A^T = V diag(sigma) U^T) for each kappa; U, V from QR of seeded
+    Gaussians, shared by the kappas."""
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    U, _ = torch.linalg.qr(torch.randn((n, n), generator=g, device=dev, dtype=torch.float64))
+    V, _ = torch.linalg.qr(torch.randn((n, n), generator=g, device=dev, dtype=torch.float64))
+    idx = torch.arange(n, device=dev, dtype=torch.float64) / (n - 1)
+    for kappa in kappas:
+        sigma = torch.pow(torch.tensor(float(kappa), dtype=torch.float64, device=dev), -idx)
+        yield kappa, (V * sigma) @ U.T
+        torch.cuda.synchronize()
+
+
+def roofline_from(stats: dict, steps: int, gemm_prec: str) -> dict:
+    """Roofline of the kernel class with the most summed time."""
+    cls, st = max(stats.items(), key=lambda kv: kv[1]["ms"])
+    secs = st["ms"] / 1e3
+    base = {"kernel_class": cls, "launches_per_step": st["launches"] // max(1, steps),
+            "avg_launch_ms": round(st["ms"] / max(1, st["launches"]), 4)}
+    if cls in ("gemm", "trsm", "panel"):
+        # flops in the GEMM's product precision; panel and trsm compute in the
+        # storage precision (FP64, or FP32 for every mixed factorization)
+        achieved = st["work"] / secs / 1e12 if secs > 0 else 0.0
+        if cls == "gemm":
+            peak = PEAK[gemm_prec]
+        else:
+            peak = PEAK["fp64" if gemm_prec == "fp64" else "fp32"]
+        out = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
+               "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None}
+    else:
+        achieved = st["work"] / secs / 1e9 if secs > 0 else 0.0
+        out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+               "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None}
+    out.update(base)
+    return out
+
+
+def run_fp64(torch, h, AT0, B0, steps, warmup):
+    n, nrhs = AT0.shape[0], B0.shape[0]
+    AT = torch.empty_like(AT0)
+    B = torch.empty_like(B0)
+    ipiv = torch.zeros(n, dtype=torch.int32, device=AT0.device)
+    for _ in range(warmup):
+        AT.copy_(AT0)
+        B.copy_(B0)
+        torch.cuda.synchronize()
+        h.dgesv(n, nrhs, AT, n, ipiv, B, n)
+    torch.cuda.synchronize()
+    h.set("profile", 1)
+    h.reset_stats()
+    times, sres = [], []
+    for _ in range(steps):
+        AT.copy_(AT0)
+        B.copy_(B0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        info = h.dgesv(n, nrhs, AT, n, ipiv, B, n)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        assert info == 0, f"zero pivot at {info}"
+        sres.append(scaled_residual(torch, AT0, B0, B))
+    stats = h.stats()
+    h.set("profile", 0)
+    return times, sres, stats
+
+
+def run_mixed(torch, h, AT, B0, steps, warmup, prec, refine):
+    n, nrhs = AT.shape[0], B0.shape[0]
+    X = torch.zeros_like(B0)
+    ipiv = torch.zeros(n, dtype=torch.int32, device=AT.device)
+    for _ in range(warmup):
+        it, _ = h.dxgesv(n, nrhs, AT, n, ipiv, B0, n, X, n, prec=prec, refine=refine)
+        if it < 0:
+            raise RuntimeError(f"mixed solve fell back to FP64 (ITER {it}) in the warmup")
+    torch.cuda.synchronize()
+    h.set("profile", 1)
+    h.reset_stats()
+    times, sres, infos = [], [], []
+    for _ in range(steps):
+        X.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        it, info = h.dxgesv(n, nrhs, AT, n, ipiv, B0, n, X, n, prec=prec, refine=refine)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        assert info == 0, f"zero pivot at {info}"
+        st = h.last_solve_info()
+        infos.append(st)
+        sres.append(scaled_residual(torch, AT, B0, X))
+        if it < 0:
+            raise RuntimeError(f"mixed solve fell back to FP64 (ITER {it}); A was factored "
+                               f"in place, so the next step would not be a mixed solve")
+    stats = h.stats()
+    h.set("profile", 0)
+    return times, sres, stats, infos
 
 
 def main() -> int:
@@ -88,87 +216,94 @@ def main() -> int:
     n, nrhs, nb = args.n, args.nrhs, args.nb
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
-    g = torch.Generator(device=dev)
-    g.manual_seed(100)
-    # column-major n x n: storage (n cols, n rows); A i.i.d. uniform[-1, 1)
-    A0 = torch.rand((n, n), generator=g, device=dev, dtype=torch.float64).mul_(2.0).sub_(1.0)
-    B0 = A0.sum(dim=0, keepdim=True).expand(nrhs, n).contiguous()  # b = A * ones
-    A = torch.empty_like(A0)
-    B = torch.empty_like(B0)
-    ipiv = torch.zeros(n, dtype=torch.int32, device=dev)
-
     h = Handle(nb=nb)
-    stream = torch.cuda.current_stream()
-    h.set_stream(stream)
+    h.set_stream(torch.cuda.current_stream())
+    line = {"metric": METRIC, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None}
+    flops = (2.0 / 3.0) * n ** 3
 
-    def step():
-        return h.dgesv(n, nrhs, A, n, ipiv, B, n)
-
-    for _ in range(args.warmup):
-        A.copy_(A0)
-        B.copy_(B0)
-        torch.cuda.synchronize()
-        step()
-    torch.cuda.synchronize()
-
-    h.set("profile", 1)
-    h.reset_stats()
-    times = []
-    for _ in range(args.steps):
-        A.copy_(A0)
-        B.copy_(B0)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        info = step()
-        torch.cuda.synchronize()
-        times.append(time.perf_counter() - t0)
-        assert info == 0, f"zero pivot at {info}"
-    stats = h.stats()
-    h.set("profile", 0)
-
-    # backward error of the last solve (FP64 on the GPU)
-    X = B
-    R = B0.T - A0.T @ X.T
-    anorm = A0.abs().sum(dim=0).max().item()       # ||A||_inf: max row sum
-    eps = 2.0 ** -53
-    sres = (R.abs().amax(dim=0) / (X.T.abs().amax(dim=0) * anorm * eps * n)).max().item()
-
-    ms = 1e3 * sum(times) / len(times)
-    value = (2.0 / 3.0) * n ** 3 / (ms / 1e3) / 1e12
-    gm = stats["gemm"]
-    gemm_tflops = gm["work"] / (gm["ms"] / 1e3) / 1e12 if gm["ms"] > 0 else 0.0
-    roof = {"bound": "mfma", "achieved": round(gemm_tflops, 3), "peak": FP64_PEAK_TFLOPS,
-            "unit": "TFLOP/s", "frac": round(gemm_tflops / FP64_PEAK_TFLOPS, 4), "traffic": None,
-            "kernel": "dgemm_nn_sub_kernel (trailing update A22 -= L21 U12)",
-            "launches_per_step": gm["launches"] // max(1, args.steps),
-            "avg_launch_ms": round(gm["ms"] / max(1, gm["launches"]), 4),
-            "avg_launch_gflop": round(gm["work"] / max(1, gm["launches"]) / 1e9, 3)}
-    breakdown = {k: round(v["ms"] / args.steps, 2) for k, v in stats.items()}
-
-    cpu = None if args.no_cpu else cpu_baseline(args.cpu_n)
-    line = {
-        "metric": "2N^3/3 TFLOPS-equiv at FP64 backward error: mixed-prec LU+IR vs FP64 LU, N=32k",
-        "value": round(value, 3),
-        "unit": "TFLOPS",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms, 2),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic: A i.i.d. uniform[-1,1) (torch seed 100), b = A*ones, resident in HBM",
-        "config": {"workload": f"dense LU solve PDGESV (1x1 grid), N={n}, NRHS={nrhs}, NB={nb}, "
-                               f"plain FP64 path", "n": n, "nrhs": nrhs, "nb": nb,
-                   "precision": "fp64"},
-        "scaled_residual": sres,
-        "kernel_ms_per_step": breakdown,
-        "roofline": roof,
-        "cpu_baseline": cpu,
-    }
+    if args.mode == "fp64":
+        g = torch.Generator(device=dev)
+        g.manual_seed(100)
+        AT0 = torch.rand((n, n), generator=g, device=dev, dtype=torch.float64).mul_(2.0).sub_(1.0)
+        B0 = AT0.sum(dim=0, keepdim=True).expand(nrhs, n).contiguous()  # b = A * ones
+        times, sres, stats = run_fp64(torch, h, AT0, B0, args.steps, args.warmup)
+        ms = 1e3 * sum(times) / len(times)
+        line.update({
+            "value": round(flops / (ms / 1e3) / 1e12, 3), "unit": "TFLOPS",
+            "ms_per_step": round(ms, 2), "dtype": "f64",
+            "data": "synthetic: A i.i.d. uniform[-1,1) (torch seed 100), b = A*ones, in HBM",
+            "config": {"workload": f"plain FP64 LU solve PDGESV (1x1 grid), N={n}, NRHS={nrhs}, "
+                                   f"NB={nb}", "n": n, "nrhs": nrhs, "nb": nb,
+                       "precision": "fp64"},
+            "scaled_residual": max(sres),
+            "kernel_ms_per_step": {k: round(v["ms"] / args.steps, 2) for k, v in stats.items()},
+            "roofline": roofline_from(stats, args.steps, "fp64"),
+        })
+        ok = max(sres) < 16.0
+    else:
+        kappas = [float(k) for k in args.kappas.split(",")]
+        t_gen = time.perf_counter()
+        per_kappa, all_stats, total_t, total_steps, ok = [], {}, 0.0, 0, True
+        fp64_line = None
+        rng = torch.Generator(device=dev)
+        rng.manual_seed(200)
+        XT = torch.randn((nrhs, n), generator=rng, device=dev, dtype=torch.float64)
+        for kappa, AT in svd_matrices(torch, n, kappas):
+            gen_s = time.perf_counter() - t_gen
+            B0 = (AT.T @ XT.T).T.contiguous()  # b = A x_true
+            refine = "gmres" if kappa >= args.gmres_from else "ir"
+            times, sres, stats, infos = run_mixed(torch, h, AT, B0, args.steps, args.warmup,
+                                                  args.prec, refine)
+            for k, v in stats.items():
+                a = all_stats.setdefault(k, {"ms": 0.0, "work": 0.0, "launches": 0})
+                for f in a:
+                    a[f] += v[f]
+            total_t += sum(times)
+            total_steps += len(times)
+            ok &= max(sres) < 16.0
+            per_kappa.append({"kappa": kappa, "refine": refine,
+                              "ms_per_solve": round(1e3 * sum(times) / len(times), 2),
+                              "tflops_equiv": round(flops / (sum(times) / len(times)) / 1e12, 3),
+                              "scaled_residual": max(sres), "outer": infos[-1]["outer"],
+                              "gmres_steps": infos[-1]["inner"],
+                              "refinement_used": infos[-1]["refinement"],
+                              "kernel_ms_per_step": {k: round(v["ms"] / args.steps, 2)
+                                                     for k, v in stats.items()}})
+            if fp64_line is None and not args.no_fp64:
+                ft, fs, fstats = run_fp64(torch, h, AT, B0, args.steps, args.warmup)
+                fms = 1e3 * sum(ft) / len(ft)
+                fp64_line = {"kappa": kappa, "ms_per_solve": round(fms, 2),
+                             "tflops_equiv": round(flops / (fms / 1e3) / 1e12, 3),
+                             "scaled_residual": max(fs),
+                             "kernel_ms_per_step": {k: round(v["ms"] / args.steps, 2)
+                                                    for k, v in fstats.items()},
+                             "gemm_roofline": roofline_from({"gemm": fstats["gemm"]},
+                                                            args.steps, "fp64")}
+            del AT, B0
+            t_gen = time.perf_counter()
+        ms = 1e3 * total_t / total_steps
+        value = flops / (ms / 1e3) / 1e12
+        line.update({
+            "value": round(value, 3), "unit": "TFLOPS", "ms_per_step": round(ms, 2),
+            "dtype": args.prec if args.prec != "fp32" else "f32",
+            "data": "synthetic: A = U diag(sigma) V^T, U,V = Q of seeded Gaussian N x N "
+                    "(torch seed 100), sigma geometric 1..1/kappa; b = A x_true (seed 200); "
+                    "in HBM",
+            "config": {"workload": f"mixed-precision LU + refinement to FP64 backward error, "
+                                   f"N={n}, NRHS={nrhs}, NB={nb}, factorization {args.prec} "
+                                   f"(FP32 accumulation), kappa in {args.kappas}",
+                       "n": n, "nrhs": nrhs, "nb": nb, "precision": args.prec,
+                       "kappas": kappas},
+            "per_kappa": per_kappa,
+            "fp64_lu": fp64_line,
+            "speedup_vs_fp64_lu": (round(value / fp64_line["tflops_equiv"], 3)
+                                   if fp64_line else None),
+            "roofline": roofline_from(all_stats, total_steps, args.prec),
+        })
+    line["cpu_baseline"] = None if args.no_cpu else cpu_baseline(args.cpu_n)
     print(json.dumps(line), flush=True)
-    return 0 if sres < 16.0 else 1
+    return 0 if ok else 1
 
 
 if __name__ == "__main__":

@@ -84,7 +84,10 @@ int sla_destroy(sla_handle_t h);
  * handle's own stream).  Calls return after the stream has drained. */
 int sla_set_stream(sla_handle_t h, void *stream);
 /* Tuning knobs: "nb" (block column width), "leaf_w" (panel leaf width,
- * <= 32), "panel_wgs" (workgroups per panel leaf), "profile" (0/1). */
+ * <= 32), "panel_wgs" (workgroups per panel leaf), "profile" (0/1); for the
+ * mixed-precision solve "ir_max_iter" (default 30), "ir_gmres_on_stall"
+ * (0/1, default 1), "gmres_m" (GMRES steps per correction, default 50),
+ * "gmres_tol_exp" (e: relative reduction 10^-e per correction, default 6). */
 int sla_set_param(sla_handle_t h, const char *name, int value);
 int sla_get_param(sla_handle_t h, const char *name, int *value);
 const char *sla_last_error(void);
@@ -111,6 +114,56 @@ int sla_dgesv(sla_handle_t h, int n, int nrhs, double *dA, int lda, int *dipiv, 
  * 333-352).  For kernel-level checks and measurement. */
 int sla_dgemm_sub(sla_handle_t h, int m, int n, int k, const double *dA, int lda,
                   const double *dB, int ldb, double *dC, int ldc);
+
+/* ------------------------------------------------------------------------ */
+/* Part 3: mixed precision (no counterpart in the reference; the contract is  */
+/* that of LAPACK's DSGESV, the mixed-precision form of PDGESV's solve).      */
+/* ------------------------------------------------------------------------ */
+
+/* Factorization precisions: FP64 (plain), FP32 storage with FP32, BF16 or
+ * FP8-E4M3 (per-block-column scales) trailing updates, FP32 accumulation. */
+#define SLA_PREC_FP64 0
+#define SLA_PREC_FP32 1
+#define SLA_PREC_BF16 2
+#define SLA_PREC_FP8 3
+/* Refinement: classical iterative refinement, or GMRES-IR (flexible GMRES on
+ * A d = r preconditioned by the low-precision factors). */
+#define SLA_REFINE_IR 0
+#define SLA_REFINE_GMRES 1
+
+typedef struct {
+    int precision;   /* factorization precision asked for (SLA_PREC_*) */
+    int refinement;  /* refinement that produced the result (SLA_REFINE_*) */
+    int iter;        /* DSGESV's ITER: refinement steps; < 0 after FP64 fallback */
+    int outer;       /* FP64 residuals computed after the first solve */
+    int inner;       /* GMRES steps in total */
+    int fell_back;   /* 1 if the FP64 factorization produced X */
+    double anorm;    /* ||A||_inf */
+    double bwd_err;  /* max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf) */
+    double sresid;   /* bwd_err / (N eps), eps = 2^-53: the testers' scaled residual */
+} sla_solve_info_t;
+
+/* X = A^{-1} B (n x n FP64 dA, nrhs <= 16, device pointers) by a
+ * factorization in precision `prec` and refinement `refine` to FP64 backward
+ * error: ||r||_inf <= ||x||_inf ||A||_inf sqrt(n) eps per column, or a stall
+ * below the scaled-residual threshold 16.  dA is unchanged unless the FP64
+ * fallback ran (then it holds L\U and dipiv the pivots, as sla_dgetrf);
+ * dB is not modified.  *iter as DSGESV's ITER (-2 overflow in FP32, -3 zero
+ * pivot in low precision, -k no convergence after k-1 steps, -1 prec was
+ * FP64); *info as PDGESV's INFO. */
+int sla_dxgesv(sla_handle_t h, int prec, int refine, int n, int nrhs, double *dA, int lda,
+               int *dipiv, const double *dB, int ldb, double *dX, int ldx, int *iter, int *info);
+/* Details of the last sla_dxgesv on this handle. */
+int sla_last_solve_info(sla_handle_t h, sla_solve_info_t *out);
+
+/* FP32-storage LU (the low-precision factorization alone), trailing updates
+ * in `prec` (SLA_PREC_FP32, _BF16 or _FP8).  Pivots and *info as sla_dgetrf. */
+int sla_sgetrf(sla_handle_t h, int prec, int m, int n, float *dA, int lda, int *dipiv,
+               int *info);
+/* The low-precision trailing-update kernel alone: C -= A * B (FP32
+ * column-major operands, products in `prec`, FP32 accumulation). */
+int sla_gemm_lowp_sub(sla_handle_t h, int prec, int m, int n, int k, const float *dA, int lda,
+                      const float *dB, int ldb, float *dC, int ldc);
 
 #ifdef __cplusplus
 }

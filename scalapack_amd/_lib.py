@@ -21,6 +21,17 @@ class LibraryMissing(RuntimeError):
     pass
 
 
+class SolveInfo(ctypes.Structure):
+    """sla_solve_info_t of include/scalapack_amd.h."""
+    _fields_ = [("precision", ctypes.c_int), ("refinement", ctypes.c_int),
+                ("iter", ctypes.c_int), ("outer", ctypes.c_int), ("inner", ctypes.c_int),
+                ("fell_back", ctypes.c_int), ("anorm", ctypes.c_double),
+                ("bwd_err", ctypes.c_double), ("sresid", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _lib = None
 _shim = None
 
@@ -45,6 +56,10 @@ def _declare(lib: ctypes.CDLL) -> None:
         "sla_dgetrs": [vp, i, i, vp, i, vp, vp, i],
         "sla_dgesv": [vp, i, i, vp, i, vp, vp, i, vp],
         "sla_dgemm_sub": [vp, i, i, i, vp, i, vp, i, vp, i],
+        "sla_dxgesv": [vp, i, i, i, i, vp, i, vp, vp, i, vp, i, vp, vp],
+        "sla_last_solve_info": [vp, vp],
+        "sla_sgetrf": [vp, i, i, i, vp, i, vp, vp],
+        "sla_gemm_lowp_sub": [vp, i, i, i, i, vp, i, vp, i, vp, i],
     }
     for name, args in sigs.items():
         f = getattr(lib, name)

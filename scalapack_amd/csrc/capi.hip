@@ -11,6 +11,7 @@
 #include "sla_common.h"
 #include "sla_handle.h"
 #include "sla_kernels.h"
+#include "sla_mixed.h"
 
 #include "scalapack_amd.h"
 #include "sla_blacs1x1.h"
@@ -19,6 +20,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -499,7 +501,8 @@ int sla_set_param(sla_handle_t h, const char *name, int value)
 {
     auto *H = reinterpret_cast<sla::Handle *>(h);
     const std::string k(name ? name : "");
-    static const char *known[] = {"nb", "leaf_w", "panel_wgs", "profile"};
+    static const char *known[] = {"nb",      "leaf_w",  "panel_wgs",        "profile",
+                                  "ir_max_iter", "gmres_m", "ir_gmres_on_stall", "gmres_tol_exp"};
     if (std::find_if(std::begin(known), std::end(known), [&](const char *x) { return k == x; }) ==
         std::end(known)) {
         g_last_error = "sla_set_param: unknown parameter " + k;
@@ -510,6 +513,10 @@ int sla_set_param(sla_handle_t h, const char *name, int value)
     else if (k == "leaf_w" && value >= 1 && value <= 32) H->leaf_w = value;
     else if (k == "panel_wgs" && value >= 1 && value <= 256) H->panel_wgs = value;
     else if (k == "profile") H->profile = value != 0;
+    else if (k == "ir_max_iter" && value >= 0 && value <= 1000) H->ir_max_iter = value;
+    else if (k == "gmres_m" && value >= 1 && value <= 1000) H->gmres_m = value;
+    else if (k == "ir_gmres_on_stall") H->ir_gmres_on_stall = value != 0;
+    else if (k == "gmres_tol_exp" && value >= 1 && value <= 15) H->gmres_tol = std::pow(10.0, -value);
     else {
         g_last_error = "sla_set_param: value out of range for " + k;
         return SLA_ERR_ARG;
@@ -526,6 +533,10 @@ int sla_get_param(sla_handle_t h, const char *name, int *value)
     else if (k == "leaf_w") *value = H->leaf_w;
     else if (k == "panel_wgs") *value = H->panel_wgs;
     else if (k == "profile") *value = H->profile ? 1 : 0;
+    else if (k == "ir_max_iter") *value = H->ir_max_iter;
+    else if (k == "gmres_m") *value = H->gmres_m;
+    else if (k == "ir_gmres_on_stall") *value = H->ir_gmres_on_stall;
+    else if (k == "gmres_tol_exp") *value = (int)std::lround(-std::log10(H->gmres_tol));
     else {
         g_last_error = "sla_get_param: unknown parameter " + k;
         return SLA_ERR_ARG;
@@ -605,6 +616,119 @@ int sla_dgesv(sla_handle_t h, int n, int nrhs, double *dA, int lda, int *dipiv, 
         auto &H = *reinterpret_cast<sla::Handle *>(h);
         *info = sla::dgetrf_device(H, n, n, dA, lda, dipiv);
         if (*info == 0) sla::dgetrs_device(H, n, nrhs, dA, lda, dipiv, dB, ldb);
+    })
+}
+
+int sla_dxgesv(sla_handle_t h, int prec, int refine, int n, int nrhs, double *dA, int lda,
+               int *dipiv, const double *dB, int ldb, double *dX, int ldx, int *iter, int *info)
+{
+    SLA_NEED_HANDLE(h);
+    if (prec < SLA_PREC_FP64 || prec > SLA_PREC_FP8 || refine < SLA_REFINE_IR ||
+        refine > SLA_REFINE_GMRES) {
+        g_last_error = "sla_dxgesv: unknown precision or refinement";
+        return SLA_ERR_ARG;
+    }
+    if (n < 0 || nrhs < 0 || nrhs > 16 || lda < std::max(1, n) || ldb < std::max(1, n) ||
+        ldx < std::max(1, n)) {
+        g_last_error = "sla_dxgesv: illegal n, nrhs (<= 16), lda, ldb or ldx";
+        return SLA_ERR_ARG;
+    }
+    SLA_TRY({
+        auto &H = *reinterpret_cast<sla::Handle *>(h);
+        sla::MixedInfo mi;
+        *info = sla::dxgesv_device(H, prec, refine, n, nrhs, dA, lda, dipiv, dB, ldb, dX, ldx, mi);
+        *iter = mi.iter;
+        H.last_mixed = mi;
+    })
+}
+
+int sla_last_solve_info(sla_handle_t h, sla_solve_info_t *out)
+{
+    SLA_NEED_HANDLE(h);
+    if (!out) {
+        g_last_error = "sla_last_solve_info: null output";
+        return SLA_ERR_ARG;
+    }
+    const sla::MixedInfo &mi = reinterpret_cast<sla::Handle *>(h)->last_mixed;
+    out->precision = mi.precision;
+    out->refinement = mi.refinement;
+    out->iter = mi.iter;
+    out->outer = mi.outer;
+    out->inner = mi.inner;
+    out->fell_back = mi.fell_back;
+    out->anorm = mi.anorm;
+    out->bwd_err = mi.bwd_err;
+    out->sresid = mi.sresid;
+    return SLA_OK;
+}
+
+static int schur_of(int prec)
+{
+    return prec == SLA_PREC_BF16 ? sla::SCHUR_BF16 : prec == SLA_PREC_FP8 ? sla::SCHUR_FP8
+                                                                          : sla::SCHUR_NATIVE;
+}
+
+int sla_sgetrf(sla_handle_t h, int prec, int m, int n, float *dA, int lda, int *dipiv, int *info)
+{
+    SLA_NEED_HANDLE(h);
+    if (prec < SLA_PREC_FP32 || prec > SLA_PREC_FP8) {
+        g_last_error = "sla_sgetrf: precision must be FP32, BF16 or FP8";
+        return SLA_ERR_ARG;
+    }
+    if (m < 0 || n < 0 || lda < std::max(1, m)) {
+        g_last_error = "sla_sgetrf: illegal m, n or lda";
+        return SLA_ERR_ARG;
+    }
+    SLA_TRY(*info = sla::sgetrf_device(*reinterpret_cast<sla::Handle *>(h), m, n, dA, lda, dipiv,
+                                       schur_of(prec)))
+}
+
+int sla_gemm_lowp_sub(sla_handle_t h, int prec, int m, int n, int k, const float *dA, int lda,
+                      const float *dB, int ldb, float *dC, int ldc)
+{
+    SLA_NEED_HANDLE(h);
+    if (prec < SLA_PREC_FP32 || prec > SLA_PREC_FP8) {
+        g_last_error = "sla_gemm_lowp_sub: precision must be FP32, BF16 or FP8";
+        return SLA_ERR_ARG;
+    }
+    if (m < 0 || n < 0 || k < 0 || lda < std::max(1, m) || ldb < std::max(1, k) ||
+        ldc < std::max(1, m)) {
+        g_last_error = "sla_gemm_lowp_sub: illegal dimensions";
+        return SLA_ERR_ARG;
+    }
+    SLA_TRY({
+        auto &H = *reinterpret_cast<sla::Handle *>(h);
+        hipStream_t s = H.stream;
+        if (m > 0 && n > 0 && k > 0) {
+            H.timing_begin(sla::CAT_GEMM);
+            if (prec == SLA_PREC_FP32) {
+                sla::sgemm_nn_sub(m, n, k, dA, lda, dB, ldb, dC, ldc, s);
+            } else {
+                const int kq = prec == SLA_PREC_BF16 ? 32 : 128;
+                const int kpad = (k + kq - 1) / kq * kq;
+                const size_t esz = prec == SLA_PREC_BF16 ? 2 : 1;
+                void *lp = H.buffer(sla::BUF_PACK_L, (size_t)m * kpad * esz);
+                void *up = H.buffer(sla::BUF_PACK_U, (size_t)n * kpad * esz);
+                float *sc = static_cast<float *>(H.buffer(sla::BUF_SCALES, 64 * sizeof(float)));
+                if (prec == SLA_PREC_BF16) {
+                    sla::pack_rows_bf16(m, k, kpad, dA, lda, static_cast<__bf16 *>(lp), s);
+                    sla::pack_cols_bf16(k, kpad, n, dB, ldb, static_cast<__bf16 *>(up), s);
+                    sla::gemm_bf16_nn_sub(m, n, kpad, static_cast<__bf16 *>(lp), kpad,
+                                          static_cast<__bf16 *>(up), kpad, dC, ldc, s);
+                } else {
+                    sla::fill_u32(reinterpret_cast<unsigned *>(sc), 2, 0u, s);
+                    sla::absmax_scale(m, k, dA, lda, sc, s);
+                    sla::absmax_scale(k, n, dB, ldb, sc + 1, s);
+                    sla::pack_rows_fp8(m, k, kpad, dA, lda, sc, static_cast<uint8_t *>(lp), s);
+                    sla::pack_cols_fp8(k, kpad, n, dB, ldb, sc + 1, static_cast<uint8_t *>(up), s);
+                    sla::gemm_fp8_nn_sub(m, n, kpad, static_cast<uint8_t *>(lp), kpad,
+                                         static_cast<uint8_t *>(up), kpad, sc, dC, ldc, s);
+                }
+            }
+            H.timing_end(2.0 * m * n * k);
+        }
+        SLA_HIP(hipStreamSynchronize(s));
+        H.timing_collect();
     })
 }
 

@@ -1,0 +1,525 @@
+// gemm_lowp.hip -- the trailing update of the FP32-storage factorization used
+// by the mixed-precision solver (a factorization in low precision, refined to
+// FP64 accuracy): C[m x n] -= A[m x k] * B[k x n] with C in FP32 and the
+// products in
+//   * FP32 MFMA (v_mfma_f32_16x16x4_f32), or
+//   * BF16 MFMA (v_mfma_f32_16x16x32_bf16) with FP32 accumulation, operands
+//     rounded to BF16 once per block column, or
+//   * FP8-E4M3 MFMA (v_mfma_scale_f32_16x16x128_f8f6f4, unit block scales)
+//     with FP32 accumulation, each operand block divided by its own scale
+//     (max|block| / 448) before rounding and the product scaled back.
+// It is the same dgemm_('N','N', ..., -1, L21, U12, +1, A22) of PDGETRF
+// (SRC/pdgetrf.f:288-290), in lower precision.
+//
+// Lane layouts (checked on gfx950 with one-hot probes):
+//   f32 16x16x4 : A lane l = A[l%16][l/16], B = B[l/16][l%16],
+//                 D element r of lane l = D[4*(l/16) + r][l%16]
+//   bf16 16x16x32: A lane l element e = A[l%16][8*(l/16) + e],
+//                 B = B[8*(l/16) + e][l%16], D as f32 16x16x4.
+//   f8 16x16x128 : A lane l byte e = A[l%16][32*(l/16) + e],
+//                 B = B[32*(l/16) + e][l%16], D as f32 16x16x4.
+#include "sla_common.h"
+#include "sla_kernels.h"
+
+#include <hip/hip_bf16.h>
+
+#include <algorithm>
+
+namespace sla {
+
+namespace {
+
+typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
+typedef short s8_t __attribute__((ext_vector_type(8)));
+typedef int i8_t __attribute__((ext_vector_type(8)));
+
+// E4M3 (OCP) has a largest finite value of 448; a block is divided by
+// max|block| / 448 before rounding so that its range fills the format.
+constexpr float E4M3_MAX = 448.0f;
+__device__ __forceinline__ float fp8_scale(float absmax)
+{
+    return absmax > 0.f ? absmax / E4M3_MAX : 1.0f;
+}
+
+template <typename TO>
+__device__ __forceinline__ TO to_lowp(float v);
+template <>
+__device__ __forceinline__ __bf16 to_lowp<__bf16>(float v)
+{
+    return (__bf16)v;  // round to nearest even
+}
+template <>
+__device__ __forceinline__ uint8_t to_lowp<uint8_t>(float v)
+{
+    // saturating round-to-nearest conversion of the hardware (E4M3 on gfx950)
+    v = fminf(fmaxf(v, -E4M3_MAX), E4M3_MAX);
+    return (uint8_t)(__builtin_amdgcn_cvt_pk_fp8_f32(v, 0.f, 0, false) & 0xff);
+}
+
+__device__ __forceinline__ void tile_coords_lp(int tiles_m, int tiles_n, int &tm, int &tn)
+{
+    const int id = blockIdx.x, total = tiles_m * tiles_n;
+    int lid = id;
+    if (total >= 64) {
+        const int per = total / 8, rem = total % 8;
+        const int xcd = id % 8, k = id / 8;
+        const int start = xcd * per + (xcd < rem ? xcd : rem);
+        const int len = per + (xcd < rem ? 1 : 0);
+        lid = (k < len) ? start + k : id;
+    }
+    const int band = 8, bands_full = tiles_n / band, in_full = bands_full * band * tiles_m;
+    if (lid < in_full) {
+        const int b = lid / (band * tiles_m), r = lid % (band * tiles_m);
+        tm = r % tiles_m;
+        tn = b * band + r / tiles_m;
+    } else {
+        const int r = lid - in_full;
+        tm = r % tiles_m;
+        tn = bands_full * band + r / tiles_m;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP32, LDS-free, 128 x 128 per 256-thread workgroup, 64 x 64 per wave.
+// Row permutation: block i, operand row lr <-> C row wm + 4*lr + i;
+// K permutation: k-lane lk, step s <-> k = k0 + 4*lk + s.  A lane therefore
+// loads 4 consecutive rows (a float4) per k and 4 consecutive k (a float4)
+// per column.  D element r (block row 4*lk + r) is C row wm + 16*lk + 4*r + i.
+constexpr int SDEPTH = 3;
+
+__global__ __launch_bounds__(256, 2) void sgemm_nn_sub_direct_kernel(
+    int m, int n, int k, const float *__restrict__ A, int lda, const float *__restrict__ B,
+    int ldb, float *__restrict__ C, int ldc)
+{
+    int tmi, tni;
+    tile_coords_lp(m / 128, n / 128, tmi, tni);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = tmi * 128 + (wave & 1) * 64, wn = tni * 128 + (wave >> 1) * 64;
+    const int lr = lane & 15, lk = lane >> 4;
+    const float *Ap = A + (wm + 4 * lr) + (size_t)(4 * lk) * lda;
+    const float *Bp = B + (size_t)(wn + lr) * ldb + 4 * lk;
+    const size_t bstep = (size_t)16 * ldb;
+    f4_t a[SDEPTH][4], b[SDEPTH][4];
+    auto load = [&](int t, int q) {
+        const size_t k0 = (size_t)16 * t;
+#pragma unroll
+        for (int s = 0; s < 4; s++) a[q][s] = *(const f4_t *)(Ap + (k0 + s) * lda);
+#pragma unroll
+        for (int j = 0; j < 4; j++) b[q][j] = *(const f4_t *)(Bp + j * bstep + k0);
+    };
+    f4_t acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
+    const int nk = k / 16;
+#pragma unroll
+    for (int q = 0; q < SDEPTH - 1; q++)
+        if (q < nk) load(q, q);
+    for (int t = 0; t < nk; t += SDEPTH) {
+#pragma unroll
+        for (int q = 0; q < SDEPTH; q++) {
+            if (t + q < nk) {
+                const int nt = t + q + SDEPTH - 1;
+                if (nt < nk) load(nt, (q + SDEPTH - 1) % SDEPTH);
+#pragma unroll
+                for (int s = 0; s < 4; s++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+#pragma unroll
+                        for (int j = 0; j < 4; j++)
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q][s][i], b[q][j][s],
+                                                                            acc[i][j], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        float *Cc = C + (size_t)(wn + 16 * j + lr) * ldc;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            f4_t *cp = (f4_t *)(Cc + wm + 16 * lk + 4 * r);
+            f4_t c = *cp;
+            c[0] -= acc[0][j][r];
+            c[1] -= acc[1][j][r];
+            c[2] -= acc[2][j][r];
+            c[3] -= acc[3][j][r];
+            *cp = c;
+        }
+    }
+}
+
+// Ragged edges (and small shapes) of either precision: one thread per element
+// of C, operands read through a converter.  Only a thin border ever lands here.
+template <typename TA>
+__device__ __forceinline__ float to_f(TA v) { return (float)v; }
+
+template <typename TA, bool AROW>
+__global__ __launch_bounds__(256) void gemm_lp_edge_kernel(int m, int n, int k, const TA *A,
+                                                           int lda, const TA *B, int ldb,
+                                                           float *C, int ldc)
+{
+    const int i = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int j = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (i >= m || j >= n) return;
+    float acc = 0.f;
+    for (int kk = 0; kk < k; kk++) {
+        const float av = AROW ? to_f(A[(size_t)i * lda + kk]) : to_f(A[i + (size_t)kk * lda]);
+        acc = fmaf(av, to_f(B[kk + (size_t)j * ldb]), acc);
+    }
+    C[i + (size_t)j * ldc] -= acc;
+}
+
+// ---------------------------------------------------------------------------
+// BF16 operands packed once per block column:
+//   Ap : the m x k multiplier block L21, row-major (k contiguous, ld = ldak)
+//   Bp : the k x n block row U12, column-major (k contiguous, ld = ldbk)
+// so each lane's 8 k-values are one 16-byte load.  128 x 128 per workgroup,
+// 64 x 64 per wave: 4 x 4 accumulators of 16 x 16.
+constexpr int BDEPTH = 3;
+
+__global__ __launch_bounds__(256, 2) void gemm_bf16_nn_sub_direct_kernel(
+    int m, int n, int k, const __bf16 *__restrict__ Ap, int ldak, const __bf16 *__restrict__ Bp,
+    int ldbk, float *__restrict__ C, int ldc)
+{
+    int tmi, tni;
+    tile_coords_lp(m / 128, n / 128, tmi, tni);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = tmi * 128 + (wave & 1) * 64, wn = tni * 128 + (wave >> 1) * 64;
+    const int lr = lane & 15, lk = lane >> 4;
+    const __bf16 *Ar = Ap + (size_t)(wm + lr) * ldak + 8 * lk;
+    const __bf16 *Bc = Bp + (size_t)(wn + lr) * ldbk + 8 * lk;
+    const size_t astep = (size_t)16 * ldak, bstep = (size_t)16 * ldbk;
+    s8_t a[BDEPTH][4], b[BDEPTH][4];
+    auto load = [&](int t, int q) {
+        const size_t k0 = (size_t)32 * t;
+#pragma unroll
+        for (int i = 0; i < 4; i++) a[q][i] = *(const s8_t *)(Ar + i * astep + k0);
+#pragma unroll
+        for (int j = 0; j < 4; j++) b[q][j] = *(const s8_t *)(Bc + j * bstep + k0);
+    };
+    f4_t acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
+    const int nk = k / 32;
+#pragma unroll
+    for (int q = 0; q < BDEPTH - 1; q++)
+        if (q < nk) load(q, q);
+    for (int t = 0; t < nk; t += BDEPTH) {
+#pragma unroll
+        for (int q = 0; q < BDEPTH; q++) {
+            if (t + q < nk) {
+                const int nt = t + q + BDEPTH - 1;
+                if (nt < nk) load(nt, (q + BDEPTH - 1) % BDEPTH);
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            __builtin_bit_cast(bf8_t, a[q][i]), __builtin_bit_cast(bf8_t, b[q][j]),
+                            acc[i][j], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        float *Cc = C + (size_t)(wn + 16 * j + lr) * ldc;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            f4_t *cp = (f4_t *)(Cc + wm + 16 * i + 4 * lk);
+            f4_t c = *cp;
+            c[0] -= acc[i][j][0];
+            c[1] -= acc[i][j][1];
+            c[2] -= acc[i][j][2];
+            c[3] -= acc[i][j][3];
+            *cp = c;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP8-E4M3 operands, packed like the BF16 ones (k contiguous, ld multiple of
+// 128), on v_mfma_scale_f32_16x16x128_f8f6f4 with unit block scales: lane l
+// holds 32 consecutive k values of row (column) l%16 starting at 32*(l/16);
+// D as f32 16x16x4.  The products are multiplied back by sa * sb.
+constexpr int FDEPTH = 2;
+
+__global__ __launch_bounds__(256, 2) void gemm_fp8_nn_sub_direct_kernel(
+    int m, int n, int k, const uint8_t *__restrict__ Ap, int ldak, const uint8_t *__restrict__ Bp,
+    int ldbk, const float *__restrict__ scales, float *__restrict__ C, int ldc)
+{
+    int tmi, tni;
+    tile_coords_lp(m / 128, n / 128, tmi, tni);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = tmi * 128 + (wave & 1) * 64, wn = tni * 128 + (wave >> 1) * 64;
+    const int lr = lane & 15, lk = lane >> 4;
+    const uint8_t *Ar = Ap + (size_t)(wm + lr) * ldak + 32 * lk;
+    const uint8_t *Bc = Bp + (size_t)(wn + lr) * ldbk + 32 * lk;
+    const size_t astep = (size_t)16 * ldak, bstep = (size_t)16 * ldbk;
+    i8_t a[FDEPTH][4], b[FDEPTH][4];
+    auto load = [&](int t, int q) {
+        const size_t k0 = (size_t)128 * t;
+#pragma unroll
+        for (int i = 0; i < 4; i++) a[q][i] = *(const i8_t *)(Ar + i * astep + k0);
+#pragma unroll
+        for (int j = 0; j < 4; j++) b[q][j] = *(const i8_t *)(Bc + j * bstep + k0);
+    };
+    f4_t acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
+    const int nk = k / 128;
+#pragma unroll
+    for (int q = 0; q < FDEPTH - 1; q++)
+        if (q < nk) load(q, q);
+    for (int t = 0; t < nk; t += FDEPTH) {
+#pragma unroll
+        for (int q = 0; q < FDEPTH; q++) {
+            if (t + q < nk) {
+                const int nt = t + q + FDEPTH - 1;
+                if (nt < nk) load(nt, (q + FDEPTH - 1) % FDEPTH);
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
+                            a[q][i], b[q][j], acc[i][j], 0, 0, 0, 0, 0, 0);
+            }
+        }
+    }
+    const float sab = fp8_scale(scales[0]) * fp8_scale(scales[1]);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        float *Cc = C + (size_t)(wn + 16 * j + lr) * ldc;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            f4_t *cp = (f4_t *)(Cc + wm + 16 * i + 4 * lk);
+            f4_t c = *cp;
+            c[0] -= sab * acc[i][j][0];
+            c[1] -= sab * acc[i][j][1];
+            c[2] -= sab * acc[i][j][2];
+            c[3] -= sab * acc[i][j][3];
+            *cp = c;
+        }
+    }
+}
+
+__device__ __forceinline__ float fp8_value(uint8_t b)
+{
+    const int e = (b >> 3) & 15, mnt = b & 7;
+    const float v = e == 0 ? ldexpf((float)mnt * 0.125f, -6) : ldexpf(1.0f + mnt * 0.125f, e - 7);
+    return (b & 0x80) ? -v : v;
+}
+
+// Ragged edges of the FP8 form (row-major A, column-major B, both k-contiguous).
+__global__ __launch_bounds__(256) void gemm_fp8_edge_kernel(int m, int n, int k,
+                                                            const uint8_t *A, int lda,
+                                                            const uint8_t *B, int ldb,
+                                                            const float *scales, float *C, int ldc)
+{
+    const int i = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int j = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (i >= m || j >= n) return;
+    float acc = 0.f;
+    for (int kk = 0; kk < k; kk++)
+        acc = fmaf(fp8_value(A[(size_t)i * lda + kk]), fp8_value(B[kk + (size_t)j * ldb]), acc);
+    C[i + (size_t)j * ldc] -= fp8_scale(scales[0]) * fp8_scale(scales[1]) * acc;
+}
+
+// ---- packing and conversions ------------------------------------------------
+// L21 (m x k, column-major, ld lda, FP32) -> row-major (ld kpad), zero padded.
+template <typename TO, bool SCALED>
+__global__ void pack_rows_kernel(int m, int k, int kpad, const float *__restrict__ A, int lda,
+                                 const float *__restrict__ scale, TO *__restrict__ P)
+{
+    __shared__ float t[32][33];
+    const int r0 = blockIdx.x * 32, k0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    const float inv = SCALED ? 1.0f / fp8_scale(*scale) : 1.0f;
+    for (int y = ty; y < 32; y += 8) {
+        const int r = r0 + tx, kk = k0 + y;
+        t[y][tx] = (r < m && kk < k) ? A[r + (size_t)kk * lda] * inv : 0.f;
+    }
+    __syncthreads();
+    for (int y = ty; y < 32; y += 8) {
+        const int r = r0 + y, kk = k0 + tx;
+        if (r < m && kk < kpad) P[(size_t)r * kpad + kk] = to_lowp<TO>(t[tx][y]);
+    }
+}
+
+// U12 (k x n, column-major, ld ldb, FP32) -> column-major (ld kpad), zero padded.
+template <typename TO, bool SCALED>
+__global__ void pack_cols_kernel(int k, int kpad, int n, const float *__restrict__ B, int ldb,
+                                 const float *__restrict__ scale, TO *__restrict__ P)
+{
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)kpad * n) return;
+    const float inv = SCALED ? 1.0f / fp8_scale(*scale) : 1.0f;
+    const int kk = (int)(idx % kpad), j = (int)(idx / kpad);
+    P[idx] = to_lowp<TO>(kk < k ? B[kk + (size_t)j * ldb] * inv : 0.f);
+}
+
+// Bit pattern of max |A| over an m x n block into *scale (atomic max on the
+// bits, which order like the values for non-negative floats).
+__global__ __launch_bounds__(256) void absmax_kernel(int m, int n, const float *__restrict__ A,
+                                                     int lda, unsigned *__restrict__ out)
+{
+    float v = 0.f;
+    const size_t total = (size_t)m * n, stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += stride) {
+        const int i = (int)(idx % m), j = (int)(idx / m);
+        v = fmaxf(v, fabsf(A[i + (size_t)j * lda]));
+    }
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(v));
+}
+
+__global__ void convert_f64_f32_kernel(int m, int n, const double *__restrict__ A, int lda,
+                                       float *__restrict__ B, int ldb, int *overflow)
+{
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)m * n) return;
+    const int i = (int)(idx % m), j = (int)(idx / m);
+    const double a = A[i + (size_t)j * lda];
+    if (!(fabs(a) <= 3.4028234663852886e38)) *overflow = 1;  // also NaN
+    B[i + (size_t)j * ldb] = (float)a;
+}
+
+}  // namespace
+
+static bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+
+void sgemm_nn_sub(int m, int n, int k, const float *A, int lda, const float *B, int ldb, float *C,
+                  int ldc, hipStream_t s)
+{
+    if (m <= 0 || n <= 0 || k <= 0) return;
+    const int mf = (m / 128) * 128, nf = (n / 128) * 128;
+    if (mf > 0 && nf > 0 && k % 16 == 0 && lda % 4 == 0 && ldb % 4 == 0 && ldc % 4 == 0 &&
+        aligned16(A) && aligned16(B) && aligned16(C)) {
+        hipLaunchKernelGGL(sgemm_nn_sub_direct_kernel, dim3((mf / 128) * (nf / 128)), dim3(256), 0,
+                           s, mf, nf, k, A, lda, B, ldb, C, ldc);
+        SLA_LAUNCH_CHECK();
+        if (m > mf) sgemm_nn_sub(m - mf, n, k, A + mf, lda, B, ldb, C + mf, ldc, s);
+        if (n > nf)
+            sgemm_nn_sub(mf, n - nf, k, A, lda, B + (size_t)nf * ldb, ldb, C + (size_t)nf * ldc,
+                         ldc, s);
+        return;
+    }
+    hipLaunchKernelGGL((gemm_lp_edge_kernel<float, false>), dim3(ceil_div(m, 16), ceil_div(n, 16)),
+                       dim3(256), 0, s, m, n, k, A, lda, B, ldb, C, ldc);
+    SLA_LAUNCH_CHECK();
+}
+
+void gemm_bf16_nn_sub(int m, int n, int k, const __bf16 *Ap, int ldak, const __bf16 *Bp, int ldbk,
+                      float *C, int ldc, hipStream_t s)
+{
+    if (m <= 0 || n <= 0 || k <= 0) return;
+    const int mf = (m / 128) * 128, nf = (n / 128) * 128;
+    if (mf > 0 && nf > 0 && k % 32 == 0 && ldak % 8 == 0 && ldbk % 8 == 0 && ldc % 4 == 0 &&
+        aligned16(Ap) && aligned16(Bp) && aligned16(C)) {
+        hipLaunchKernelGGL(gemm_bf16_nn_sub_direct_kernel, dim3((mf / 128) * (nf / 128)), dim3(256),
+                           0, s, mf, nf, k, Ap, ldak, Bp, ldbk, C, ldc);
+        SLA_LAUNCH_CHECK();
+        if (m > mf)
+            gemm_bf16_nn_sub(m - mf, n, k, Ap + (size_t)mf * ldak, ldak, Bp, ldbk, C + mf, ldc, s);
+        if (n > nf)
+            gemm_bf16_nn_sub(mf, n - nf, k, Ap, ldak, Bp + (size_t)nf * ldbk, ldbk,
+                             C + (size_t)nf * ldc, ldc, s);
+        return;
+    }
+    hipLaunchKernelGGL((gemm_lp_edge_kernel<__bf16, true>), dim3(ceil_div(m, 16), ceil_div(n, 16)),
+                       dim3(256), 0, s, m, n, k, Ap, ldak, Bp, ldbk, C, ldc);
+    SLA_LAUNCH_CHECK();
+}
+
+template <typename TO, bool SCALED>
+static void pack_rows(int m, int k, int kpad, const float *A, int lda, const float *scale, TO *P,
+                      hipStream_t s)
+{
+    if (m <= 0 || kpad <= 0) return;
+    hipLaunchKernelGGL((pack_rows_kernel<TO, SCALED>), dim3(ceil_div(m, 32), ceil_div(kpad, 32)),
+                       dim3(256), 0, s, m, k, kpad, A, lda, scale, P);
+    SLA_LAUNCH_CHECK();
+}
+
+template <typename TO, bool SCALED>
+static void pack_cols(int k, int kpad, int n, const float *B, int ldb, const float *scale, TO *P,
+                      hipStream_t s)
+{
+    if (n <= 0 || kpad <= 0) return;
+    const size_t total = (size_t)kpad * n;
+    hipLaunchKernelGGL((pack_cols_kernel<TO, SCALED>), dim3((unsigned)((total + 255) / 256)),
+                       dim3(256), 0, s, k, kpad, n, B, ldb, scale, P);
+    SLA_LAUNCH_CHECK();
+}
+
+void pack_rows_bf16(int m, int k, int kpad, const float *A, int lda, __bf16 *P, hipStream_t s)
+{
+    pack_rows<__bf16, false>(m, k, kpad, A, lda, nullptr, P, s);
+}
+
+void pack_cols_bf16(int k, int kpad, int n, const float *B, int ldb, __bf16 *P, hipStream_t s)
+{
+    pack_cols<__bf16, false>(k, kpad, n, B, ldb, nullptr, P, s);
+}
+
+void pack_rows_fp8(int m, int k, int kpad, const float *A, int lda, const float *scale,
+                   uint8_t *P, hipStream_t s)
+{
+    pack_rows<uint8_t, true>(m, k, kpad, A, lda, scale, P, s);
+}
+
+void pack_cols_fp8(int k, int kpad, int n, const float *B, int ldb, const float *scale,
+                   uint8_t *P, hipStream_t s)
+{
+    pack_cols<uint8_t, true>(k, kpad, n, B, ldb, scale, P, s);
+}
+
+void absmax_scale(int m, int n, const float *A, int lda, float *scale, hipStream_t s)
+{
+    if (m <= 0 || n <= 0) return;
+    const size_t total = (size_t)m * n;
+    const unsigned blocks = (unsigned)std::min<size_t>((total + 255) / 256, 2048);
+    hipLaunchKernelGGL(absmax_kernel, dim3(blocks), dim3(256), 0, s, m, n, A, lda,
+                       reinterpret_cast<unsigned *>(scale));
+    SLA_LAUNCH_CHECK();
+}
+
+void gemm_fp8_nn_sub(int m, int n, int k, const uint8_t *Ap, int ldak, const uint8_t *Bp,
+                     int ldbk, const float *scales, float *C, int ldc, hipStream_t s)
+{
+    if (m <= 0 || n <= 0 || k <= 0) return;
+    const int mf = (m / 128) * 128, nf = (n / 128) * 128;
+    if (mf > 0 && nf > 0 && k % 128 == 0 && ldak % 32 == 0 && ldbk % 32 == 0 && ldc % 4 == 0 &&
+        aligned16(Ap) && aligned16(Bp) && aligned16(C)) {
+        hipLaunchKernelGGL(gemm_fp8_nn_sub_direct_kernel, dim3((mf / 128) * (nf / 128)), dim3(256),
+                           0, s, mf, nf, k, Ap, ldak, Bp, ldbk, scales, C, ldc);
+        SLA_LAUNCH_CHECK();
+        if (m > mf)
+            gemm_fp8_nn_sub(m - mf, n, k, Ap + (size_t)mf * ldak, ldak, Bp, ldbk, scales, C + mf,
+                            ldc, s);
+        if (n > nf)
+            gemm_fp8_nn_sub(mf, n - nf, k, Ap, ldak, Bp + (size_t)nf * ldbk, ldbk, scales,
+                            C + (size_t)nf * ldc, ldc, s);
+        return;
+    }
+    hipLaunchKernelGGL(gemm_fp8_edge_kernel, dim3(ceil_div(m, 16), ceil_div(n, 16)), dim3(256), 0,
+                       s, m, n, k, Ap, ldak, Bp, ldbk, scales, C, ldc);
+    SLA_LAUNCH_CHECK();
+}
+
+void convert_f64_f32(int m, int n, const double *A, int lda, float *B, int ldb, int *overflow,
+                     hipStream_t s)
+{
+    if (m <= 0 || n <= 0) return;
+    const size_t total = (size_t)m * n;
+    hipLaunchKernelGGL(convert_f64_f32_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       s, m, n, A, lda, B, ldb, overflow);
+    SLA_LAUNCH_CHECK();
+}
+
+}  // namespace sla

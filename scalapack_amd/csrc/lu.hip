@@ -88,6 +88,8 @@ Handle::~Handle()
                     (void *)slot_rowj, (void *)pairs, (void *)pair_counts, (void *)trsv_flags,
                     (void *)perm, (void *)work})
         if (p) hipFree(p);
+    for (auto &b : bufs)
+        if (b.p) hipFree(b.p);
     for (auto &e : ev_pool) hipEventDestroy(e);
     hipEventDestroy(ev_next);
     hipEventDestroy(ev_panel);
@@ -141,6 +143,19 @@ void Handle::ensure_work(size_t bytes)
     if (work) SLA_HIP(hipFree(work));
     work = alloc_dev<char>(bytes);
     work_cap = bytes;
+}
+
+void *Handle::buffer(int id, size_t bytes)
+{
+    Buf &b = bufs.at(id);
+    if (bytes > b.cap) {
+        SLA_HIP(hipStreamSynchronize(stream));
+        SLA_HIP(hipStreamSynchronize(stream2));
+        if (b.p) SLA_HIP(hipFree(b.p));
+        b.p = alloc_dev<char>(bytes);
+        b.cap = bytes;
+    }
+    return b.p;
 }
 
 // --- per-kernel timing -----------------------------------------------------
@@ -211,6 +226,8 @@ struct Lu {
     int lda, M, N;
     int *ipiv;  // device, indexed by global row (0-based), 1-based values
     PanelWork<T> pw;
+    int schur = SCHUR_NATIVE;  // precision of the outer trailing update
+    bool l_packed = false;     // L21 of the current block column is packed
 
     T *at(int i, int j) const { return A + (size_t)i + (size_t)j * lda; }
 
@@ -234,6 +251,11 @@ struct Lu {
     }
 
     void gemm(int m, int n, int k, const T *a, const T *b, T *c, hipStream_t s);
+
+    // Outer trailing update A22 -= L21 U12 of one block column (k = its
+    // width): the storage-precision GEMM, or for FP32 storage the BF16 / FP8
+    // MFMA update on operands packed once per block column.
+    void update(int m, int n, int k, const T *l21, const T *u12, T *c, hipStream_t s);
 
     // B[m x n] <- L^{-1} B for the unit lower m x m L at l: recursive halving
     // down to <= 32-row diagonal solves, the off-diagonal products on the
@@ -288,7 +310,7 @@ struct Lu {
     int run()
     {
         const int mn = std::min(M, N), nb = h.nb;
-        if (nb > 512) throw std::logic_error("dgetrf: NB above 512");
+        if (nb > 512) throw std::logic_error("getrf: NB above 512");
         hipStream_t s0 = h.stream, s1 = h.stream2;
         int jprev = 0;
         {
@@ -304,9 +326,10 @@ struct Lu {
             if (jn >= N) break;
             const bool has_next = jn < mn;
             const int jb2 = has_next ? std::min(nb, mn - jn) : 0;
+            l_packed = false;
             if (has_next) {
                 trsm(jb, jb2, at(j, j), at(j, jn), s0);
-                gemm(M - jn, jb2, jb, at(jn, j), at(j, jn), at(jn, jn), s0);
+                update(M - jn, jb2, jb, at(jn, j), at(j, jn), at(jn, jn), s0);
                 SLA_HIP(hipEventRecord(h.ev_next, s0));
                 SLA_HIP(hipStreamWaitEvent(s1, h.ev_next, 0));
                 panel(jn, jn, jb2, jn, jn + jb2, s1);
@@ -317,7 +340,7 @@ struct Lu {
             const int c = jn + jb2, rest = N - c;
             if (rest > 0) {
                 trsm(jb, rest, at(j, j), at(j, c), s0);
-                if (jn < M) gemm(M - jn, rest, jb, at(jn, j), at(j, c), at(jn, c), s0);
+                if (jn < M) update(M - jn, rest, jb, at(jn, j), at(j, c), at(jn, c), s0);
             }
             if (has_next) SLA_HIP(hipStreamWaitEvent(s0, h.ev_panel, 0));
             jprev = j;
@@ -336,9 +359,68 @@ void Lu<double>::gemm(int m, int n, int k, const double *a, const double *b, dou
     h.timing_end(2.0 * m * n * k, s);
 }
 
+template <>
+void Lu<double>::update(int m, int n, int k, const double *l21, const double *u12, double *c,
+                        hipStream_t s)
+{
+    gemm(m, n, k, l21, u12, c, s);
+}
+
+template <>
+void Lu<float>::gemm(int m, int n, int k, const float *a, const float *b, float *c, hipStream_t s)
+{
+    if (m <= 0 || n <= 0 || k <= 0) return;
+    h.timing_begin(CAT_GEMM, s);
+    sgemm_nn_sub(m, n, k, a, lda, b, lda, c, lda, s);
+    h.timing_end(2.0 * m * n * k, s);
+}
+
+template <>
+void Lu<float>::update(int m, int n, int k, const float *l21, const float *u12, float *c,
+                       hipStream_t s)
+{
+    if (m <= 0 || n <= 0 || k <= 0) return;
+    if (schur == SCHUR_NATIVE) {
+        gemm(m, n, k, l21, u12, c, s);
+        return;
+    }
+    // operands k-contiguous, k padded with zeros to the MFMA's K step
+    const int kq = schur == SCHUR_BF16 ? 32 : 128;
+    const int kpad = ceil_div(k, kq) * kq;
+    const size_t esz = schur == SCHUR_BF16 ? 2 : 1;
+    char *lp = static_cast<char *>(h.buffer(BUF_PACK_L, (size_t)M * kpad * esz));
+    char *up = static_cast<char *>(h.buffer(BUF_PACK_U, (size_t)N * kpad * esz));
+    float *scales = static_cast<float *>(h.buffer(BUF_SCALES, 64 * sizeof(float)));
+    h.timing_begin(CAT_OTHER, s);
+    if (schur == SCHUR_BF16) {
+        if (!l_packed) pack_rows_bf16(m, k, kpad, l21, lda, reinterpret_cast<__bf16 *>(lp), s);
+        pack_cols_bf16(k, kpad, n, u12, lda, reinterpret_cast<__bf16 *>(up), s);
+    } else {
+        if (!l_packed) {
+            fill_u32(reinterpret_cast<unsigned *>(scales), 1, 0u, s);
+            absmax_scale(m, k, l21, lda, scales, s);
+            pack_rows_fp8(m, k, kpad, l21, lda, scales, reinterpret_cast<uint8_t *>(lp), s);
+        }
+        fill_u32(reinterpret_cast<unsigned *>(scales) + 1, 1, 0u, s);
+        absmax_scale(k, n, u12, lda, scales + 1, s);
+        pack_cols_fp8(k, kpad, n, u12, lda, scales + 1, reinterpret_cast<uint8_t *>(up), s);
+    }
+    h.timing_end(0.0, s);
+    l_packed = true;
+    h.timing_begin(CAT_GEMM, s);
+    if (schur == SCHUR_BF16)
+        gemm_bf16_nn_sub(m, n, kpad, reinterpret_cast<const __bf16 *>(lp), kpad,
+                         reinterpret_cast<const __bf16 *>(up), kpad, c, lda, s);
+    else
+        gemm_fp8_nn_sub(m, n, kpad, reinterpret_cast<const uint8_t *>(lp), kpad,
+                        reinterpret_cast<const uint8_t *>(up), kpad, scales, c, lda, s);
+    h.timing_end(2.0 * m * n * k, s);
+}
+
 }  // namespace
 
-int dgetrf_device(Handle &h, int m, int n, double *dA, int lda, int *dipiv)
+template <typename T>
+static int getrf_device(Handle &h, int m, int n, T *dA, int lda, int *dipiv, int schur)
 {
     if (m <= 0 || n <= 0) return 0;
     const int mn = std::min(m, n);
@@ -350,7 +432,8 @@ int dgetrf_device(Handle &h, int m, int n, double *dA, int lda, int *dipiv)
     // the panel stream starts after everything queued so far on the main one
     SLA_HIP(hipEventRecord(h.ev_join, h.stream));
     SLA_HIP(hipStreamWaitEvent(h.stream2, h.ev_join, 0));
-    Lu<double> lu{h, dA, lda, m, n, dipiv, h.panel_work<double>()};
+    Lu<T> lu{h, dA, lda, m, n, dipiv, h.panel_work<T>()};
+    lu.schur = schur;
     lu.run();
     SLA_HIP(hipEventRecord(h.ev_join, h.stream2));
     SLA_HIP(hipStreamWaitEvent(h.stream, h.ev_join, 0));
@@ -358,12 +441,24 @@ int dgetrf_device(Handle &h, int m, int n, double *dA, int lda, int *dipiv)
     SLA_HIP(hipMemcpyAsync(flags, h.ctl, sizeof flags, hipMemcpyDeviceToHost, h.stream));
     SLA_HIP(hipStreamSynchronize(h.stream));
     h.timing_collect();
-    if (flags[1]) throw std::runtime_error("dgetrf: a device-wide barrier timed out (panel)");
+    if (flags[1]) throw std::runtime_error("getrf: a device-wide barrier timed out (panel)");
     h.perm_key_a = dA;
     h.perm_key_ipiv = dipiv;
     h.perm_m = m;
     const int zc = flags[2];
     return (zc != INT_MAX && zc <= mn) ? zc : 0;
+}
+
+int dgetrf_device(Handle &h, int m, int n, double *dA, int lda, int *dipiv)
+{
+    return getrf_device<double>(h, m, n, dA, lda, dipiv, SCHUR_NATIVE);
+}
+
+int sgetrf_device(Handle &h, int m, int n, float *dA, int lda, int *dipiv, int schur)
+{
+    if (schur != SCHUR_NATIVE && schur != SCHUR_BF16 && schur != SCHUR_FP8)
+        throw std::invalid_argument("sgetrf: unknown Schur-update precision");
+    return getrf_device<float>(h, m, n, dA, lda, dipiv, schur);
 }
 
 // Row permutation of the factorization from IPIV alone (for a solve that did

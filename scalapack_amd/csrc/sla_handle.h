@@ -14,6 +14,37 @@ namespace sla {
 
 enum Category { CAT_PANEL = 0, CAT_LASWP, CAT_TRSM, CAT_GEMM, CAT_TRSV, CAT_OTHER, CAT_COUNT };
 
+// Precision of the outer trailing update of an FP32-storage factorization.
+enum Schur { SCHUR_NATIVE = 0, SCHUR_BF16 = 1, SCHUR_FP8 = 2 };
+
+enum Precision { PREC_FP64 = 0, PREC_FP32 = 1, PREC_BF16 = 2, PREC_FP8 = 3 };
+enum Refinement { REFINE_IR = 0, REFINE_GMRES = 1 };
+
+struct MixedInfo {
+    int precision = 0;   // factorization precision asked for (Precision)
+    int refinement = 0;  // refinement that produced the result (Refinement)
+    int iter = 0;        // DSGESV's ITER: refinement steps, < 0 after the FP64 fallback
+    int outer = 0;       // FP64 residuals computed after the first solve
+    int inner = 0;       // GMRES steps in total
+    int fell_back = 0;   // 1 if the FP64 factorization produced X
+    double anorm = 0.0;  // ||A||_inf
+    double bwd_err = 0.0;  // max_j ||r_j||_inf / (||A||_inf ||x_j||_inf) of the last residual
+    double sresid = 0.0;   // the same divided by N eps (the testers' scaled residual)
+};
+
+// Reusable device buffers, grown on demand.
+enum BufId {
+    BUF_PACK_L = 0,  // packed low-precision L21 of one block column
+    BUF_PACK_U,      // packed low-precision U12 of one block row
+    BUF_SCALES,      // FP8 block scales
+    BUF_SA,          // FP32 copy of the matrix (mixed-precision solve)
+    BUF_IPIV_S,      // its pivots
+    BUF_VEC,         // refinement vectors (FP64 / FP32)
+    BUF_KRYLOV,      // GMRES bases
+    BUF_PARTIAL,     // partial sums of the residual / reductions
+    BUF_COUNT
+};
+
 struct Stat {
     double ms = 0.0;      // summed kernel time (HIP events around each launch)
     double work = 0.0;    // summed algorithmic flops (GEMM/TRSM/PANEL) or bytes (LASWP)
@@ -61,10 +92,26 @@ struct Handle {
     char *work = nullptr;
     size_t work_cap = 0;
 
+    struct Buf {
+        void *p = nullptr;
+        size_t cap = 0;
+    };
+    std::array<Buf, BUF_COUNT> bufs{};
+    // Device buffer `id` of at least `bytes` (the streams are synchronised
+    // before it grows).
+    void *buffer(int id, size_t bytes);
+
     // tuning knobs
     int nb = 256;          // block column width of the outer factorization
     int leaf_w = 32;       // panel leaf width (columns per device-wide-barrier kernel)
     int panel_wgs = 32;    // preferred workgroups per leaf launch
+    // mixed-precision refinement
+    int ir_max_iter = 30;        // refinement steps before the FP64 fallback (DSGESV's ITERMAX)
+    int ir_gmres_on_stall = 1;   // classical refinement that stalls continues as GMRES-IR
+    int gmres_m = 50;            // GMRES steps per correction
+    double gmres_tol = 1e-6;     // relative residual reduction per correction
+
+    MixedInfo last_mixed;  // details of the last mixed-precision solve
 
     // timing
     bool profile = false;
@@ -94,6 +141,9 @@ struct Handle {
 // Returns INFO > 0 for an exactly zero pivot U(INFO, INFO); throws on a
 // runtime failure.
 int dgetrf_device(Handle &h, int m, int n, double *dA, int lda, int *dipiv);
+// The same on FP32 storage; `schur` (enum Schur) selects the precision of the
+// outer trailing update.
+int sgetrf_device(Handle &h, int m, int n, float *dA, int lda, int *dipiv, int schur);
 int dgetrs_device(Handle &h, int n, int nrhs, const double *dA, int lda, const int *dipiv,
                   double *dB, int ldb);
 // PDLASWP on a device matrix: row interchanges k1 .. k2-1 (0-based) with

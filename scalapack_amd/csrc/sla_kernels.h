@@ -9,6 +9,36 @@ namespace sla {
 void dgemm_nn_sub(int m, int n, int k, const double *A, int lda, const double *B, int ldb,
                   double *C, int ldc, hipStream_t s);
 
+// ---- low-precision Schur-complement updates (FP32 storage) -----------------
+// C[m x n] -= A[m x k] * B[k x n], FP32 operands on FP32 MFMA.
+void sgemm_nn_sub(int m, int n, int k, const float *A, int lda, const float *B, int ldb, float *C,
+                  int ldc, hipStream_t s);
+// Same with BF16 operands packed k-contiguous: Ap row-major m x k (ld ldak),
+// Bp column-major k x n (ld ldbk); FP32 accumulation into C.
+void gemm_bf16_nn_sub(int m, int n, int k, const __bf16 *Ap, int ldak, const __bf16 *Bp, int ldbk,
+                      float *C, int ldc, hipStream_t s);
+// FP8-E4M3 (OCP) operands packed like the BF16 ones; the products are scaled
+// back by sa * sb (the per-panel scales the operands were divided by).
+void gemm_fp8_nn_sub(int m, int n, int k, const uint8_t *Ap, int ldak, const uint8_t *Bp,
+                     int ldbk, const float *scales, float *C, int ldc, hipStream_t s);
+// Packing: A (m x k, column-major) -> row-major with k padded by zeros to
+// kpad; B (k x n, column-major) -> column-major with ld kpad, zero padded.
+void pack_rows_bf16(int m, int k, int kpad, const float *A, int lda, __bf16 *P, hipStream_t s);
+void pack_cols_bf16(int k, int kpad, int n, const float *B, int ldb, __bf16 *P, hipStream_t s);
+// FP8 packing divides by the scale held at *scale (written by absmax_scale).
+void pack_rows_fp8(int m, int k, int kpad, const float *A, int lda, const float *scale,
+                   uint8_t *P, hipStream_t s);
+void pack_cols_fp8(int k, int kpad, int n, const float *B, int ldb, const float *scale,
+                   uint8_t *P, hipStream_t s);
+// *scale = max|A| / 448 (the E4M3 maximum) over an m x n block, or 1 if the
+// block is zero; *scale must hold 0 on entry (bit-pattern max).
+void absmax_scale(int m, int n, const float *A, int lda, float *scale, hipStream_t s);
+
+// ---- precision conversions -------------------------------------------------
+// B = (float)A; sets *overflow when some |A(i,j)| exceeds the FP32 range.
+void convert_f64_f32(int m, int n, const double *A, int lda, float *B, int ldb, int *overflow,
+                     hipStream_t s);
+
 // ---- panel leaf (PDGETF2 on <= 32 columns) --------------------------------
 template <typename T>
 struct PanelSlots {

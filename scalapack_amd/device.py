@@ -9,9 +9,13 @@ from __future__ import annotations
 
 import ctypes
 
-from ._lib import last_error, load
+from ._lib import SolveInfo, last_error, load
 
-STAT_CLASSES = ("panel", "laswp", "trsm", "gemm", "trsv")
+STAT_CLASSES = ("panel", "laswp", "trsm", "gemm", "trsv", "other")
+
+# factorization precisions and refinements (SLA_PREC_*, SLA_REFINE_*)
+PRECISIONS = {"fp64": 0, "fp32": 1, "bf16": 2, "fp8": 3}
+REFINEMENTS = {"ir": 0, "gmres": 1}
 
 
 class SlaError(RuntimeError):
@@ -96,3 +100,35 @@ class Handle:
         _check(self._lib.sla_dgesv(self._h, n, nrhs, _ptr(a), lda, _ptr(ipiv), _ptr(b), ldb,
                                    ctypes.byref(info)))
         return info.value
+
+    # ------------------------------------------------------- mixed precision
+    def dxgesv(self, n: int, nrhs: int, a, lda: int, ipiv, b, ldb: int, x, ldx: int,
+               prec: str = "fp32", refine: str = "ir") -> tuple[int, int]:
+        """X = A^{-1} B by a `prec` factorization refined to FP64 backward
+        error; returns (iter, info) as LAPACK DSGESV's ITER and INFO."""
+        it, info = ctypes.c_int(), ctypes.c_int()
+        _check(self._lib.sla_dxgesv(self._h, PRECISIONS[prec], REFINEMENTS[refine], n, nrhs,
+                                    _ptr(a), lda, _ptr(ipiv), _ptr(b), ldb, _ptr(x), ldx,
+                                    ctypes.byref(it), ctypes.byref(info)))
+        return it.value, info.value
+
+    def last_solve_info(self) -> dict:
+        out = SolveInfo()
+        _check(self._lib.sla_last_solve_info(self._h, ctypes.byref(out)))
+        d = out.as_dict()
+        d["precision"] = {v: k for k, v in PRECISIONS.items()}[d["precision"]]
+        d["refinement"] = {v: k for k, v in REFINEMENTS.items()}[d["refinement"]]
+        return d
+
+    def sgetrf(self, m: int, n: int, a, lda: int, ipiv, prec: str = "fp32") -> int:
+        """FP32-storage LU with trailing updates in `prec` (fp32, bf16, fp8)."""
+        info = ctypes.c_int()
+        _check(self._lib.sla_sgetrf(self._h, PRECISIONS[prec], m, n, _ptr(a), lda, _ptr(ipiv),
+                                    ctypes.byref(info)))
+        return info.value
+
+    def gemm_lowp_sub(self, m: int, n: int, k: int, a, lda: int, b, ldb: int, c, ldc: int,
+                      prec: str = "fp32") -> None:
+        """C -= A B, FP32 operands, products in `prec`, FP32 accumulation."""
+        _check(self._lib.sla_gemm_lowp_sub(self._h, PRECISIONS[prec], m, n, k, _ptr(a), lda,
+                                           _ptr(b), ldb, _ptr(c), ldc))

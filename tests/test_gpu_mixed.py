@@ -1,0 +1,250 @@
+"""GPU tests of the mixed-precision path, through the C-ABI.
+
+* The low-precision trailing-update kernels (FP32, BF16, FP8-E4M3 products,
+  FP32 accumulation) against a float64 product of the same rounded operands
+  (torch's bfloat16 / float8_e4m3fn rounding, the FP8 per-block scale
+  max|block| / 448 applied as in the kernels); tolerance: FP32 accumulation,
+  2 k 2^-24 (|A||B|) + 2^-24 |C|.
+* The FP32-storage factorization: P A = L U to a backward error set by the
+  update precision (FP32 1e-5, BF16 3e-2, FP8 0.5 of max|A|), pivots a valid
+  interchange sequence.
+* The refined solve: the scaled residual ||b - A x|| / (||A|| ||x|| N eps) < 16
+  of the LU testers (TESTING/traditional/LIN/pdlaschk.f:296) for every
+  result; the forward error against the oracle's FP64 LU solution is
+  reported and bounded by 100x the oracle's own forward error (x_true
+  known); the DSGESV contract (A untouched unless ITER < 0, B untouched,
+  ITER codes -1 / -2 / -3, INFO of the FP64 fallback).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+EPS = 2.0 ** -53
+
+
+def _note(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def dev():
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    from scalapack_amd.device import Handle
+    h = Handle()
+    yield h
+    h.close()
+
+
+def to_dev(a: np.ndarray, dtype=None):
+    t = torch.from_numpy(np.ascontiguousarray(np.asfortranarray(a).T))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.cuda()
+
+
+def from_dev(t) -> np.ndarray:
+    return np.asfortranarray(t.cpu().numpy().T)
+
+
+def svd_matrix(n: int, kappa: float, seed: int) -> np.ndarray:
+    """A = U diag(sigma) V^T, U and V Q factors of seeded Gaussians, sigma
+    geometric from 1 to 1/kappa."""
+    rng = np.random.default_rng(seed)
+    u, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    v, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    s = kappa ** (-np.arange(n) / (n - 1))
+    return np.asfortranarray((u * s) @ v.T)
+
+
+def scaled_residual(a, x, b) -> float:
+    r = b - a @ x
+    an = np.abs(a).sum(axis=1).max()
+    n = a.shape[0]
+    return float((np.abs(r).max(axis=0) / (an * np.abs(x).max(axis=0) * n * EPS)).max())
+
+
+def quantize(a32: np.ndarray, prec: str) -> tuple[np.ndarray, float]:
+    """Operand as the kernels round it (float64 values) and its scale."""
+    t = torch.from_numpy(np.ascontiguousarray(a32))
+    if prec == "fp32":
+        return a32.astype(np.float64), 1.0
+    if prec == "bf16":
+        return t.to(torch.bfloat16).to(torch.float64).numpy(), 1.0
+    amax = np.float32(np.abs(a32).max())
+    scale = np.float32(amax / np.float32(448.0)) if amax > 0 else np.float32(1.0)
+    inv = np.float32(np.float32(1.0) / scale)
+    q = torch.from_numpy(np.ascontiguousarray((a32 * inv).astype(np.float32)))
+    q = q.clamp(-448.0, 448.0).to(torch.float8_e4m3fn).to(torch.float64).numpy()
+    return q, float(scale)
+
+
+# ------------------------------------------------------------------ kernels
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "fp8"])
+@pytest.mark.parametrize("m,n,k", [(128, 128, 128), (256, 384, 256), (1024, 1024, 512),
+                                   (300, 200, 77), (130, 260, 32)])
+def test_gemm_lowp_parity(dev, prec, m, n, k):
+    _note(f"gemm_lowp {prec} m={m} n={n} k={k}")
+    rng = np.random.default_rng(m + 3 * n + 7 * k)
+    A = np.asfortranarray(rng.uniform(-1, 1, (m, k)).astype(np.float32))
+    B = np.asfortranarray(rng.uniform(-2, 2, (k, n)).astype(np.float32))
+    C = np.asfortranarray(rng.uniform(-1, 1, (m, n)).astype(np.float32))
+    dA, dB, dC = to_dev(A), to_dev(B), to_dev(C)
+    dev.gemm_lowp_sub(m, n, k, dA, m, dB, k, dC, m, prec=prec)
+    torch.cuda.synchronize()
+    got = from_dev(dC).astype(np.float64)
+    qa, sa = quantize(A, prec)
+    qb, sb = quantize(B, prec)
+    prod = (qa @ qb) * (sa * sb)
+    ref = C.astype(np.float64) - prod
+    bound = 2.0 * k * 2.0 ** -24 * (np.abs(qa) @ np.abs(qb)) * (sa * sb) + 2.0 ** -23 * np.abs(C)
+    err = np.abs(got - ref)
+    assert np.all(err <= bound + 1e-30), f"max err {err.max()} vs bound {bound.max()}"
+
+
+# --------------------------------------------------------- factorization
+@pytest.mark.parametrize("prec,tol", [("fp32", 1e-5), ("bf16", 3e-2), ("fp8", 0.5)])
+@pytest.mark.parametrize("n", [1024, 1000])
+def test_sgetrf_backward_error(dev, prec, tol, n):
+    _note(f"sgetrf {prec} n={n}")
+    rng = np.random.default_rng(n + len(prec))
+    A = np.asfortranarray(rng.uniform(-1, 1, (n, n)).astype(np.float32))
+    dA = to_dev(A)
+    ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
+    info = dev.sgetrf(n, n, dA, n, ipiv, prec=prec)
+    torch.cuda.synchronize()
+    assert info == 0
+    lu = from_dev(dA).astype(np.float64)
+    piv = ipiv.cpu().numpy()
+    assert np.all(piv >= np.arange(1, n + 1)) and np.all(piv <= n)
+    perm = np.arange(n)
+    for i, p in enumerate(piv):
+        perm[[i, p - 1]] = perm[[p - 1, i]]
+    L = np.tril(lu, -1) + np.eye(n)
+    U = np.triu(lu)
+    err = np.abs(A.astype(np.float64)[perm] - L @ U).max() / np.abs(A).max()
+    _note(f"  backward error {err:.3e}")
+    assert err <= tol
+    assert np.abs(L).max() <= 1.0 + 1e-6  # partial pivoting keeps |l| <= 1
+
+
+# ----------------------------------------------------------- refined solve
+def _solve(dev, A, B, prec, refine):
+    n, nrhs = B.shape
+    dA, dB = to_dev(A), to_dev(B)
+    dA0, dB0 = dA.clone(), dB.clone()
+    dX = torch.zeros_like(dB)
+    ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
+    it, info = dev.dxgesv(n, nrhs, dA, n, ipiv, dB, n, dX, n, prec=prec, refine=refine)
+    torch.cuda.synchronize()
+    st = dev.last_solve_info()
+    assert torch.equal(dB, dB0), "B was modified"
+    if it >= 0:
+        assert torch.equal(dA, dA0), "A was modified although refinement succeeded"
+    return from_dev(dX), it, info, st, dA, ipiv
+
+
+def _oracle_x(oracle, A, B, nb=64):
+    lu = A.copy(order="F")
+    piv, info = oracle.pdgetrf(lu, nb)
+    assert info == 0
+    return oracle.pdgetrs(lu, piv, B.copy(order="F"), nb)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+@pytest.mark.parametrize("n,nrhs", [(2048, 1), (1500, 3), (1024, 16)])
+def test_dxgesv_uniform(dev, oracle, prec, n, nrhs):
+    _note(f"dxgesv {prec} IR uniform n={n} nrhs={nrhs}")
+    rng = np.random.default_rng(n + nrhs)
+    A = np.asfortranarray(rng.uniform(-1, 1, (n, n)))
+    xt = np.asfortranarray(rng.standard_normal((n, nrhs)))
+    B = np.asfortranarray(A @ xt)
+    X, it, info, st, _, _ = _solve(dev, A, B, prec, "ir")
+    assert info == 0
+    s = scaled_residual(A, X, B)
+    x64 = _oracle_x(oracle, A, B)
+    fe = np.abs(X - xt).max() / np.abs(xt).max()
+    fe64 = np.abs(x64 - xt).max() / np.abs(xt).max()
+    _note(f"  iter={it} {st} sresid={s:.3e} fwd={fe:.3e} fwd(FP64 LU)={fe64:.3e}")
+    assert s < 16.0
+    assert abs(st["sresid"] - s) <= 0.5 * s + 1.0  # the library's own measure agrees
+    assert fe <= 100 * fe64 + 1e-14
+
+
+@pytest.mark.parametrize("prec,refine,kappa", [("fp32", "gmres", 1e8), ("fp32", "ir", 1e4),
+                                               ("bf16", "gmres", 1e4), ("fp32", "gmres", 1e6)])
+def test_dxgesv_ill_conditioned(dev, oracle, prec, refine, kappa):
+    n = 1024
+    _note(f"dxgesv {prec} {refine} kappa={kappa:g} n={n}")
+    A = svd_matrix(n, kappa, 7)
+    rng = np.random.default_rng(8)
+    xt = np.asfortranarray(rng.standard_normal((n, 1)))
+    B = np.asfortranarray(A @ xt)
+    X, it, info, st, _, _ = _solve(dev, A, B, prec, refine)
+    assert info == 0
+    s = scaled_residual(A, X, B)
+    x64 = _oracle_x(oracle, A, B)
+    fe = np.abs(X - xt).max() / np.abs(xt).max()
+    fe64 = np.abs(x64 - xt).max() / np.abs(xt).max()
+    _note(f"  iter={it} {st} sresid={s:.3e} fwd={fe:.3e} fwd(FP64 LU)={fe64:.3e}")
+    assert s < 16.0
+    assert it >= 0, "refinement fell back to FP64"
+    assert fe <= 100 * fe64 + 1e-14
+
+
+def test_dxgesv_fp8_falls_back_or_converges(dev, oracle):
+    n = 1024
+    _note(f"dxgesv fp8 gmres kappa=1e6 n={n}")
+    A = svd_matrix(n, 1e6, 11)
+    xt = np.ones((n, 1))
+    B = np.asfortranarray(A @ xt)
+    X, it, info, st, dA, ipiv = _solve(dev, A, B, "fp8", "gmres")
+    assert info == 0
+    s = scaled_residual(A, X, B)
+    _note(f"  iter={it} {st} sresid={s:.3e}")
+    assert s < 16.0
+
+
+def test_dxgesv_overflow_falls_back(dev, oracle):
+    n = 512
+    _note("dxgesv overflow -> FP64 fallback")
+    rng = np.random.default_rng(3)
+    A = np.asfortranarray(rng.uniform(-1, 1, (n, n)) + n * np.eye(n))
+    A[5, 5] = 1e39
+    B = np.asfortranarray(rng.uniform(-1, 1, (n, 1)))
+    X, it, info, st, dA, ipiv = _solve(dev, A, B, "fp32", "ir")
+    assert (it, info, st["fell_back"]) == (-2, 0, 1)
+    assert scaled_residual(A, X, B) < 16.0
+    lu = A.copy(order="F")
+    piv, _ = oracle.pdgetrf(lu, 64)
+    assert np.array_equal(ipiv.cpu().numpy(), piv[:n])
+
+
+def test_dxgesv_zero_pivot(dev, oracle):
+    n = 256
+    _note("dxgesv exactly singular -> ITER -3, INFO of FP64 LU")
+    rng = np.random.default_rng(4)
+    A = np.asfortranarray(rng.uniform(-1, 1, (n, n)))
+    A[:, 100] = 0.0
+    B = np.asfortranarray(rng.uniform(-1, 1, (n, 1)))
+    lu = A.copy(order="F")
+    _, info_ref = oracle.pdgetrf(lu, 64)
+    _, it, info, st, _, _ = _solve(dev, A, B, "fp32", "ir")
+    assert it == -3 and info == info_ref == 101
+
+
+def test_dxgesv_fp64_request(dev, oracle):
+    n = 384
+    _note("dxgesv prec=fp64 -> plain FP64 LU, ITER -1")
+    rng = np.random.default_rng(5)
+    A = np.asfortranarray(rng.uniform(-1, 1, (n, n)))
+    B = np.asfortranarray(rng.uniform(-1, 1, (n, 2)))
+    X, it, info, st, _, _ = _solve(dev, A, B, "fp64", "ir")
+    assert (it, info) == (-1, 0)
+    x64 = _oracle_x(oracle, A, B)
+    assert np.allclose(X, x64, rtol=1e-9, atol=1e-12)
