@@ -179,7 +179,8 @@ def run_mixed(torch, h, AT, B0, steps, warmup, prec, refine):
     for _ in range(warmup):
         it, _ = h.dxgesv(n, nrhs, AT, n, ipiv, B0, n, X, n, prec=prec, refine=refine)
         if it < 0:
-            raise RuntimeError(f"mixed solve fell back to FP64 (ITER {it}) in the warmup")
+            raise RuntimeError(f"mixed solve fell back to FP64 (ITER {it}) in the warmup: "
+                               f"{h.last_solve_info()}")
     torch.cuda.synchronize()
     h.set("profile", 1)
     h.reset_stats()
@@ -268,6 +269,7 @@ def main() -> int:
                               "scaled_residual": max(sres), "outer": infos[-1]["outer"],
                               "gmres_steps": infos[-1]["inner"],
                               "refinement_used": infos[-1]["refinement"],
+                              "bwd_history": [float(f"{v:.3e}") for v in infos[-1]["history"]],
                               "kernel_ms_per_step": {k: round(v["ms"] / args.steps, 2)
                                                      for k, v in stats.items()}})
             if fp64_line is None and not args.no_fp64:

@@ -80,14 +80,19 @@ typedef struct sla_handle_s *sla_handle_t;
 /* Creates a handle bound to the current HIP device. */
 int sla_create(sla_handle_t *h);
 int sla_destroy(sla_handle_t h);
-/* Work of the handle goes to `stream` (a hipStream_t; NULL restores the
- * handle's own stream).  Calls return after the stream has drained. */
+/* Work of the handle goes to `stream` (a hipStream_t; NULL is HIP's default
+ * stream, as for rocBLAS).  A new handle uses a non-blocking stream of its
+ * own: inputs produced on other streams must be complete before a call.
+ * Calls return after the stream has drained. */
 int sla_set_stream(sla_handle_t h, void *stream);
 /* Tuning knobs: "nb" (block column width), "leaf_w" (panel leaf width,
  * <= 32), "panel_wgs" (workgroups per panel leaf), "profile" (0/1); for the
- * mixed-precision solve "ir_max_iter" (default 30), "ir_gmres_on_stall"
+ * mixed-precision solve "ir_max_iter" (3..1000, default 30), "ir_gmres_on_stall"
  * (0/1, default 1), "gmres_m" (GMRES steps per correction, default 50),
- * "gmres_tol_exp" (e: relative reduction 10^-e per correction, default 6). */
+ * "gmres_tol_exp" (e: relative reduction 10^-e per correction, default 6),
+ * "pivoting" (FP32-storage factorizations: 0 partial pivoting, 1 tournament
+ * pivoting of the panel leaves, default 1; the FP64 path always pivots as
+ * PDGETF2). */
 int sla_set_param(sla_handle_t h, const char *name, int value);
 int sla_get_param(sla_handle_t h, const char *name, int *value);
 const char *sla_last_error(void);
@@ -139,8 +144,10 @@ typedef struct {
     int inner;       /* GMRES steps in total */
     int fell_back;   /* 1 if the FP64 factorization produced X */
     double anorm;    /* ||A||_inf */
-    double bwd_err;  /* max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf) */
+    double bwd_err;  /* max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf); NaN after
+                        the FP64 fallback (A then holds the factors) */
     double sresid;   /* bwd_err / (N eps), eps = 2^-53: the testers' scaled residual */
+    double history[16]; /* bwd_err after the first solve and each refinement step */
 } sla_solve_info_t;
 
 /* X = A^{-1} B (n x n FP64 dA, nrhs <= 16, device pointers) by a
@@ -148,9 +155,9 @@ typedef struct {
  * error: ||r||_inf <= ||x||_inf ||A||_inf sqrt(n) eps per column, or a stall
  * below the scaled-residual threshold 16.  dA is unchanged unless the FP64
  * fallback ran (then it holds L\U and dipiv the pivots, as sla_dgetrf);
- * dB is not modified.  *iter as DSGESV's ITER (-2 overflow in FP32, -3 zero
- * pivot in low precision, -k no convergence after k-1 steps, -1 prec was
- * FP64); *info as PDGESV's INFO. */
+ * dB is not modified.  *iter as DSGESV's ITER (-1 prec was FP64, -2 overflow
+ * in FP32, -3 zero pivot in low precision, -(ir_max_iter+1) no convergence);
+ * *info as PDGESV's INFO. */
 int sla_dxgesv(sla_handle_t h, int prec, int refine, int n, int nrhs, double *dA, int lda,
                int *dipiv, const double *dB, int ldb, double *dX, int ldx, int *iter, int *info);
 /* Details of the last sla_dxgesv on this handle. */
@@ -160,6 +167,14 @@ int sla_last_solve_info(sla_handle_t h, sla_solve_info_t *out);
  * in `prec` (SLA_PREC_FP32, _BF16 or _FP8).  Pivots and *info as sla_dgetrf. */
 int sla_sgetrf(sla_handle_t h, int prec, int m, int n, float *dA, int lda, int *dipiv,
                int *info);
+/* Solve A X = B with the FP32 factors of sla_sgetrf (FP32 right-hand sides). */
+int sla_sgetrs(sla_handle_t h, int n, int nrhs, const float *dA, int lda, const int *dipiv,
+               float *dB, int ldb);
+/* The refinement's FP64 residual alone: R = B - A X (n x n A, device
+ * pointers), rnorm[j] = ||R(:, j)||_inf (host array of nrhs doubles).  The
+ * sum order is fixed, so R is reproducible bit for bit. */
+int sla_dresidual(sla_handle_t h, int n, int nrhs, const double *dA, int lda, const double *dX,
+                  int ldx, const double *dB, int ldb, double *dR, int ldr, double *rnorm);
 /* The low-precision trailing-update kernel alone: C -= A * B (FP32
  * column-major operands, products in `prec`, FP32 accumulation). */
 int sla_gemm_lowp_sub(sla_handle_t h, int prec, int m, int n, int k, const float *dA, int lda,

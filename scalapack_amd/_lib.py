@@ -26,10 +26,13 @@ class SolveInfo(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("refinement", ctypes.c_int),
                 ("iter", ctypes.c_int), ("outer", ctypes.c_int), ("inner", ctypes.c_int),
                 ("fell_back", ctypes.c_int), ("anorm", ctypes.c_double),
-                ("bwd_err", ctypes.c_double), ("sresid", ctypes.c_double)]
+                ("bwd_err", ctypes.c_double), ("sresid", ctypes.c_double),
+                ("history", ctypes.c_double * 16)]
 
     def as_dict(self) -> dict:
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["history"] = [v for v in self.history[: min(16, self.outer + 1)]]
+        return d
 
 
 _lib = None
@@ -60,6 +63,8 @@ def _declare(lib: ctypes.CDLL) -> None:
         "sla_last_solve_info": [vp, vp],
         "sla_sgetrf": [vp, i, i, i, vp, i, vp, vp],
         "sla_gemm_lowp_sub": [vp, i, i, i, i, vp, i, vp, i, vp, i],
+        "sla_sgetrs": [vp, i, i, vp, i, vp, vp, i],
+        "sla_dresidual": [vp, i, i, vp, i, vp, i, vp, i, vp, i, vp],
     }
     for name, args in sigs.items():
         f = getattr(lib, name)

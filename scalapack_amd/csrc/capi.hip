@@ -494,7 +494,7 @@ int sla_set_stream(sla_handle_t h, void *stream)
 {
     SLA_NEED_HANDLE(h);
     auto *H = reinterpret_cast<sla::Handle *>(h);
-    SLA_TRY(H->stream = stream ? static_cast<hipStream_t>(stream) : H->own_stream)
+    SLA_TRY(H->stream = static_cast<hipStream_t>(stream))
 }
 
 int sla_set_param(sla_handle_t h, const char *name, int value)
@@ -502,7 +502,8 @@ int sla_set_param(sla_handle_t h, const char *name, int value)
     auto *H = reinterpret_cast<sla::Handle *>(h);
     const std::string k(name ? name : "");
     static const char *known[] = {"nb",      "leaf_w",  "panel_wgs",        "profile",
-                                  "ir_max_iter", "gmres_m", "ir_gmres_on_stall", "gmres_tol_exp"};
+                                  "ir_max_iter", "gmres_m", "ir_gmres_on_stall", "gmres_tol_exp",
+                                  "pivoting"};
     if (std::find_if(std::begin(known), std::end(known), [&](const char *x) { return k == x; }) ==
         std::end(known)) {
         g_last_error = "sla_set_param: unknown parameter " + k;
@@ -513,9 +514,10 @@ int sla_set_param(sla_handle_t h, const char *name, int value)
     else if (k == "leaf_w" && value >= 1 && value <= 32) H->leaf_w = value;
     else if (k == "panel_wgs" && value >= 1 && value <= 256) H->panel_wgs = value;
     else if (k == "profile") H->profile = value != 0;
-    else if (k == "ir_max_iter" && value >= 0 && value <= 1000) H->ir_max_iter = value;
+    else if (k == "ir_max_iter" && value >= 3 && value <= 1000) H->ir_max_iter = value;
     else if (k == "gmres_m" && value >= 1 && value <= 1000) H->gmres_m = value;
     else if (k == "ir_gmres_on_stall") H->ir_gmres_on_stall = value != 0;
+    else if (k == "pivoting" && (value == 0 || value == 1)) H->pivoting = value;
     else if (k == "gmres_tol_exp" && value >= 1 && value <= 15) H->gmres_tol = std::pow(10.0, -value);
     else {
         g_last_error = "sla_set_param: value out of range for " + k;
@@ -536,6 +538,7 @@ int sla_get_param(sla_handle_t h, const char *name, int *value)
     else if (k == "ir_max_iter") *value = H->ir_max_iter;
     else if (k == "gmres_m") *value = H->gmres_m;
     else if (k == "ir_gmres_on_stall") *value = H->ir_gmres_on_stall;
+    else if (k == "pivoting") *value = H->pivoting;
     else if (k == "gmres_tol_exp") *value = (int)std::lround(-std::log10(H->gmres_tol));
     else {
         g_last_error = "sla_get_param: unknown parameter " + k;
@@ -659,6 +662,7 @@ int sla_last_solve_info(sla_handle_t h, sla_solve_info_t *out)
     out->anorm = mi.anorm;
     out->bwd_err = mi.bwd_err;
     out->sresid = mi.sresid;
+    for (int i = 0; i < 16; i++) out->history[i] = mi.history[i];
     return SLA_OK;
 }
 
@@ -681,6 +685,31 @@ int sla_sgetrf(sla_handle_t h, int prec, int m, int n, float *dA, int lda, int *
     }
     SLA_TRY(*info = sla::sgetrf_device(*reinterpret_cast<sla::Handle *>(h), m, n, dA, lda, dipiv,
                                        schur_of(prec)))
+}
+
+int sla_sgetrs(sla_handle_t h, int n, int nrhs, const float *dA, int lda, const int *dipiv,
+               float *dB, int ldb)
+{
+    SLA_NEED_HANDLE(h);
+    if (n < 0 || nrhs < 0 || lda < std::max(1, n) || ldb < std::max(1, n)) {
+        g_last_error = "sla_sgetrs: illegal n, nrhs, lda or ldb";
+        return SLA_ERR_ARG;
+    }
+    SLA_TRY(sla::sgetrs_device(*reinterpret_cast<sla::Handle *>(h), n, nrhs, dA, lda, dipiv, dB,
+                               ldb))
+}
+
+int sla_dresidual(sla_handle_t h, int n, int nrhs, const double *dA, int lda, const double *dX,
+                  int ldx, const double *dB, int ldb, double *dR, int ldr, double *rnorm)
+{
+    SLA_NEED_HANDLE(h);
+    if (n < 0 || nrhs < 0 || lda < std::max(1, n) || ldx < std::max(1, n) ||
+        ldb < std::max(1, n) || ldr < std::max(1, n) || (nrhs > 0 && !rnorm)) {
+        g_last_error = "sla_dresidual: illegal dimensions or null rnorm";
+        return SLA_ERR_ARG;
+    }
+    SLA_TRY(sla::dresidual_device(*reinterpret_cast<sla::Handle *>(h), n, nrhs, dA, lda, dX, ldx,
+                                  dB, ldb, dR, ldr, rnorm))
 }
 
 int sla_gemm_lowp_sub(sla_handle_t h, int prec, int m, int n, int k, const float *dA, int lda,

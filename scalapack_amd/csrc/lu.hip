@@ -252,6 +252,9 @@ struct Lu {
 
     void gemm(int m, int n, int k, const T *a, const T *b, T *c, hipStream_t s);
 
+    // One panel leaf; returns the device-wide barrier arrivals it adds.
+    unsigned leaf(T *a, int m, int w, int lcols, int rcols, int row0, int col0, hipStream_t s);
+
     // Outer trailing update A22 -= L21 U12 of one block column (k = its
     // width): the storage-precision GEMM, or for FP32 storage the BF16 / FP8
     // MFMA update on operands packed once per block column.
@@ -283,8 +286,7 @@ struct Lu {
         const int m = M - r0;
         if (n <= h.leaf_w) {
             h.timing_begin(CAT_PANEL, s);
-            h.ctr_base += panel_leaf<T>(at(r0, c0), lda, m, n, c0 - pc0, pc1 - c0 - n, r0, c0,
-                                        ipiv, pw, h.ctr_base, s);
+            h.ctr_base += leaf(at(r0, c0), m, n, c0 - pc0, pc1 - c0 - n, r0, c0, s);
             h.timing_end((double)m * n * n, s);
             return;
         }
@@ -364,6 +366,29 @@ void Lu<double>::update(int m, int n, int k, const double *l21, const double *u1
                         hipStream_t s)
 {
     gemm(m, n, k, l21, u12, c, s);
+}
+
+template <>
+unsigned Lu<double>::leaf(double *a, int m, int w, int lcols, int rcols, int row0, int col0,
+                          hipStream_t s)
+{
+    return panel_leaf<double>(a, lda, m, w, lcols, rcols, row0, col0, ipiv, pw, h.ctr_base, s);
+}
+
+template <>
+unsigned Lu<float>::leaf(float *a, int m, int w, int lcols, int rcols, int row0, int col0,
+                         hipStream_t s)
+{
+    if (h.pivoting == 1 && panel_tslu_fits<float>(m, w, pw.target_wgs)) {
+        char *b = static_cast<char *>(h.buffer(BUF_TSLU, tslu_slot_bytes(sizeof(float))));
+        TsluSlots<float> slots;
+        slots.cand_r = reinterpret_cast<int *>(b);
+        slots.cand_v = reinterpret_cast<float *>(b + 32 * 32 * 4);
+        slots.top_v = slots.cand_v + 32 * 32 * 32;
+        return panel_tslu<float>(a, lda, m, w, lcols, rcols, row0, col0, ipiv, pw, slots,
+                                 h.ctr_base, s);
+    }
+    return panel_leaf<float>(a, lda, m, w, lcols, rcols, row0, col0, ipiv, pw, h.ctr_base, s);
 }
 
 template <>
@@ -463,7 +488,7 @@ int sgetrf_device(Handle &h, int m, int n, float *dA, int lda, int *dipiv, int s
 
 // Row permutation of the factorization from IPIV alone (for a solve that did
 // not follow this handle's own factorization).
-static void perm_from_ipiv(Handle &h, int n, const int *dipiv)
+void perm_from_ipiv(Handle &h, int n, const int *dipiv)
 {
     h.ensure_perm(n);
     iota_int(h.perm, n, 0, h.stream);
@@ -493,10 +518,10 @@ int dgetrs_device(Handle &h, int n, int nrhs, const double *dA, int lda, const i
         gather_rows<double>(Bj, ldb, W, n, h.perm, n, nr, h.stream);
         h.timing_end(16.0 * n * nr);
         h.timing_begin(CAT_TRSV);
-        trsv_persistent<double>(true, n, nr, dA, lda, W, n, tw, h.stream);
+        trsv_persistent<double, double>(true, n, nr, dA, lda, W, n, tw, h.stream);
         h.timing_end(4.0 * (double)n * n);
         h.timing_begin(CAT_TRSV);
-        trsv_persistent<double>(false, n, nr, dA, lda, W, n, tw, h.stream);
+        trsv_persistent<double, double>(false, n, nr, dA, lda, W, n, tw, h.stream);
         h.timing_end(4.0 * (double)n * n);
         SLA_HIP(hipMemcpy2DAsync(Bj, (size_t)ldb * sizeof(double), W, (size_t)n * sizeof(double),
                                  (size_t)n * sizeof(double), nr, hipMemcpyDeviceToDevice, h.stream));

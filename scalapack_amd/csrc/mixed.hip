@@ -12,7 +12,8 @@
 // ITER is negative:
 //   -2   converting A to FP32 overflowed
 //   -3   the low-precision factorization hit an exactly zero pivot
-//   -k   refinement did not converge in k-1 steps (k-1 = max_iter or a stall)
+//   -(max_iter+1)  refinement did not converge (max_iter steps, or a stall
+//        above the threshold; DSGESV's -31 for its ITERMAX = 30)
 // Convergence test (DSGESV's): ||r||_inf <= ||x||_inf ||A||_inf sqrt(N) eps
 // per column, eps = 2^-53.  A step that does not at least halve the
 // normwise backward error counts as a stall (the stopping rule of PDGERFS,
@@ -254,14 +255,6 @@ __global__ __launch_bounds__(256) void combine_kernel(int n, int nj, const doubl
     if ((threadIdx.x & 63) == 0) atomicMax(xmax, (unsigned long long)__double_as_longlong(a));
 }
 
-// z = (double) U^{-1} L^{-1} (float) P v is composed from gather_narrow,
-// the two persistent triangular solves and this widening copy.
-__global__ void widen_kernel(int n, const float *__restrict__ w, double *__restrict__ z)
-{
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < n) z[i] = (double)w[i];
-}
-
 unsigned blocks_for(size_t n) { return (unsigned)((n + 255) / 256); }
 
 // ---------------------------------------------------------------------------
@@ -338,10 +331,27 @@ struct Mixed {
         h.timing_end(12.0 * n * nr);
         TrsvWork tw = h.trsv_work();
         h.timing_begin(CAT_TRSV);
-        trsv_persistent<float>(true, n, nr, S, lds, W, n, tw, st());
+        trsv_persistent<float, float>(true, n, nr, S, lds, W, n, tw, st());
         h.timing_end(2.0 * (double)n * n);
         h.timing_begin(CAT_TRSV);
-        trsv_persistent<float>(false, n, nr, S, lds, W, n, tw, st());
+        trsv_persistent<float, float>(false, n, nr, S, lds, W, n, tw, st());
+        h.timing_end(2.0 * (double)n * n);
+        h.trsv_epoch = tw.epoch;
+    }
+
+    // Z = U^{-1} L^{-1} P V (one column, FP64 arithmetic on the FP32
+    // factors: GMRES-IR's preconditioner, applied in the working precision).
+    void precond64(const double *v, double *z)
+    {
+        h.timing_begin(CAT_LASWP);
+        gather_rows<double>(v, n, z, n, h.perm, n, 1, st());
+        h.timing_end(16.0 * n);
+        TrsvWork tw = h.trsv_work();
+        h.timing_begin(CAT_TRSV);
+        trsv_persistent<float, double>(true, n, 1, S, lds, z, n, tw, st());
+        h.timing_end(2.0 * (double)n * n);
+        h.timing_begin(CAT_TRSV);
+        trsv_persistent<float, double>(false, n, 1, S, lds, z, n, tw, st());
         h.timing_end(2.0 * (double)n * n);
         h.trsv_epoch = tw.epoch;
     }
@@ -412,9 +422,7 @@ struct Mixed {
         for (; k < mmax; k++) {
             double *vk = V + nn * k, *zk = Z + nn * k, *w = V + nn * (k + 1);
             // z_k = M^{-1} v_k
-            lowp_solve(vk, n, 1);
-            hipLaunchKernelGGL(widen_kernel, dim3(blocks_for(n)), dim3(256), 0, st(), n, W, zk);
-            SLA_LAUNCH_CHECK();
+            precond64(vk, zk);
             // w = A z_k (FP64)
             residual(nullptr, n, 0.0, zk, n, w, n, 1, nullptr);
             hipLaunchKernelGGL(scale_copy_kernel, dim3(blocks_for(n)), dim3(256), 0, st(), n, -1.0,
@@ -478,6 +486,47 @@ struct Mixed {
 
 }  // namespace
 
+void dresidual_device(Handle &h, int n, int nrhs, const double *dA, int lda, const double *dX,
+                      int ldx, const double *dB, int ldb, double *dR, int ldr, double *rnorm)
+{
+    if (n <= 0 || nrhs <= 0) return;
+    Mixed mx{h, n, nrhs, const_cast<double *>(dA), lda, dB, ldb, nullptr, 0};
+    mx.alloc();
+    for (int j = 0; j < nrhs; j += 16) {
+        const int nr = std::min(16, nrhs - j);
+        mx.zero_norms(0, nr);
+        mx.residual(dB + (size_t)j * ldb, ldb, 1.0, dX + (size_t)j * ldx, ldx,
+                    dR + (size_t)j * ldr, ldr, nr, mx.norms);
+        const std::vector<double> v = mx.read_norms(0, nr);
+        for (int c = 0; c < nr; c++) rnorm[j + c] = v[c];
+    }
+}
+
+void sgetrs_device(Handle &h, int n, int nrhs, const float *dA, int lda, const int *dipiv,
+                   float *dB, int ldb)
+{
+    if (n <= 0 || nrhs <= 0) return;
+    if (!(h.perm_key_a == dA && h.perm_key_ipiv == dipiv && h.perm_m == n))
+        perm_from_ipiv(h, n, dipiv);
+    float *W = static_cast<float *>(h.buffer(BUF_VEC, (size_t)n * 16 * 4 + 256));
+    TrsvWork tw = h.trsv_work();
+    fill_int(h.abort_flag, 1, 0, h.stream);
+    for (int j = 0; j < nrhs; j += 16) {
+        const int nr = std::min(16, nrhs - j);
+        float *Bj = dB + (size_t)j * ldb;
+        gather_rows<float>(Bj, ldb, W, n, h.perm, n, nr, h.stream);
+        trsv_persistent<float, float>(true, n, nr, dA, lda, W, n, tw, h.stream);
+        trsv_persistent<float, float>(false, n, nr, dA, lda, W, n, tw, h.stream);
+        SLA_HIP(hipMemcpy2DAsync(Bj, (size_t)ldb * 4, W, (size_t)n * 4, (size_t)n * 4, nr,
+                                 hipMemcpyDeviceToDevice, h.stream));
+    }
+    h.trsv_epoch = tw.epoch;
+    int flags[2];
+    SLA_HIP(hipMemcpyAsync(flags, h.ctl, sizeof flags, hipMemcpyDeviceToHost, h.stream));
+    SLA_HIP(hipStreamSynchronize(h.stream));
+    if (flags[1]) throw std::runtime_error("sgetrs: a dependency wait timed out");
+}
+
 int dxgesv_device(Handle &h, int prec, int refine, int n, int nrhs, double *dA, int lda,
                   int *dipiv, const double *dB, int ldb, double *dX, int ldx, MixedInfo &info)
 {
@@ -489,9 +538,11 @@ int dxgesv_device(Handle &h, int prec, int refine, int n, int nrhs, double *dA, 
     const double cte = std::sqrt((double)n) * EPS64;
 
     auto fallback = [&](int iter) {
-        // FP64 factorization in place and solve of X = A^{-1} B
+        // FP64 factorization in place and solve of X = A^{-1} B; A then holds
+        // the factors, so the errors of this X are not measured (NaN)
         info.fell_back = 1;
         info.iter = iter;
+        info.bwd_err = info.sresid = std::nan("");
         SLA_HIP(hipMemcpy2DAsync(dX, (size_t)ldx * 8, dB, (size_t)ldb * 8, (size_t)n * 8, nrhs,
                                  hipMemcpyDeviceToDevice, h.stream));
         const int zi = dgetrf_device(h, n, n, dA, lda, dipiv);
@@ -499,6 +550,7 @@ int dxgesv_device(Handle &h, int prec, int refine, int n, int nrhs, double *dA, 
         dgetrs_device(h, n, nrhs, dA, lda, dipiv, dX, ldx);
         return 0;
     };
+
     if (prec == PREC_FP64) return fallback(-1);
 
     Mixed mx{h, n, nrhs, dA, lda, dB, ldb, dX, ldx};
@@ -562,6 +614,7 @@ int dxgesv_device(Handle &h, int prec, int refine, int n, int nrhs, double *dA, 
         info.bwd_err = worst;
         info.sresid = worst / (n * EPS64);
         info.outer = it;
+        if (it < 16) info.history[it] = worst;
         if (converged || (stalled && acceptable)) {
             info.iter = it;
             return 0;
@@ -581,7 +634,7 @@ int dxgesv_device(Handle &h, int prec, int refine, int n, int nrhs, double *dA, 
             mx.correct_classical();
         }
     }
-    return fallback(-(info.outer + 1));
+    return fallback(-(h.ir_max_iter + 1));
 }
 
 }  // namespace sla

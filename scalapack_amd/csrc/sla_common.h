@@ -133,6 +133,20 @@ __device__ __forceinline__ bool grid_sync(unsigned *ctr, unsigned target, int *a
 // (|v|, row) ordering of the pivot search: larger magnitude wins, the lower
 // row wins a tie -- IDAMAX's "first index of the maximum" on the current row
 // order (PBLAS/SRC/pdamax_.c:421-425).
+// Value of v in lane `lane` (uniform index) of the wave.
+template <typename T>
+__device__ __forceinline__ T lane_read(T v, int lane)
+{
+    if constexpr (sizeof(T) == 8) {
+        const uint64_t u = __builtin_bit_cast(uint64_t, v);
+        const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, lane);
+        const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), lane);
+        return __builtin_bit_cast(T, ((uint64_t)hi << 32) | lo);
+    } else {
+        return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(uint32_t, v), lane));
+    }
+}
+
 __device__ __forceinline__ bool pivot_better(double v1, int r1, double v2, int r2)
 {
     return v1 > v2 || (v1 == v2 && r1 < r2);

@@ -30,6 +30,7 @@ struct MixedInfo {
     double anorm = 0.0;  // ||A||_inf
     double bwd_err = 0.0;  // max_j ||r_j||_inf / (||A||_inf ||x_j||_inf) of the last residual
     double sresid = 0.0;   // the same divided by N eps (the testers' scaled residual)
+    double history[16] = {};  // bwd_err after the first solve and each refinement step
 };
 
 // Reusable device buffers, grown on demand.
@@ -42,6 +43,7 @@ enum BufId {
     BUF_VEC,         // refinement vectors (FP64 / FP32)
     BUF_KRYLOV,      // GMRES bases
     BUF_PARTIAL,     // partial sums of the residual / reductions
+    BUF_TSLU,        // candidate exchange of the tournament-pivoting leaf
     BUF_COUNT
 };
 
@@ -59,7 +61,8 @@ struct Handle {
 
     int device = 0;
     hipStream_t own_stream = nullptr;
-    hipStream_t stream = nullptr;   // main stream (the caller's, or own_stream)
+    hipStream_t stream = nullptr;   // main stream (own_stream, or the caller's; may be
+                                    // HIP's default stream, nullptr)
     hipStream_t stream2 = nullptr;  // look-ahead panel stream (default priority)
     hipEvent_t ev_next = nullptr, ev_panel = nullptr, ev_join = nullptr;
     uint64_t spin_limit = 0;
@@ -105,6 +108,7 @@ struct Handle {
     int nb = 256;          // block column width of the outer factorization
     int leaf_w = 32;       // panel leaf width (columns per device-wide-barrier kernel)
     int panel_wgs = 32;    // preferred workgroups per leaf launch
+    int pivoting = 1;      // FP32-storage factorization: 0 partial, 1 tournament
     // mixed-precision refinement
     int ir_max_iter = 30;        // refinement steps before the FP64 fallback (DSGESV's ITERMAX)
     int ir_gmres_on_stall = 1;   // classical refinement that stalls continues as GMRES-IR
@@ -146,6 +150,8 @@ int dgetrf_device(Handle &h, int m, int n, double *dA, int lda, int *dipiv);
 int sgetrf_device(Handle &h, int m, int n, float *dA, int lda, int *dipiv, int schur);
 int dgetrs_device(Handle &h, int n, int nrhs, const double *dA, int lda, const int *dipiv,
                   double *dB, int ldb);
+// The factorization's net row permutation rebuilt from IPIV alone.
+void perm_from_ipiv(Handle &h, int n, const int *dipiv);
 // PDLASWP on a device matrix: row interchanges k1 .. k2-1 (0-based) with
 // rows ipiv[k]-1 over ncols columns, forward or backward; and the column form.
 void laswp_rows_device(Handle &h, double *dA, int lda, int ncols, int k1, int k2,

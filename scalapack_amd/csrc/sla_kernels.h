@@ -69,6 +69,25 @@ template <typename T>
 unsigned panel_leaf(T *A, int lda, int m, int wa, int lcols, int rcols, int row0, int col0,
                     int *ipiv, const PanelWork<T> &pw, unsigned base, hipStream_t s);
 
+// ---- panel leaf with tournament pivoting (low-precision factorization) -------
+template <typename T>
+struct TsluSlots {
+    int *cand_r;  // [32 workgroups][32] candidate rows (leaf-relative)
+    T *cand_v;    // [32][32][32] their original values
+    T *top_v;     // [32][32] the leaf's top rows, original values
+};
+constexpr size_t tslu_slot_bytes(size_t esz) { return 32 * 32 * 4 + (32 * 32 * 32 + 32 * 32) * esz + 256; }
+// Whether the tournament leaf handles an m x wa leaf (wa <= 32 <= m, m at
+// most 32768 rows).
+template <typename T>
+bool panel_tslu_fits(int m, int wa, int target_wgs);
+// Factors the m x wa leaf like panel_leaf, choosing the pivots by a
+// tournament (one device-wide barrier); returns the barrier arrivals.
+template <typename T>
+unsigned panel_tslu(T *A, int lda, int m, int wa, int lcols, int rcols, int row0, int col0,
+                    int *ipiv, const PanelWork<T> &pw, const TsluSlots<T> &slots, unsigned base,
+                    hipStream_t s);
+
 // ---- row interchanges (PDLASWP) --------------------------------------------
 // Net effect of the forward interchanges ipiv[k1 .. k2-1] (0-based global row
 // indices of the pivoted rows, ipiv 1-based global) as a list of
@@ -102,8 +121,10 @@ struct TrsvWork {
     int *abort_flag;
     uint64_t spin_limit;
 };
-template <typename T>
-void trsv_persistent(bool forward, int n, int nrhs, const T *A, int lda, T *X, int ldx,
+// TA: storage type of the triangle, T: type of X and of the arithmetic
+// (<double, double>, <float, float>, <float, double>).
+template <typename TA, typename T>
+void trsv_persistent(bool forward, int n, int nrhs, const TA *A, int lda, T *X, int ldx,
                      TrsvWork &tw, hipStream_t s);
 
 // ---- small helpers ---------------------------------------------------------

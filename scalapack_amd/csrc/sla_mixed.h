@@ -12,4 +12,11 @@ namespace sla {
 int dxgesv_device(Handle &h, int prec, int refine, int n, int nrhs, double *dA, int lda,
                   int *dipiv, const double *dB, int ldb, double *dX, int ldx, MixedInfo &info);
 
+// R = B - A X in FP64 (reproducible two-stage GEMV), rnorm[j] = max_i |R(i,j)|.
+void dresidual_device(Handle &h, int n, int nrhs, const double *dA, int lda, const double *dX,
+                      int ldx, const double *dB, int ldb, double *dR, int ldr, double *rnorm);
+// B = A^{-1} B with FP32 factors of sgetrf_device.
+void sgetrs_device(Handle &h, int n, int nrhs, const float *dA, int lda, const int *dipiv,
+                   float *dB, int ldb);
+
 }  // namespace sla

@@ -65,26 +65,15 @@ __global__ __launch_bounds__(256) void trsm_llnu_strip_kernel(int m, int n, cons
 // ---------------------------------------------------------------------------
 constexpr int TV_BS = 64, TV_NR = 16;
 
-template <typename T>
-__device__ __forceinline__ T lane_read(T v, int lane)
-{
-    if constexpr (sizeof(T) == 8) {
-        const uint64_t u = __builtin_bit_cast(uint64_t, v);
-        const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, lane);
-        const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), lane);
-        return __builtin_bit_cast(T, ((uint64_t)hi << 32) | lo);
-    } else {
-        return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(uint32_t, v), lane));
-    }
-}
-
 // FORWARD: X = L^{-1} X with L unit lower.  Otherwise X = U^{-1} X with U
 // upper, non-unit, scaled by the reciprocals of its diagonal, computed before
 // any wait so they stay off the dependency chain.  X is n x nrhs
 // (nrhs <= NR <= 16) with leading dimension ldx; the matrix is n x n.
-template <typename T, bool FORWARD, int NR>
+// TA: storage type of the triangle; T: type of X and of the arithmetic (a
+// FP32 factorization may be applied in FP64, as GMRES-IR's preconditioner).
+template <typename TA, typename T, bool FORWARD, int NR>
 __global__ __launch_bounds__(256) void trsv_persistent_kernel(
-    int n, int nrhs, const T *__restrict__ A, int lda, T *X, int ldx, int *ticket, int *flags,
+    int n, int nrhs, const TA *__restrict__ A, int lda, T *X, int ldx, int *ticket, int *flags,
     int epoch, int *abort_flag, uint64_t spin_limit)
 {
     __shared__ int s_b, s_abort;
@@ -105,10 +94,10 @@ __global__ __launch_bounds__(256) void trsv_persistent_kernel(
     // other blocks); wave 0 lane r keeps 1 / U(r, r).
     for (int idx = tid; idx < TV_BS * TV_BS; idx += 256) {
         const int r = idx % TV_BS, k = idx / TV_BS;
-        Ds[r][k] = (r < rows && k < rows) ? A[(size_t)(rb + r) + (size_t)(rb + k) * lda] : T(0);
+        Ds[r][k] = (r < rows && k < rows) ? (T)A[(size_t)(rb + r) + (size_t)(rb + k) * lda] : T(0);
     }
     T rdiag = T(1);
-    if (!FORWARD && wave == 0 && lane < rows) rdiag = T(1) / A[(size_t)(rb + lane) * (lda + 1)];
+    if (!FORWARD && wave == 0 && lane < rows) rdiag = T(1) / (T)A[(size_t)(rb + lane) * (lda + 1)];
 
     T part[NR];
 #pragma unroll
@@ -146,12 +135,12 @@ __global__ __launch_bounds__(256) void trsv_persistent_kernel(
         __syncthreads();
         // wave w covers k in [16w, 16w+16) of the 64 x 64 block A[rb.., cb..]
         if (lane < rows) {
-            const T *Ar = A + (size_t)(rb + lane) + (size_t)cb * lda;
+            const TA *Ar = A + (size_t)(rb + lane) + (size_t)cb * lda;
 #pragma unroll 4
             for (int kk = 0; kk < 16; kk++) {
                 const int k = wave * 16 + kk;
                 if (k < crows) {
-                    const T a = Ar[(size_t)k * lda];
+                    const T a = (T)Ar[(size_t)k * lda];
 #pragma unroll
                     for (int j = 0; j < NR; j++) part[j] = fma(a, Xs[k][j], part[j]);
                 }
@@ -220,8 +209,8 @@ void trsm_llnu(int m, int n, const T *L, int ldl, T *B, int ldb, hipStream_t s)
 template void trsm_llnu<double>(int, int, const double *, int, double *, int, hipStream_t);
 template void trsm_llnu<float>(int, int, const float *, int, float *, int, hipStream_t);
 
-template <typename T>
-void trsv_persistent(bool forward, int n, int nrhs, const T *A, int lda, T *X, int ldx,
+template <typename TA, typename T>
+void trsv_persistent(bool forward, int n, int nrhs, const TA *A, int lda, T *X, int ldx,
                      TrsvWork &tw, hipStream_t s)
 {
     if (n <= 0 || nrhs <= 0) return;
@@ -231,7 +220,7 @@ void trsv_persistent(bool forward, int n, int nrhs, const T *A, int lda, T *X, i
     fill_int(tw.ticket, 1, 0, s);
     const int epoch = ++tw.epoch;
 #define SLA_TRSV(FW, NRV)                                                                       \
-    hipLaunchKernelGGL((trsv_persistent_kernel<T, FW, NRV>), dim3(nblk), dim3(256), 0, s, n,    \
+    hipLaunchKernelGGL((trsv_persistent_kernel<TA, T, FW, NRV>), dim3(nblk), dim3(256), 0, s, n, \
                        nrhs, A, lda, X, ldx, tw.ticket, tw.flags, epoch, tw.abort_flag,        \
                        tw.spin_limit)
 #define SLA_TRSV_NR(FW)                           \
@@ -248,7 +237,11 @@ void trsv_persistent(bool forward, int n, int nrhs, const T *A, int lda, T *X, i
 #undef SLA_TRSV
     SLA_LAUNCH_CHECK();
 }
-template void trsv_persistent<double>(bool, int, int, const double *, int, double *, int, TrsvWork &, hipStream_t);
-template void trsv_persistent<float>(bool, int, int, const float *, int, float *, int, TrsvWork &, hipStream_t);
+template void trsv_persistent<double, double>(bool, int, int, const double *, int, double *, int,
+                                              TrsvWork &, hipStream_t);
+template void trsv_persistent<float, float>(bool, int, int, const float *, int, float *, int,
+                                            TrsvWork &, hipStream_t);
+template void trsv_persistent<float, double>(bool, int, int, const float *, int, double *, int,
+                                             TrsvWork &, hipStream_t);
 
 }  // namespace sla

@@ -36,13 +36,23 @@ def _ptr(t) -> ctypes.c_void_p:
 
 
 class Handle:
-    """One library handle (streams, workspace, tuning) on the current device."""
+    """One library handle (streams, workspace, tuning) on the current device.
 
-    def __init__(self, **params):
+    The handle's work is ordered on one HIP stream.  By default that is
+    torch's current stream, so tensors produced by torch are ready when a call
+    starts; pass stream=None to keep the handle's own stream (then the caller
+    synchronises its producers first)."""
+
+    def __init__(self, stream="torch", **params):
         self._lib = load()
         h = ctypes.c_void_p()
         _check(self._lib.sla_create(ctypes.byref(h)))
         self._h = h
+        if stream == "torch":
+            import torch
+            stream = torch.cuda.current_stream()
+        if stream is not None:
+            self.set_stream(stream)
         for k, v in params.items():
             self.set(k, v)
 
@@ -66,8 +76,9 @@ class Handle:
         return v.value
 
     def set_stream(self, stream) -> None:
-        """stream: a torch.cuda.Stream, a raw hipStream_t int, or None."""
-        raw = None if stream is None else (stream if isinstance(stream, int) else stream.cuda_stream)
+        """stream: a torch.cuda.Stream or a raw hipStream_t int (0: HIP's
+        default stream)."""
+        raw = stream if isinstance(stream, int) else stream.cuda_stream
         _check(self._lib.sla_set_stream(self._h, ctypes.c_void_p(raw)))
 
     def stats(self) -> dict[str, dict[str, float]]:
@@ -132,3 +143,15 @@ class Handle:
         """C -= A B, FP32 operands, products in `prec`, FP32 accumulation."""
         _check(self._lib.sla_gemm_lowp_sub(self._h, PRECISIONS[prec], m, n, k, _ptr(a), lda,
                                            _ptr(b), ldb, _ptr(c), ldc))
+
+    def sgetrs(self, n: int, nrhs: int, a, lda: int, ipiv, b, ldb: int) -> None:
+        """B = A^{-1} B with the FP32 factors of sgetrf."""
+        _check(self._lib.sla_sgetrs(self._h, n, nrhs, _ptr(a), lda, _ptr(ipiv), _ptr(b), ldb))
+
+    def dresidual(self, n: int, nrhs: int, a, lda: int, x, ldx: int, b, ldb: int, r,
+                  ldr: int) -> list[float]:
+        """R = B - A X in FP64; returns ||R(:, j)||_inf per column."""
+        out = (ctypes.c_double * max(1, nrhs))()
+        _check(self._lib.sla_dresidual(self._h, n, nrhs, _ptr(a), lda, _ptr(x), ldx, _ptr(b),
+                                       ldb, _ptr(r), ldr, out))
+        return list(out[:nrhs])
