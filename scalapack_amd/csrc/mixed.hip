@@ -27,6 +27,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -399,6 +401,7 @@ struct Mixed {
     // number of Arnoldi steps taken.
     int correct_gmres(int j, int mmax, double tol)
     {
+        static const bool trace = getenv("SLA_TRACE_GMRES") != nullptr;
         const size_t nn = (size_t)n;
         double *V = static_cast<double *>(h.buffer(BUF_KRYLOV, nn * (2 * mmax + 1) * 8 + 8 * 4096));
         double *Z = V + nn * (mmax + 1);
@@ -463,6 +466,8 @@ struct Mixed {
             hc[k + 1] = 0.0;
             g[k + 1] = -sn[k] * g[k];
             g[k] = cs[k] * g[k];
+            if (trace) fprintf(stderr, "gmres step %d: |g|/beta = %.3e  h(k+1,k) = %.3e\n", k + 1,
+                               std::fabs(g[k + 1]) / beta, hk1);
             if (std::fabs(g[k + 1]) <= tol * beta || hk1 == 0.0) {
                 k++;
                 break;
@@ -548,6 +553,7 @@ int dxgesv_device(Handle &h, int prec, int refine, int n, int nrhs, double *dA, 
         const int zi = dgetrf_device(h, n, n, dA, lda, dipiv);
         if (zi > 0) return zi;
         dgetrs_device(h, n, nrhs, dA, lda, dipiv, dX, ldx);
+        h.timing_collect();
         return 0;
     };
 
@@ -617,6 +623,7 @@ int dxgesv_device(Handle &h, int prec, int refine, int n, int nrhs, double *dA, 
         if (it < 16) info.history[it] = worst;
         if (converged || (stalled && acceptable)) {
             info.iter = it;
+            h.timing_collect();
             return 0;
         }
         if (it >= h.ir_max_iter) break;

@@ -48,21 +48,38 @@ constexpr int TGMAX = TNW * 64 * TMR / TW;  // workgroups per launch at most (32
 // key = row identity and tie-break, INT_MAX = no row; pid = payload id).
 // Selects up to wa pivots in order: out_key[j], out_pid[j] (uniform, written
 // by lane 0), and with OUT_LU the pivot row's values when chosen
-// (multipliers left of j, U from j on).  Column j is a template parameter so
-// that every register index is a compile-time constant.
-template <typename T, int NR, bool OUT_LU, bool FULL, int J>
-__device__ __forceinline__ void gepp_step(T (&v)[NR][TW], const int (&key)[NR],
-                                          const int (&pid)[NR], bool (&act)[NR], int wa,
-                                          int *out_key, int *out_pid, T (*out_lu)[TW + 1])
+// (multipliers left of j, U from j on).
+//
+// The columns rotate through the registers: at step j, register column 0
+// holds matrix column j, and the rank-1 update writes column c+1's new value
+// into register column c.  Every register index is a compile-time constant
+// while the step loop itself stays rolled (one copy of the body).  With
+// OUT_LU the multipliers rotate in from the right (mul[.][W-1] is the newest).
+template <typename T, int NR, bool OUT_LU, bool FULL>
+__device__ __forceinline__ void wave_gepp(T (&v)[NR][TW], const int (&key)[NR],
+                                          const int (&pid)[NR], int wa, int *out_key,
+                                          int *out_pid, T (*out_lu)[TW + 1])
 {
     const int lane = threadIdx.x & 63;
-    if (FULL || J < wa) {
+    bool act[NR];
+#pragma unroll
+    for (int q = 0; q < NR; q++) act[q] = key[q] != INT_MAX;
+    T mul[OUT_LU ? NR : 1][TW];
+    if constexpr (OUT_LU) {
+#pragma unroll
+        for (int q = 0; q < NR; q++)
+#pragma unroll
+            for (int c = 0; c < TW; c++) mul[q][c] = T(0);
+    }
+    const int steps = FULL ? TW : wa;
+#pragma unroll 1
+    for (int j = 0; j < steps; j++) {
         double bv = -1.0;
         int bk = INT_MAX, bq = 0;
 #pragma unroll
         for (int q = 0; q < NR; q++) {
             if (act[q]) {
-                const double x = fabs((double)v[q][J]);
+                const double x = fabs((double)v[q][0]);
                 if (pivot_better(x, key[q], bv, bk)) {
                     bv = x;
                     bk = key[q];
@@ -81,65 +98,72 @@ __device__ __forceinline__ void gepp_step(T (&v)[NR][TW], const int (&key)[NR],
                 wk = ok;
             }
         }
-        if (wk != INT_MAX) {  // uniform
-            const bool mine = (bk == wk);
-            const int wl = __builtin_ctzll(__ballot(mine));
-            // the pivot row: columns J.. (and the multipliers left of J for OUT_LU)
-            constexpr int C0 = OUT_LU ? 0 : J;
-            T prow[TW];
-#pragma unroll
-            for (int c = C0; c < TW; c++) {
-                T sel = v[0][c];
-#pragma unroll
-                for (int q = 1; q < NR; q++) sel = (bq == q) ? v[q][c] : sel;
-                prow[c] = lane_read(sel, wl);
-            }
-            int sp = pid[0];
-#pragma unroll
-            for (int q = 1; q < NR; q++) sp = (bq == q) ? pid[q] : sp;
-            const int wpid = __builtin_amdgcn_readlane(sp, wl);
+        if (wk == INT_MAX) {  // uniform: no rows left
             if (lane == 0) {
-                out_key[J] = wk;
-                out_pid[J] = wpid;
+                out_key[j] = INT_MAX;
+                out_pid[j] = -1;
+            }
+            continue;
+        }
+        const bool mine = (bk == wk);
+        const int wl = __builtin_ctzll(__ballot(mine));
+        T prow[TW];
+#pragma unroll
+        for (int c = 0; c < TW; c++) {
+            T sel = v[0][c];
+#pragma unroll
+            for (int q = 1; q < NR; q++) sel = (bq == q) ? v[q][c] : sel;
+            prow[c] = lane_read(sel, wl);
+        }
+        int sp = pid[0];
+#pragma unroll
+        for (int q = 1; q < NR; q++) sp = (bq == q) ? pid[q] : sp;
+        const int wpid = __builtin_amdgcn_readlane(sp, wl);
+        if constexpr (OUT_LU) {
+            // row j of L11\U11: multipliers l_0..l_{j-1} are mul[TW-j..TW-1]
+            // of the pivot row, U(j, j..) is prow[0..TW-1-j]
+            T pm[TW];
+#pragma unroll
+            for (int c = 0; c < TW; c++) {
+                T sel = mul[0][c];
+#pragma unroll
+                for (int q = 1; q < NR; q++) sel = (bq == q) ? mul[q][c] : sel;
+                pm[c] = lane_read(sel, wl);
+            }
+            if (lane == 0) {
+#pragma unroll
+                for (int c = 0; c < TW; c++) {
+                    if (c < TW - j) out_lu[j][j + c] = prow[c];
+                    if (c >= TW - j) out_lu[j][c - (TW - j)] = pm[c];
+                }
+            }
+        }
+        if (lane == 0) {
+            out_key[j] = wk;
+            out_pid[j] = wpid;
+        }
+#pragma unroll
+        for (int q = 0; q < NR; q++)
+            if (mine && bq == q) act[q] = false;
+        const T pj = prow[0];
+        const bool nz = pj != T(0);
+        const T rcp = nz ? T(1) / pj : T(0);
+#pragma unroll
+        for (int q = 0; q < NR; q++) {
+            if (act[q]) {
+                T l = v[q][0];
+                if (nz) l = l * rcp;
+#pragma unroll
+                for (int c = 1; c < TW; c++) v[q][c - 1] = fma(-l, prow[c], v[q][c]);
+                v[q][TW - 1] = T(0);
                 if constexpr (OUT_LU) {
 #pragma unroll
-                    for (int c = 0; c < TW; c++) out_lu[J][c] = prow[c];
+                    for (int c = 1; c < TW; c++) mul[q][c - 1] = mul[q][c];
+                    mul[q][TW - 1] = l;
                 }
             }
-#pragma unroll
-            for (int q = 0; q < NR; q++)
-                if (mine && bq == q) act[q] = false;
-            const T pj = prow[J];
-            const bool nz = pj != T(0);
-            const T rcp = nz ? T(1) / pj : T(0);
-#pragma unroll
-            for (int q = 0; q < NR; q++) {
-                if (act[q]) {
-                    T l = v[q][J];
-                    if (nz) l = l * rcp;
-                    v[q][J] = l;
-#pragma unroll
-                    for (int c = J + 1; c < TW; c++) v[q][c] = fma(-l, prow[c], v[q][c]);
-                }
-            }
-        } else if (lane == 0) {
-            out_key[J] = INT_MAX;
-            out_pid[J] = -1;
         }
     }
-    if constexpr (J + 1 < TW)
-        gepp_step<T, NR, OUT_LU, FULL, J + 1>(v, key, pid, act, wa, out_key, out_pid, out_lu);
-}
-
-template <typename T, int NR, bool OUT_LU, bool FULL>
-__device__ __forceinline__ void wave_gepp(T (&v)[NR][TW], const int (&key)[NR],
-                                          const int (&pid)[NR], int wa, int *out_key,
-                                          int *out_pid, T (*out_lu)[TW + 1])
-{
-    bool act[NR];
-#pragma unroll
-    for (int q = 0; q < NR; q++) act[q] = key[q] != INT_MAX;
-    gepp_step<T, NR, OUT_LU, FULL, 0>(v, key, pid, act, wa, out_key, out_pid, out_lu);
 }
 
 // Loads candidate rows for an elimination level: NR per lane, the id of

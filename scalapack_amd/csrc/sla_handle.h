@@ -108,7 +108,7 @@ struct Handle {
     int nb = 256;          // block column width of the outer factorization
     int leaf_w = 32;       // panel leaf width (columns per device-wide-barrier kernel)
     int panel_wgs = 32;    // preferred workgroups per leaf launch
-    int pivoting = 1;      // FP32-storage factorization: 0 partial, 1 tournament
+    int pivoting = 0;      // FP32-storage factorization: 0 partial, 1 tournament
     // mixed-precision refinement
     int ir_max_iter = 30;        // refinement steps before the FP64 fallback (DSGESV's ITERMAX)
     int ir_gmres_on_stall = 1;   // classical refinement that stalls continues as GMRES-IR
