@@ -119,13 +119,24 @@ def svd_matrices(torch, n: int, kappas, seed: int = 100):
         torch.cuda.synchronize()
 
 
+KERNEL_OF = {  # kernel symbol behind each timing class (for matching kernel traces)
+    ("gemm", "fp64"): "dgemm_nn_sub_direct_kernel<1> (trailing update A22 -= L21 U12)",
+    ("gemm", "fp32"): "sgemm_nn_sub_direct_kernel<1> (trailing update, FP32 MFMA)",
+    ("gemm", "bf16"): "gemm_bf16_nn_sub_direct_kernel (trailing update, BF16 MFMA)",
+    ("gemm", "fp8"): "gemm_fp8_nn_sub_direct_kernel (trailing update, FP8 MFMA)",
+    ("panel", "fp64"): "panel_leaf_kernel<double, 32, RPT>",
+}
+
+
 def roofline_from(stats: dict, steps: int, gemm_prec: str) -> dict:
     """Roofline of the kernel class with the most summed time."""
     cls, st = max(stats.items(), key=lambda kv: kv[1]["ms"])
     secs = st["ms"] / 1e3
-    base = {"kernel_class": cls, "launches_per_step": st["launches"] // max(1, steps),
-            "avg_launch_ms": round(st["ms"] / max(1, st["launches"]), 4)}
-    if cls in ("gemm", "trsm", "panel"):
+    base = {"kernel_class": cls, "kernel": KERNEL_OF.get((cls, gemm_prec), cls),
+            "launches_per_step": st["launches"] // max(1, steps),
+            "avg_launch_ms": round(st["ms"] / max(1, st["launches"]), 4),
+            "avg_launch_work": st["work"] / max(1, st["launches"])}
+    if cls in ("gemm", "gemm_panel", "trsm", "panel"):
         # flops in the GEMM's product precision; panel and trsm compute in the
         # storage precision (FP64, or FP32 for every mixed factorization)
         achieved = st["work"] / secs / 1e12 if secs > 0 else 0.0

@@ -98,7 +98,9 @@ int sla_get_param(sla_handle_t h, const char *name, int *value);
 const char *sla_last_error(void);
 
 /* Per-kernel-class timing collected while "profile" is 1: classes
- * 0 panel, 1 laswp, 2 trsm, 3 gemm, 4 trsv.  ms = summed kernel time,
+ * 0 panel, 1 laswp, 2 trsm, 3 gemm (the outer trailing updates), 4 trsv,
+ * 5 other (conversions, residuals, packing), 6 the GEMMs inside the recursive
+ * panel and TRSM.  ms = summed kernel time,
  * work = summed algorithmic flops (bytes for laswp), launches = count. */
 int sla_get_stats(sla_handle_t h, int cls, double *ms, double *work, long long *launches);
 int sla_reset_stats(sla_handle_t h);
@@ -113,10 +115,11 @@ int sla_dgetrs(sla_handle_t h, int n, int nrhs, const double *dA, int lda, const
 int sla_dgesv(sla_handle_t h, int n, int nrhs, double *dA, int lda, int *dipiv, double *dB,
               int ldb, int *info);
 
-/* The trailing-update kernel alone: C[m x n] -= A[m x k] * B[k x n] (FP64,
+/* The update GEMM alone: C[m x n] -= A[m x k] * B[k x n] (FP64,
  * column-major, device pointers) -- PDGEMM's local dgemm_('N','N', ..., -1,
  * L21, U12, +1, A22) of SRC/pdgetrf.f:288 (PBLAS/SRC/PTOOLS/PB_CpgemmAB.c:
- * 333-352).  For kernel-level checks and measurement. */
+ * 333-352), through the same shape dispatch as the factorization's GEMMs.
+ * For kernel-level checks and measurement. */
 int sla_dgemm_sub(sla_handle_t h, int m, int n, int k, const double *dA, int lda,
                   const double *dB, int ldb, double *dC, int ldc);
 

@@ -173,6 +173,9 @@ __global__ __launch_bounds__(256, 2) void dgemm_nn_sub_kernel(
 typedef double dv4_t __attribute__((ext_vector_type(4)));
 constexpr int DEPTH = 3;
 
+// TRAILING distinguishes the launches of the outer trailing update (1) from
+// every other use (0) in kernel traces; the code is the same.
+template <int TRAILING>
 __global__ __launch_bounds__(256, 1) void dgemm_nn_sub_direct_kernel(
     int m, int n, int k, const double *__restrict__ A, int lda,
     const double *__restrict__ B, int ldb, double *__restrict__ C, int ldc)
@@ -271,9 +274,10 @@ __global__ __launch_bounds__(256) void dgemm_nn_sub_skinny_kernel(
 }  // namespace
 
 void dgemm_nn_sub(int m, int n, int k, const double *A, int lda, const double *B, int ldb,
-                  double *C, int ldc, hipStream_t s)
+                  double *C, int ldc, hipStream_t s, bool trailing)
 {
     if (m <= 0 || n <= 0 || k <= 0) return;
+    if (!trailing && gemm_small<double>(m, n, k, A, lda, B, ldb, C, ldc, s)) return;
     if (n <= 16 && (size_t)k * n * sizeof(double) <= 48 * 1024) {
         const int grid = ceil_div(m, 256);
         const size_t sh = (size_t)k * n * sizeof(double);
@@ -291,15 +295,19 @@ void dgemm_nn_sub(int m, int n, int k, const double *A, int lda, const double *B
     if (mf > 0 && nf > 0 && k % 16 == 0 && lda % 4 == 0 && ldb % 4 == 0 && ldc % 4 == 0 &&
         aligned(A) && aligned(B) && aligned(C)) {
         const int tiles = (mf / 128) * (nf / 128);
-        hipLaunchKernelGGL(dgemm_nn_sub_direct_kernel, dim3(tiles), dim3(256), 0, s, mf, nf, k,
-                           A, lda, B, ldb, C, ldc);
+        if (trailing)
+            hipLaunchKernelGGL(dgemm_nn_sub_direct_kernel<1>, dim3(tiles), dim3(256), 0, s, mf, nf,
+                               k, A, lda, B, ldb, C, ldc);
+        else
+            hipLaunchKernelGGL(dgemm_nn_sub_direct_kernel<0>, dim3(tiles), dim3(256), 0, s, mf, nf,
+                               k, A, lda, B, ldb, C, ldc);
         SLA_LAUNCH_CHECK();
         // ragged edges: the rows below the interior (all columns) and the
         // columns right of it
-        if (m > mf) dgemm_nn_sub(m - mf, n, k, A + mf, lda, B, ldb, C + mf, ldc, s);
+        if (m > mf) dgemm_nn_sub(m - mf, n, k, A + mf, lda, B, ldb, C + mf, ldc, s, trailing);
         if (n > nf)
             dgemm_nn_sub(mf, n - nf, k, A, lda, B + (size_t)nf * ldb, ldb, C + (size_t)nf * ldc,
-                         ldc, s);
+                         ldc, s, trailing);
         return;
     }
     const int tiles = ceil_div(m, TM) * ceil_div(n, TN);

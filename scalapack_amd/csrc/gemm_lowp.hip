@@ -88,6 +88,9 @@ __device__ __forceinline__ void tile_coords_lp(int tiles_m, int tiles_n, int &tm
 // per column.  D element r (block row 4*lk + r) is C row wm + 16*lk + 4*r + i.
 constexpr int SDEPTH = 3;
 
+// TRAILING: 1 for the outer trailing update's launches (kernel traces tell
+// them apart), 0 otherwise; the code is the same.
+template <int TRAILING>
 __global__ __launch_bounds__(256, 2) void sgemm_nn_sub_direct_kernel(
     int m, int n, int k, const float *__restrict__ A, int lda, const float *__restrict__ B,
     int ldb, float *__restrict__ C, int ldc)
@@ -394,19 +397,24 @@ __global__ void convert_f64_f32_kernel(int m, int n, const double *__restrict__ 
 static bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
 
 void sgemm_nn_sub(int m, int n, int k, const float *A, int lda, const float *B, int ldb, float *C,
-                  int ldc, hipStream_t s)
+                  int ldc, hipStream_t s, bool trailing)
 {
     if (m <= 0 || n <= 0 || k <= 0) return;
+    if (!trailing && gemm_small<float>(m, n, k, A, lda, B, ldb, C, ldc, s)) return;
     const int mf = (m / 128) * 128, nf = (n / 128) * 128;
     if (mf > 0 && nf > 0 && k % 16 == 0 && lda % 4 == 0 && ldb % 4 == 0 && ldc % 4 == 0 &&
         aligned16(A) && aligned16(B) && aligned16(C)) {
-        hipLaunchKernelGGL(sgemm_nn_sub_direct_kernel, dim3((mf / 128) * (nf / 128)), dim3(256), 0,
-                           s, mf, nf, k, A, lda, B, ldb, C, ldc);
+        if (trailing)
+            hipLaunchKernelGGL(sgemm_nn_sub_direct_kernel<1>, dim3((mf / 128) * (nf / 128)),
+                               dim3(256), 0, s, mf, nf, k, A, lda, B, ldb, C, ldc);
+        else
+            hipLaunchKernelGGL(sgemm_nn_sub_direct_kernel<0>, dim3((mf / 128) * (nf / 128)),
+                               dim3(256), 0, s, mf, nf, k, A, lda, B, ldb, C, ldc);
         SLA_LAUNCH_CHECK();
-        if (m > mf) sgemm_nn_sub(m - mf, n, k, A + mf, lda, B, ldb, C + mf, ldc, s);
+        if (m > mf) sgemm_nn_sub(m - mf, n, k, A + mf, lda, B, ldb, C + mf, ldc, s, trailing);
         if (n > nf)
             sgemm_nn_sub(mf, n - nf, k, A, lda, B + (size_t)nf * ldb, ldb, C + (size_t)nf * ldc,
-                         ldc, s);
+                         ldc, s, trailing);
         return;
     }
     hipLaunchKernelGGL((gemm_lp_edge_kernel<float, false>), dim3(ceil_div(m, 16), ceil_div(n, 16)),

@@ -261,12 +261,14 @@ struct Lu {
     void update(int m, int n, int k, const T *l21, const T *u12, T *c, hipStream_t s);
 
     // B[m x n] <- L^{-1} B for the unit lower m x m L at l: recursive halving
-    // down to <= 32-row diagonal solves, the off-diagonal products on the
-    // MFMA GEMM (the 32-row blocking mirrors PB_CptrsmAB.c:359-412).
+    // down to <= 32-row diagonal solves (or one strip launch for a small block
+    // inside the panel), the off-diagonal products on the GEMMs (the 32-row
+    // blocking mirrors PB_CptrsmAB.c:359-412).
     void trsm(int m, int n, const T *l, T *b, hipStream_t s)
     {
         if (m <= 0 || n <= 0) return;
-        if (m <= 32) {
+        // <= 32 rows, or a small block inside the panel: one strip launch
+        if (m <= 32 || (m <= 256 && n <= 256)) {
             h.timing_begin(CAT_TRSM, s);
             trsm_llnu<T>(m, n, l, lda, b, lda, s);
             h.timing_end((double)m * m * n, s);
@@ -356,7 +358,7 @@ void Lu<double>::gemm(int m, int n, int k, const double *a, const double *b, dou
                       hipStream_t s)
 {
     if (m <= 0 || n <= 0 || k <= 0) return;
-    h.timing_begin(CAT_GEMM, s);
+    h.timing_begin(CAT_GEMM_PANEL, s);
     dgemm_nn_sub(m, n, k, a, lda, b, lda, c, lda, s);
     h.timing_end(2.0 * m * n * k, s);
 }
@@ -365,7 +367,10 @@ template <>
 void Lu<double>::update(int m, int n, int k, const double *l21, const double *u12, double *c,
                         hipStream_t s)
 {
-    gemm(m, n, k, l21, u12, c, s);
+    if (m <= 0 || n <= 0 || k <= 0) return;
+    h.timing_begin(CAT_GEMM, s);
+    dgemm_nn_sub(m, n, k, l21, lda, u12, lda, c, lda, s, true);
+    h.timing_end(2.0 * m * n * k, s);
 }
 
 template <>
@@ -395,7 +400,7 @@ template <>
 void Lu<float>::gemm(int m, int n, int k, const float *a, const float *b, float *c, hipStream_t s)
 {
     if (m <= 0 || n <= 0 || k <= 0) return;
-    h.timing_begin(CAT_GEMM, s);
+    h.timing_begin(CAT_GEMM_PANEL, s);
     sgemm_nn_sub(m, n, k, a, lda, b, lda, c, lda, s);
     h.timing_end(2.0 * m * n * k, s);
 }
@@ -406,7 +411,9 @@ void Lu<float>::update(int m, int n, int k, const float *l21, const float *u12, 
 {
     if (m <= 0 || n <= 0 || k <= 0) return;
     if (schur == SCHUR_NATIVE) {
-        gemm(m, n, k, l21, u12, c, s);
+        h.timing_begin(CAT_GEMM, s);
+        sgemm_nn_sub(m, n, k, l21, lda, u12, lda, c, lda, s, true);
+        h.timing_end(2.0 * m * n * k, s);
         return;
     }
     // operands k-contiguous, k padded with zeros to the MFMA's K step

@@ -12,7 +12,18 @@
 
 namespace sla {
 
-enum Category { CAT_PANEL = 0, CAT_LASWP, CAT_TRSM, CAT_GEMM, CAT_TRSV, CAT_OTHER, CAT_COUNT };
+// CAT_GEMM: the outer trailing updates; CAT_GEMM_PANEL: the GEMMs inside the
+// recursive panel and TRSM.
+enum Category {
+    CAT_PANEL = 0,
+    CAT_LASWP,
+    CAT_TRSM,
+    CAT_GEMM,
+    CAT_TRSV,
+    CAT_OTHER,
+    CAT_GEMM_PANEL,
+    CAT_COUNT
+};
 
 // Precision of the outer trailing update of an FP32-storage factorization.
 enum Schur { SCHUR_NATIVE = 0, SCHUR_BF16 = 1, SCHUR_FP8 = 2 };

@@ -6,13 +6,23 @@
 namespace sla {
 
 // ---- Schur-complement update: C[m x n] -= A[m x k] * B[k x n] (FP64) -----
+// trailing: the launch is PDGETRF's outer trailing update (a kernel symbol of
+// its own in traces).
 void dgemm_nn_sub(int m, int n, int k, const double *A, int lda, const double *B, int ldb,
-                  double *C, int ldc, hipStream_t s);
+                  double *C, int ldc, hipStream_t s, bool trailing = false);
+
+// ---- thin GEMMs of the recursive panel and TRSM (gemm_small.hip) -----------
+// C -= A * B when n, k <= 64 with m >= 4n (tall-narrow) or m, k <= 64
+// (short-wide); returns false (nothing launched) for other shapes.
+constexpr int SMALL_NMAX = 64, SMALL_KMAX = 64;
+template <typename T>
+bool gemm_small(int m, int n, int k, const T *A, int lda, const T *B, int ldb, T *C, int ldc,
+                hipStream_t s);
 
 // ---- low-precision Schur-complement updates (FP32 storage) -----------------
 // C[m x n] -= A[m x k] * B[k x n], FP32 operands on FP32 MFMA.
 void sgemm_nn_sub(int m, int n, int k, const float *A, int lda, const float *B, int ldb, float *C,
-                  int ldc, hipStream_t s);
+                  int ldc, hipStream_t s, bool trailing = false);
 // Same with BF16 operands packed k-contiguous: Ap row-major m x k (ld ldak),
 // Bp column-major k x n (ld ldbk); FP32 accumulation into C.
 void gemm_bf16_nn_sub(int m, int n, int k, const __bf16 *Ap, int ldak, const __bf16 *Bp, int ldbk,

@@ -11,7 +11,7 @@ import ctypes
 
 from ._lib import SolveInfo, last_error, load
 
-STAT_CLASSES = ("panel", "laswp", "trsm", "gemm", "trsv", "other")
+STAT_CLASSES = ("panel", "laswp", "trsm", "gemm", "trsv", "other", "gemm_panel")
 
 # factorization precisions and refinements (SLA_PREC_*, SLA_REFINE_*)
 PRECISIONS = {"fp64": 0, "fp32": 1, "bf16": 2, "fp8": 3}
