@@ -65,6 +65,8 @@ def parse():
     ap.add_argument("--cpu-n", type=int, default=16384, help="CPU baseline sample size")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="mixed mode: skip the FP64 comparison")
+    ap.add_argument("--param", action="append", default=[],
+                    help="handle knob name=value (sla_set_param), may repeat")
     return ap.parse_args()
 
 
@@ -240,6 +242,9 @@ def main() -> int:
     torch.cuda.set_device(dev)
     h = Handle(nb=nb)
     h.set_stream(torch.cuda.current_stream())
+    for kv in args.param:
+        key, val = kv.split("=", 1)
+        h.set(key, int(val))
     line = {"metric": METRIC, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None}
     flops = (2.0 / 3.0) * n ** 3
@@ -257,7 +262,7 @@ def main() -> int:
             "data": "synthetic: A i.i.d. uniform[-1,1) (torch seed 100), b = A*ones, in HBM",
             "config": {"workload": f"plain FP64 LU solve PDGESV (1x1 grid), N={n}, NRHS={nrhs}, "
                                    f"NB={nb}", "n": n, "nrhs": nrhs, "nb": nb,
-                       "precision": "fp64"},
+                       "precision": "fp64", "params": args.param},
             "scaled_residual": max(sres),
             "kernel_ms_per_step": {k: round(v["ms"] / args.steps, 2) for k, v in stats.items()},
             "roofline": roofline_from(stats, args.steps, "fp64"),

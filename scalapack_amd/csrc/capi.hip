@@ -503,7 +503,7 @@ int sla_set_param(sla_handle_t h, const char *name, int value)
     const std::string k(name ? name : "");
     static const char *known[] = {"nb",      "leaf_w",  "panel_wgs",        "profile",
                                   "ir_max_iter", "gmres_m", "ir_gmres_on_stall", "gmres_tol_exp",
-                                  "pivoting"};
+                                  "pivoting", "lookahead", "update_cus"};
     if (std::find_if(std::begin(known), std::end(known), [&](const char *x) { return k == x; }) ==
         std::end(known)) {
         g_last_error = "sla_set_param: unknown parameter " + k;
@@ -518,6 +518,8 @@ int sla_set_param(sla_handle_t h, const char *name, int value)
     else if (k == "gmres_m" && value >= 1 && value <= 1000) H->gmres_m = value;
     else if (k == "ir_gmres_on_stall") H->ir_gmres_on_stall = value != 0;
     else if (k == "pivoting" && (value == 0 || value == 1)) H->pivoting = value;
+    else if (k == "lookahead" && (value == 0 || value == 1)) H->lookahead = value;
+    else if (k == "update_cus" && value >= 0 && value <= H->num_cus) H->update_cus = value;
     else if (k == "gmres_tol_exp" && value >= 1 && value <= 15) H->gmres_tol = std::pow(10.0, -value);
     else {
         g_last_error = "sla_set_param: value out of range for " + k;
@@ -539,6 +541,8 @@ int sla_get_param(sla_handle_t h, const char *name, int *value)
     else if (k == "gmres_m") *value = H->gmres_m;
     else if (k == "ir_gmres_on_stall") *value = H->ir_gmres_on_stall;
     else if (k == "pivoting") *value = H->pivoting;
+    else if (k == "lookahead") *value = H->lookahead;
+    else if (k == "update_cus") *value = H->update_cus;
     else if (k == "gmres_tol_exp") *value = (int)std::lround(-std::log10(H->gmres_tol));
     else {
         g_last_error = "sla_get_param: unknown parameter " + k;

@@ -32,9 +32,8 @@ typedef double d4_t __attribute__((ext_vector_type(4)));
 // Maps the linear workgroup id to a (tile_m, tile_n) pair so that the
 // workgroups one XCD receives (ids congruent mod 8 are dispatched to the same
 // XCD) sweep a compact band of tiles that share L21 rows and U12 columns.
-__device__ __forceinline__ void tile_coords(int tiles_m, int tiles_n, int &tm, int &tn)
+__device__ __forceinline__ void tile_coords(int id, int tiles_m, int tiles_n, int &tm, int &tn)
 {
-    const int id = blockIdx.x;
     const int total = tiles_m * tiles_n;
     int lid = id;
     if (total >= 64) {
@@ -72,7 +71,7 @@ __global__ __launch_bounds__(256, 2) void dgemm_nn_sub_kernel(
 
     const int tiles_m = ceil_div(m, TM), tiles_n = ceil_div(n, TN);
     int tmi, tni;
-    tile_coords(tiles_m, tiles_n, tmi, tni);
+    tile_coords(blockIdx.x, tiles_m, tiles_n, tmi, tni);
     const int bm = tmi * TM, bn = tni * TN;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -175,14 +174,19 @@ constexpr int DEPTH = 3;
 
 // TRAILING distinguishes the launches of the outer trailing update (1) from
 // every other use (0) in kernel traces; the code is the same.
+// The grid may be smaller than the tile count (a persistent launch that
+// leaves CUs free for the concurrently running panel): workgroup b takes
+// tiles b, b + grid, ...; grid is a multiple of 8, so a tile index keeps the
+// XCD of the workgroup that processes it.
 template <int TRAILING>
 __global__ __launch_bounds__(256, 1) void dgemm_nn_sub_direct_kernel(
     int m, int n, int k, const double *__restrict__ A, int lda,
     const double *__restrict__ B, int ldb, double *__restrict__ C, int ldc)
 {
     const int tiles_m = m / 128, tiles_n = n / 128;
+    for (int tile = blockIdx.x; tile < tiles_m * tiles_n; tile += gridDim.x) {
     int tmi, tni;
-    tile_coords(tiles_m, tiles_n, tmi, tni);
+    tile_coords(tile, tiles_m, tiles_n, tmi, tni);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = tmi * 128 + (wave & 1) * 64, wn = tni * 128 + (wave >> 1) * 64;
     const int lr = lane & 15, lk = lane >> 4;
@@ -239,6 +243,7 @@ __global__ __launch_bounds__(256, 1) void dgemm_nn_sub_direct_kernel(
             *cp = c;
         }
     }
+    }  // tile loop
 }
 
 // C[m x n] -= A[m x k] * B[k x n] for n <= 16 (the right-hand sides of the
@@ -274,7 +279,7 @@ __global__ __launch_bounds__(256) void dgemm_nn_sub_skinny_kernel(
 }  // namespace
 
 void dgemm_nn_sub(int m, int n, int k, const double *A, int lda, const double *B, int ldb,
-                  double *C, int ldc, hipStream_t s, bool trailing)
+                  double *C, int ldc, hipStream_t s, bool trailing, int max_wgs)
 {
     if (m <= 0 || n <= 0 || k <= 0) return;
     if (!trailing && gemm_small<double>(m, n, k, A, lda, B, ldb, C, ldc, s)) return;
@@ -295,8 +300,10 @@ void dgemm_nn_sub(int m, int n, int k, const double *A, int lda, const double *B
     if (mf > 0 && nf > 0 && k % 16 == 0 && lda % 4 == 0 && ldb % 4 == 0 && ldc % 4 == 0 &&
         aligned(A) && aligned(B) && aligned(C)) {
         const int tiles = (mf / 128) * (nf / 128);
+        // a persistent grid (multiple of 8) when the caller caps the workgroups
+        const int grid = (max_wgs >= 8 && tiles > max_wgs) ? (max_wgs / 8) * 8 : tiles;
         if (trailing)
-            hipLaunchKernelGGL(dgemm_nn_sub_direct_kernel<1>, dim3(tiles), dim3(256), 0, s, mf, nf,
+            hipLaunchKernelGGL(dgemm_nn_sub_direct_kernel<1>, dim3(grid), dim3(256), 0, s, mf, nf,
                                k, A, lda, B, ldb, C, ldc);
         else
             hipLaunchKernelGGL(dgemm_nn_sub_direct_kernel<0>, dim3(tiles), dim3(256), 0, s, mf, nf,
@@ -304,10 +311,11 @@ void dgemm_nn_sub(int m, int n, int k, const double *A, int lda, const double *B
         SLA_LAUNCH_CHECK();
         // ragged edges: the rows below the interior (all columns) and the
         // columns right of it
-        if (m > mf) dgemm_nn_sub(m - mf, n, k, A + mf, lda, B, ldb, C + mf, ldc, s, trailing);
+        if (m > mf) dgemm_nn_sub(m - mf, n, k, A + mf, lda, B, ldb, C + mf, ldc, s, trailing,
+                                 max_wgs);
         if (n > nf)
             dgemm_nn_sub(mf, n - nf, k, A, lda, B + (size_t)nf * ldb, ldb, C + (size_t)nf * ldc,
-                         ldc, s, trailing);
+                         ldc, s, trailing, max_wgs);
         return;
     }
     const int tiles = ceil_div(m, TM) * ceil_div(n, TN);

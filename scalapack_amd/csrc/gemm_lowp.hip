@@ -57,9 +57,9 @@ __device__ __forceinline__ uint8_t to_lowp<uint8_t>(float v)
     return (uint8_t)(__builtin_amdgcn_cvt_pk_fp8_f32(v, 0.f, 0, false) & 0xff);
 }
 
-__device__ __forceinline__ void tile_coords_lp(int tiles_m, int tiles_n, int &tm, int &tn)
+__device__ __forceinline__ void tile_coords_lp(int id, int tiles_m, int tiles_n, int &tm, int &tn)
 {
-    const int id = blockIdx.x, total = tiles_m * tiles_n;
+    const int total = tiles_m * tiles_n;
     int lid = id;
     if (total >= 64) {
         const int per = total / 8, rem = total % 8;
@@ -95,8 +95,11 @@ __global__ __launch_bounds__(256, 2) void sgemm_nn_sub_direct_kernel(
     int m, int n, int k, const float *__restrict__ A, int lda, const float *__restrict__ B,
     int ldb, float *__restrict__ C, int ldc)
 {
+    // persistent when the grid is smaller than the tile count (see
+    // dgemm_nn_sub_direct_kernel)
+    for (int tile = blockIdx.x; tile < (m / 128) * (n / 128); tile += gridDim.x) {
     int tmi, tni;
-    tile_coords_lp(m / 128, n / 128, tmi, tni);
+    tile_coords_lp(tile, m / 128, n / 128, tmi, tni);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = tmi * 128 + (wave & 1) * 64, wn = tni * 128 + (wave >> 1) * 64;
     const int lr = lane & 15, lk = lane >> 4;
@@ -151,6 +154,7 @@ __global__ __launch_bounds__(256, 2) void sgemm_nn_sub_direct_kernel(
             *cp = c;
         }
     }
+    }  // tile loop
 }
 
 // Ragged edges (and small shapes) of either precision: one thread per element
@@ -187,7 +191,7 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16_nn_sub_direct_kernel(
     int ldbk, float *__restrict__ C, int ldc)
 {
     int tmi, tni;
-    tile_coords_lp(m / 128, n / 128, tmi, tni);
+    tile_coords_lp(blockIdx.x, m / 128, n / 128, tmi, tni);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = tmi * 128 + (wave & 1) * 64, wn = tni * 128 + (wave >> 1) * 64;
     const int lr = lane & 15, lk = lane >> 4;
@@ -255,7 +259,7 @@ __global__ __launch_bounds__(256, 2) void gemm_fp8_nn_sub_direct_kernel(
     int ldbk, const float *__restrict__ scales, float *__restrict__ C, int ldc)
 {
     int tmi, tni;
-    tile_coords_lp(m / 128, n / 128, tmi, tni);
+    tile_coords_lp(blockIdx.x, m / 128, n / 128, tmi, tni);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = tmi * 128 + (wave & 1) * 64, wn = tni * 128 + (wave >> 1) * 64;
     const int lr = lane & 15, lk = lane >> 4;
@@ -397,24 +401,27 @@ __global__ void convert_f64_f32_kernel(int m, int n, const double *__restrict__ 
 static bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
 
 void sgemm_nn_sub(int m, int n, int k, const float *A, int lda, const float *B, int ldb, float *C,
-                  int ldc, hipStream_t s, bool trailing)
+                  int ldc, hipStream_t s, bool trailing, int max_wgs)
 {
     if (m <= 0 || n <= 0 || k <= 0) return;
     if (!trailing && gemm_small<float>(m, n, k, A, lda, B, ldb, C, ldc, s)) return;
     const int mf = (m / 128) * 128, nf = (n / 128) * 128;
     if (mf > 0 && nf > 0 && k % 16 == 0 && lda % 4 == 0 && ldb % 4 == 0 && ldc % 4 == 0 &&
         aligned16(A) && aligned16(B) && aligned16(C)) {
+        const int tiles = (mf / 128) * (nf / 128);
+        const int grid = (max_wgs >= 8 && tiles > max_wgs) ? (max_wgs / 8) * 8 : tiles;
         if (trailing)
-            hipLaunchKernelGGL(sgemm_nn_sub_direct_kernel<1>, dim3((mf / 128) * (nf / 128)),
-                               dim3(256), 0, s, mf, nf, k, A, lda, B, ldb, C, ldc);
+            hipLaunchKernelGGL(sgemm_nn_sub_direct_kernel<1>, dim3(grid), dim3(256), 0, s, mf, nf,
+                               k, A, lda, B, ldb, C, ldc);
         else
             hipLaunchKernelGGL(sgemm_nn_sub_direct_kernel<0>, dim3((mf / 128) * (nf / 128)),
                                dim3(256), 0, s, mf, nf, k, A, lda, B, ldb, C, ldc);
         SLA_LAUNCH_CHECK();
-        if (m > mf) sgemm_nn_sub(m - mf, n, k, A + mf, lda, B, ldb, C + mf, ldc, s, trailing);
+        if (m > mf)
+            sgemm_nn_sub(m - mf, n, k, A + mf, lda, B, ldb, C + mf, ldc, s, trailing, max_wgs);
         if (n > nf)
             sgemm_nn_sub(mf, n - nf, k, A, lda, B + (size_t)nf * ldb, ldb, C + (size_t)nf * ldc,
-                         ldc, s, trailing);
+                         ldc, s, trailing, max_wgs);
         return;
     }
     hipLaunchKernelGGL((gemm_lp_edge_kernel<float, false>), dim3(ceil_div(m, 16), ceil_div(n, 16)),

@@ -42,6 +42,9 @@ T *alloc_dev(size_t n)
 Handle::Handle()
 {
     SLA_HIP(hipGetDevice(&device));
+    if (hipDeviceGetAttribute(&num_cus, hipDeviceAttributeMultiprocessorCount, device) !=
+            hipSuccess || num_cus <= 0)
+        num_cus = 256;
     SLA_HIP(hipStreamCreateWithFlags(&own_stream, hipStreamNonBlocking));
     SLA_HIP(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
     stream = own_stream;
@@ -228,6 +231,7 @@ struct Lu {
     PanelWork<T> pw;
     int schur = SCHUR_NATIVE;  // precision of the outer trailing update
     bool l_packed = false;     // L21 of the current block column is packed
+    hipStream_t s_main = nullptr, s_panel = nullptr;  // update stream, panel stream
 
     T *at(int i, int j) const { return A + (size_t)i + (size_t)j * lda; }
 
@@ -315,7 +319,7 @@ struct Lu {
     {
         const int mn = std::min(M, N), nb = h.nb;
         if (nb > 512) throw std::logic_error("getrf: NB above 512");
-        hipStream_t s0 = h.stream, s1 = h.stream2;
+        hipStream_t s0 = s_main, s1 = s_panel;
         int jprev = 0;
         {
             const int jb = std::min(nb, mn);
@@ -369,7 +373,7 @@ void Lu<double>::update(int m, int n, int k, const double *l21, const double *u1
 {
     if (m <= 0 || n <= 0 || k <= 0) return;
     h.timing_begin(CAT_GEMM, s);
-    dgemm_nn_sub(m, n, k, l21, lda, u12, lda, c, lda, s, true);
+    dgemm_nn_sub(m, n, k, l21, lda, u12, lda, c, lda, s, true, h.update_cus);
     h.timing_end(2.0 * m * n * k, s);
 }
 
@@ -412,7 +416,7 @@ void Lu<float>::update(int m, int n, int k, const float *l21, const float *u12, 
     if (m <= 0 || n <= 0 || k <= 0) return;
     if (schur == SCHUR_NATIVE) {
         h.timing_begin(CAT_GEMM, s);
-        sgemm_nn_sub(m, n, k, l21, lda, u12, lda, c, lda, s, true);
+        sgemm_nn_sub(m, n, k, l21, lda, u12, lda, c, lda, s, true, 2 * h.update_cus);
         h.timing_end(2.0 * m * n * k, s);
         return;
     }
@@ -461,14 +465,24 @@ static int getrf_device(Handle &h, int m, int n, T *dA, int lda, int *dipiv, int
     fill_int(h.ctl, 6, 0, h.stream);         // counter, flags
     fill_int(h.zero_col, 1, INT_MAX, h.stream);
     iota_int(h.perm, m, 0, h.stream);
-    // the panel stream starts after everything queued so far on the main one
+    // streams of the factorization: the caller's (update) + the handle's second
+    // one (panel); the panel stream starts after everything queued so far on
+    // the caller's stream, which waits for it at the end
+    hipStream_t s0 = h.stream, s1 = h.stream2;
+    if (!h.lookahead) s1 = s0;
     SLA_HIP(hipEventRecord(h.ev_join, h.stream));
-    SLA_HIP(hipStreamWaitEvent(h.stream2, h.ev_join, 0));
+    if (s0 != h.stream) SLA_HIP(hipStreamWaitEvent(s0, h.ev_join, 0));
+    if (s1 != h.stream) SLA_HIP(hipStreamWaitEvent(s1, h.ev_join, 0));
     Lu<T> lu{h, dA, lda, m, n, dipiv, h.panel_work<T>()};
     lu.schur = schur;
+    lu.s_main = s0;
+    lu.s_panel = s1;
     lu.run();
-    SLA_HIP(hipEventRecord(h.ev_join, h.stream2));
-    SLA_HIP(hipStreamWaitEvent(h.stream, h.ev_join, 0));
+    for (hipStream_t sx : {s0, s1}) {
+        if (sx == h.stream) continue;
+        SLA_HIP(hipEventRecord(h.ev_join, sx));
+        SLA_HIP(hipStreamWaitEvent(h.stream, h.ev_join, 0));
+    }
     int flags[3];
     SLA_HIP(hipMemcpyAsync(flags, h.ctl, sizeof flags, hipMemcpyDeviceToHost, h.stream));
     SLA_HIP(hipStreamSynchronize(h.stream));

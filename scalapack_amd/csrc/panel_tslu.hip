@@ -10,20 +10,17 @@
 // factors it produces are an exact LU (P A = L U up to rounding, |L| <= 1 not
 // guaranteed but in practice bounded) of the same kind partial pivoting gives.
 //
-// One launch of G workgroups of 512 threads (8 waves); the leaf's m rows are
+// One launch of G <= 32 workgroups of 512 threads; the leaf's m rows are
 // spread as in panel.hip (row g*R + q*512 + tid, R = RPT*512).
-//   1. every wave runs partial-pivoting elimination on its own 64*RPT rows,
-//      entirely in registers (wave shuffles for the argmax, lane reads for the
-//      pivot row), and keeps the W rows it chose;
-//   2. wave 0 repeats that on the 8*W rows its workgroup chose (original
-//      values), giving W candidates per workgroup, published with their
+//   1. every workgroup runs partial-pivoting elimination on its own rows in
+//      registers (per column: wave argmax, two workgroup barriers, the pivot
+//      row through LDS) and so chooses W candidate rows, published with their
 //      original values; workgroup 0 also publishes the leaf's top W rows;
-//   3. one device-wide barrier;
-//   4. every workgroup runs the same two-level tournament on the G*W
-//      candidates (8 waves on slices, then wave 0 on the 8*W survivors) and
-//      so agrees on the W pivot rows and on their factors L11\U11 (the final
+//   2. one device-wide barrier;
+//   3. every workgroup runs the same elimination over all G*W candidates and
+//      so agrees on the W pivot rows and on their factors L11\U11 (the
 //      elimination of the pivot rows, in pivot order) without another barrier;
-//   5. the pivots become W sequential interchanges (IPIV as PDGETRF returns
+//   4. the pivots become W sequential interchanges (IPIV as PDGETRF returns
 //      it); every workgroup writes its rows: the top W rows are L11\U11, every
 //      other row r is L(r) = A(src(r)) U11^{-1}, where src(r) is the row the
 //      interchanges bring to position r (a displaced top row comes from the
@@ -41,36 +38,35 @@ namespace {
 constexpr int TPT = 512;         // threads per workgroup
 constexpr int TNW = TPT / 64;    // waves per workgroup
 constexpr int TW = 32;           // leaf width
-constexpr int TMR = 2;           // candidate rows per lane in the first tournament level
-constexpr int TGMAX = TNW * 64 * TMR / TW;  // workgroups per launch at most (32)
+constexpr int TMR = 2;           // candidate rows per thread in the tournament
+constexpr int TGMAX = TPT * TMR / TW;  // workgroups per launch at most (32)
 
-// Partial-pivoting elimination of the NR x W rows a wave holds (NR per lane,
-// key = row identity and tie-break, INT_MAX = no row; pid = payload id).
-// Selects up to wa pivots in order: out_key[j], out_pid[j] (uniform, written
-// by lane 0), and with OUT_LU the pivot row's values when chosen
-// (multipliers left of j, U from j on).
+// Partial-pivoting elimination over the rows a whole workgroup holds (NR
+// per thread; key = row identity and tie-break, INT_MAX = no row; pid =
+// payload id).  Selects up to wa pivots in order: out_key[j], out_pid[j]
+// (written by thread 0).  Per column: a wave-shuffle argmax, one workgroup
+// barrier to combine the 8 wave winners (every thread folds them in the same
+// order), the winner's row through LDS, a second barrier, and the rank-1
+// update.
 //
 // The columns rotate through the registers: at step j, register column 0
-// holds matrix column j, and the rank-1 update writes column c+1's new value
-// into register column c.  Every register index is a compile-time constant
-// while the step loop itself stays rolled (one copy of the body).  With
-// OUT_LU the multipliers rotate in from the right (mul[.][W-1] is the newest).
-template <typename T, int NR, bool OUT_LU, bool FULL>
-__device__ __forceinline__ void wave_gepp(T (&v)[NR][TW], const int (&key)[NR],
-                                          const int (&pid)[NR], int wa, int *out_key,
-                                          int *out_pid, T (*out_lu)[TW + 1])
+// holds matrix column j, and the update writes column c+1's new value into
+// register column c, so every register index is a compile-time constant while
+// the step loop stays rolled.
+struct GeppLds {
+    double wv[TNW];
+    int wk[TNW];
+    int pid;
+};
+
+template <typename T, int NR, bool FULL>
+__device__ __forceinline__ void wg_gepp(T (&v)[NR][TW], const int (&key)[NR], const int (&pid)[NR],
+                                        int wa, int *out_key, int *out_pid, GeppLds &sl, T *s_prow)
 {
-    const int lane = threadIdx.x & 63;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     bool act[NR];
 #pragma unroll
     for (int q = 0; q < NR; q++) act[q] = key[q] != INT_MAX;
-    T mul[OUT_LU ? NR : 1][TW];
-    if constexpr (OUT_LU) {
-#pragma unroll
-        for (int q = 0; q < NR; q++)
-#pragma unroll
-            for (int c = 0; c < TW; c++) mul[q][c] = T(0);
-    }
     const int steps = FULL ? TW : wa;
 #pragma unroll 1
     for (int j = 0; j < steps; j++) {
@@ -98,54 +94,50 @@ __device__ __forceinline__ void wave_gepp(T (&v)[NR][TW], const int (&key)[NR],
                 wk = ok;
             }
         }
-        if (wk == INT_MAX) {  // uniform: no rows left
-            if (lane == 0) {
+        if (lane == 0) {
+            sl.wv[wave] = wv;
+            sl.wk[wave] = wk;
+        }
+        __syncthreads();
+        double fv = sl.wv[0];
+        int fk = sl.wk[0];
+#pragma unroll
+        for (int w = 1; w < TNW; w++)
+            if (pivot_better(sl.wv[w], sl.wk[w], fv, fk)) {
+                fv = sl.wv[w];
+                fk = sl.wk[w];
+            }
+        if (fk == INT_MAX) {  // uniform: no rows left
+            if (tid == 0) {
                 out_key[j] = INT_MAX;
                 out_pid[j] = -1;
             }
+            __syncthreads();
             continue;
         }
-        const bool mine = (bk == wk);
-        const int wl = __builtin_ctzll(__ballot(mine));
-        T prow[TW];
-#pragma unroll
-        for (int c = 0; c < TW; c++) {
-            T sel = v[0][c];
-#pragma unroll
-            for (int q = 1; q < NR; q++) sel = (bq == q) ? v[q][c] : sel;
-            prow[c] = lane_read(sel, wl);
-        }
-        int sp = pid[0];
-#pragma unroll
-        for (int q = 1; q < NR; q++) sp = (bq == q) ? pid[q] : sp;
-        const int wpid = __builtin_amdgcn_readlane(sp, wl);
-        if constexpr (OUT_LU) {
-            // row j of L11\U11: multipliers l_0..l_{j-1} are mul[TW-j..TW-1]
-            // of the pivot row, U(j, j..) is prow[0..TW-1-j]
-            T pm[TW];
+        const bool mine = (bk == fk);  // one thread holds the winning row
+        if (mine) {
 #pragma unroll
             for (int c = 0; c < TW; c++) {
-                T sel = mul[0][c];
+                T sel = v[0][c];
 #pragma unroll
-                for (int q = 1; q < NR; q++) sel = (bq == q) ? mul[q][c] : sel;
-                pm[c] = lane_read(sel, wl);
+                for (int q = 1; q < NR; q++) sel = (bq == q) ? v[q][c] : sel;
+                s_prow[c] = sel;
             }
-            if (lane == 0) {
+            int sp = pid[0];
 #pragma unroll
-                for (int c = 0; c < TW; c++) {
-                    if (c < TW - j) out_lu[j][j + c] = prow[c];
-                    if (c >= TW - j) out_lu[j][c - (TW - j)] = pm[c];
-                }
-            }
-        }
-        if (lane == 0) {
-            out_key[j] = wk;
-            out_pid[j] = wpid;
-        }
+            for (int q = 1; q < NR; q++) sp = (bq == q) ? pid[q] : sp;
+            sl.pid = sp;
 #pragma unroll
-        for (int q = 0; q < NR; q++)
-            if (mine && bq == q) act[q] = false;
-        const T pj = prow[0];
+            for (int q = 0; q < NR; q++)
+                if (bq == q) act[q] = false;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            out_key[j] = fk;
+            out_pid[j] = sl.pid;
+        }
+        const T pj = s_prow[0];
         const bool nz = pj != T(0);
         const T rcp = nz ? T(1) / pj : T(0);
 #pragma unroll
@@ -154,49 +146,18 @@ __device__ __forceinline__ void wave_gepp(T (&v)[NR][TW], const int (&key)[NR],
                 T l = v[q][0];
                 if (nz) l = l * rcp;
 #pragma unroll
-                for (int c = 1; c < TW; c++) v[q][c - 1] = fma(-l, prow[c], v[q][c]);
+                for (int c = 1; c < TW; c++) v[q][c - 1] = fma(-l, s_prow[c], v[q][c]);
                 v[q][TW - 1] = T(0);
-                if constexpr (OUT_LU) {
-#pragma unroll
-                    for (int c = 1; c < TW; c++) mul[q][c - 1] = mul[q][c];
-                    mul[q][TW - 1] = l;
-                }
             }
         }
     }
+    __syncthreads();  // out_* complete before anyone reads them
 }
 
-// Loads candidate rows for an elimination level: NR per lane, the id of
-// lane l's q-th row is ids[(q*64 + l)] (INT_MAX: none), its values come from
-// A (own rows) or from the published candidates.
 template <bool FULL>
 __device__ __forceinline__ int colx(int c, int wa)
 {
     return FULL ? c : min(c, wa - 1);
-}
-
-template <typename T, int NR, bool FROM_SLOTS, bool FULL>
-__device__ __forceinline__ void load_rows(T (&v)[NR][TW], int (&key)[NR], int (&pid)[NR],
-                                          const int *keys, const int *pids, int n_ids,
-                                          const T *A, int lda, const T *cand_v, int wa)
-{
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int q = 0; q < NR; q++) {
-        const int idx = q * 64 + lane;
-        const int r = idx < n_ids ? keys[idx] : INT_MAX;
-        key[q] = r;
-        pid[q] = (idx < n_ids && pids) ? pids[idx] : r;
-        if (FROM_SLOTS) {
-            const int sv = r != INT_MAX ? pid[q] : 0;
-#pragma unroll
-            for (int c = 0; c < TW; c++) v[q][c] = coherent_load(&cand_v[(size_t)sv * TW + c]);
-        } else {
-            const int rr = r != INT_MAX ? r : 0;
-#pragma unroll
-            for (int c = 0; c < TW; c++) v[q][c] = A[rr + (size_t)colx<FULL>(c, wa) * lda];
-        }
-    }
 }
 
 // FULL: the leaf is TW columns wide (no column predicates).
@@ -213,9 +174,8 @@ __global__ __launch_bounds__(TPT) void panel_tslu_kernel(
     // column c of the leaf as loaded is colx<FULL>(c, wa): columns right of a
     // narrow leaf repeat its last column (never written back, never pivoted on)
 
-    // candidate lists of the levels: [wave][TW] keys (rows) and payloads
-    __shared__ int s_k1[TNW * TW], s_p1[TNW * TW];
-    __shared__ int s_kA[TNW / 2 * TW], s_pA[TNW / 2 * TW];
+    __shared__ GeppLds s_gl;
+    __shared__ T s_prow[TW];
     __shared__ int s_k2[TW], s_p2[TW];
     __shared__ T s_lu[TW][TW + 1];
     __shared__ int s_x[TW];  // position of the k-th interchange's partner
@@ -223,12 +183,10 @@ __global__ __launch_bounds__(TPT) void panel_tslu_kernel(
 
     if (!FULL)
         for (int idx = tid; idx < TW * (TW + 1); idx += TPT) s_lu[idx / (TW + 1)][idx % (TW + 1)] = T(0);
-    // lists hold TW entries per wave; entries past wa stay "no row"
-    for (int idx = tid; idx < TNW * TW; idx += TPT) s_k1[idx] = INT_MAX;
-    if (tid < TNW / 2 * TW) s_kA[tid] = INT_MAX;
+    if (tid < TW) s_k2[tid] = INT_MAX;
     __syncthreads();
 
-    // 1. per-wave elimination of the wave's own rows
+    // 1. elimination of the workgroup's own rows: its TW candidates
     {
         T v[RPT][TW];
         int key[RPT];
@@ -240,30 +198,8 @@ __global__ __launch_bounds__(TPT) void panel_tslu_kernel(
 #pragma unroll
             for (int c = 0; c < TW; c++) v[q][c] = A[rr + (size_t)colx<FULL>(c, wa) * lda];
         }
-        wave_gepp<T, RPT, false, FULL>(v, key, key, wa, s_k1 + wave * TW, s_p1 + wave * TW,
-                                       nullptr);
+        wg_gepp<T, RPT, FULL>(v, key, key, wa, s_k2, s_p2, s_gl, s_prow);
     }
-    __syncthreads();
-
-    // 2. the workgroup's TW candidates: waves 0-3 each eliminate the 2*TW rows
-    //    two waves chose, then wave 0 the 4*TW survivors (original values)
-    if (wave < TNW / 2) {
-        T v[1][TW];
-        int key[1], pid[1];
-        load_rows<T, 1, false, FULL>(v, key, pid, s_k1 + wave * 2 * TW, nullptr, 2 * TW, A, lda,
-                               nullptr, wa);
-        wave_gepp<T, 1, false, FULL>(v, key, pid, wa, s_kA + wave * TW, s_pA + wave * TW,
-                                     nullptr);
-    }
-    __syncthreads();
-    if (wave == 0) {
-        T v[2][TW];
-        int key[2], pid[2];
-        load_rows<T, 2, false, FULL>(v, key, pid, s_kA, nullptr, TNW / 2 * TW, A, lda, nullptr,
-                                     wa);
-        wave_gepp<T, 2, false, FULL>(v, key, pid, wa, s_k2, s_p2, nullptr);
-    }
-    __syncthreads();
     // publish the workgroup's candidates (ids, original values); workgroup 0
     // also the leaf's top rows
     for (int idx = tid; idx < TW * TW; idx += TPT) {
@@ -278,24 +214,23 @@ __global__ __launch_bounds__(TPT) void panel_tslu_kernel(
         }
     }
 
-    // 3. device-wide barrier
+    // 2. device-wide barrier
     if (!grid_sync(ctr, target, abort_flag, spin_limit)) return;
 
-    // 4. the tournament, identical in every workgroup: 8 waves on slices of
-    //    the G*TW candidates, 4 waves on pairs of survivors, wave 0 on the last
-    //    4*TW (its elimination of the pivot rows is L11\U11)
-    for (int idx = tid; idx < TNW * TW; idx += TPT) s_k1[idx] = INT_MAX;
-    if (tid < TNW / 2 * TW) s_kA[tid] = INT_MAX;
+    // 3. the tournament, identical in every workgroup: one elimination over
+    //    all G*TW candidates (original values) fixes the pivot rows and their
+    //    order; wave 0 then factors those rows (original values, in that
+    //    order, no further pivoting) into L11\U11
+    if (tid < TW) s_k2[tid] = INT_MAX;
     __syncthreads();
     {
         const int total = G * TW;
-        const int cpw = ceil_div(total, TNW);  // <= TMR * 64
         T v[TMR][TW];
         int key[TMR], pid[TMR];
 #pragma unroll
         for (int q = 0; q < TMR; q++) {
-            const int il = q * 64 + lane, sid = wave * cpw + il;
-            const bool ok = il < cpw && sid < total;
+            const int sid = q * TPT + tid;
+            const bool ok = sid < total;
             const int r = ok ? coherent_load(&slots.cand_r[sid]) : INT_MAX;
             key[q] = r;
             pid[q] = ok ? sid : -1;
@@ -303,25 +238,38 @@ __global__ __launch_bounds__(TPT) void panel_tslu_kernel(
 #pragma unroll
             for (int c = 0; c < TW; c++) v[q][c] = coherent_load(&slots.cand_v[(size_t)sv * TW + c]);
         }
-        wave_gepp<T, TMR, false, FULL>(v, key, pid, wa, s_k1 + wave * TW, s_p1 + wave * TW,
-                                       nullptr);
+        wg_gepp<T, TMR, FULL>(v, key, pid, wa, s_k2, s_p2, s_gl, s_prow);
     }
-    __syncthreads();
-    if (wave < TNW / 2) {
-        T v[1][TW];
-        int key[1], pid[1];
-        load_rows<T, 1, true, FULL>(v, key, pid, s_k1 + wave * 2 * TW, s_p1 + wave * 2 * TW, 2 * TW,
-                              nullptr, lda, slots.cand_v, wa);
-        wave_gepp<T, 1, false, FULL>(v, key, pid, wa, s_kA + wave * TW, s_pA + wave * TW,
-                                     nullptr);
-    }
-    __syncthreads();
     if (wave == 0) {
-        T v[2][TW];
-        int key[2], pid[2];
-        load_rows<T, 2, true, FULL>(v, key, pid, s_kA, s_pA, TNW / 2 * TW, nullptr, lda, slots.cand_v,
-                              wa);
-        wave_gepp<T, 2, true, FULL>(v, key, pid, wa, s_k2, s_p2, s_lu);
+        // lane i < wa holds pivot row i; the columns rotate as in wg_gepp;
+        // each lane writes its multipliers, and the pivot row of the step its
+        // U part, straight into s_lu
+        T x[TW];
+        const int sid = (lane < wa) ? s_p2[lane] : 0;
+#pragma unroll
+        for (int c = 0; c < TW; c++)
+            x[c] = (lane < wa) ? coherent_load(&slots.cand_v[(size_t)sid * TW + c]) : T(0);
+        const int steps = FULL ? TW : wa;
+#pragma unroll 1
+        for (int k = 0; k < steps; k++) {
+            T prow[TW];
+#pragma unroll
+            for (int c = 0; c < TW; c++) prow[c] = lane_read(x[c], k);
+            if (lane == k) {
+#pragma unroll
+                for (int c = 0; c < TW; c++)
+                    if (c < TW - k) s_lu[k][k + c] = prow[c];
+            }
+            if (lane > k && lane < wa) {
+                const T pj = prow[0];
+                T l = x[0];
+                if (pj != T(0)) l = l * (T(1) / pj);
+                s_lu[lane][k] = l;
+#pragma unroll
+                for (int c = 1; c < TW; c++) x[c - 1] = fma(-l, prow[c], x[c]);
+                x[TW - 1] = T(0);
+            }
+        }
     }
     __syncthreads();
 

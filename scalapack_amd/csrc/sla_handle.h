@@ -120,6 +120,10 @@ struct Handle {
     int leaf_w = 32;       // panel leaf width (columns per device-wide-barrier kernel)
     int panel_wgs = 32;    // preferred workgroups per leaf launch
     int pivoting = 0;      // FP32-storage factorization: 0 partial, 1 tournament
+    int lookahead = 1;     // factor block column p+1 on a second stream during p's update
+    int update_cus = 0;    // > 0: the trailing update runs on at most this many CUs' worth
+                           // of workgroups (the rest stay free for the panel)
+    int num_cus = 256;     // CUs of the device
     // mixed-precision refinement
     int ir_max_iter = 30;        // refinement steps before the FP64 fallback (DSGESV's ITERMAX)
     int ir_gmres_on_stall = 1;   // classical refinement that stalls continues as GMRES-IR

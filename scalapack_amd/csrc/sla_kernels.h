@@ -7,9 +7,10 @@ namespace sla {
 
 // ---- Schur-complement update: C[m x n] -= A[m x k] * B[k x n] (FP64) -----
 // trailing: the launch is PDGETRF's outer trailing update (a kernel symbol of
-// its own in traces).
+// its own in traces); max_wgs > 0 caps its workgroups (a persistent grid that
+// leaves CUs to the panel running beside it).
 void dgemm_nn_sub(int m, int n, int k, const double *A, int lda, const double *B, int ldb,
-                  double *C, int ldc, hipStream_t s, bool trailing = false);
+                  double *C, int ldc, hipStream_t s, bool trailing = false, int max_wgs = 0);
 
 // ---- thin GEMMs of the recursive panel and TRSM (gemm_small.hip) -----------
 // C -= A * B when n, k <= 64 with m >= 4n (tall-narrow) or m, k <= 64
@@ -22,7 +23,7 @@ bool gemm_small(int m, int n, int k, const T *A, int lda, const T *B, int ldb, T
 // ---- low-precision Schur-complement updates (FP32 storage) -----------------
 // C[m x n] -= A[m x k] * B[k x n], FP32 operands on FP32 MFMA.
 void sgemm_nn_sub(int m, int n, int k, const float *A, int lda, const float *B, int ldb, float *C,
-                  int ldc, hipStream_t s, bool trailing = false);
+                  int ldc, hipStream_t s, bool trailing = false, int max_wgs = 0);
 // Same with BF16 operands packed k-contiguous: Ap row-major m x k (ld ldak),
 // Bp column-major k x n (ld ldbk); FP32 accumulation into C.
 void gemm_bf16_nn_sub(int m, int n, int k, const __bf16 *Ap, int ldak, const __bf16 *Bp, int ldbk,
