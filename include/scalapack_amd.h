@@ -45,9 +45,8 @@ void pdgetrf_(const int *m, const int *n, double *a, const int *ia, const int *j
               const int *desca, int *ipiv, int *info);
 
 /* PDGETRS: solve with the factors of PDGETRF.  Replaces SRC/pdgetrs.f:1-2
- * (checks :184-237, PDLAPIV :255, PDTRSM :260,265).  TRANS = 'N' only;
- * 'T' and 'C' (pdgetrs.f:273-284, not on the PDGESV path) return INFO = -1
- * with a message. */
+ * (checks :184-237; TRANS = 'N': PDLAPIV :255, PDTRSM :260,265; TRANS = 'T'
+ * or 'C': PDTRSM :276,279 with U^T and L^T, PDLAPIV backward :282). */
 void pdgetrs_(const char *trans, const int *n, const int *nrhs, const double *a, const int *ia,
               const int *ja, const int *desca, const int *ipiv, double *b, const int *ib,
               const int *jb, const int *descb, int *info, size_t trans_len);
@@ -63,6 +62,30 @@ void pdgetf2_(const int *m, const int *n, double *a, const int *ia, const int *j
 void pdlaswp_(const char *direc, const char *rowcol, const int *n, double *a, const int *ia,
               const int *ja, const int *desca, const int *k1, const int *k2, const int *ipiv,
               size_t direc_len, size_t rowcol_len);
+
+/* PDLANGE: 'M' max |a_ij|, '1'/'O' one norm, 'I' infinity norm, 'F'/'E'
+ * Frobenius norm of sub(A).  Replaces SRC/pdlange.f:1-2 (:196-330); WORK is
+ * not referenced.  The one and infinity norms sum in the reference's order. */
+double pdlange_(const char *norm, const int *m, const int *n, const double *a, const int *ia,
+                const int *ja, const int *desca, double *work, size_t norm_len);
+
+/* PDGECON: reciprocal condition number estimate from the factors of
+ * PDGETRF.  Replaces SRC/pdgecon.f:1-3 (checks :215-285, quick returns
+ * :287-297, PDLACON loop :324-389 with PDLATRS = triangular solves).  LWORK /
+ * LIWORK minimums and the LWORK = -1 query as the reference. */
+void pdgecon_(const char *norm, const int *n, const double *a, const int *ia, const int *ja,
+              const int *desca, const double *anorm, double *rcond, double *work, const int *lwork,
+              int *iwork, const int *liwork, int *info, size_t norm_len);
+
+/* PDGERFS: iterative refinement with forward / backward error bounds.
+ * Replaces SRC/pdgerfs.f:1-4 (checks :308-404, quick return :440-446,
+ * refinement :480-556, error bound :557-666). */
+void pdgerfs_(const char *trans, const int *n, const int *nrhs, const double *a, const int *ia,
+              const int *ja, const int *desca, const double *af, const int *iaf, const int *jaf,
+              const int *descaf, const int *ipiv, const double *b, const int *ib, const int *jb,
+              const int *descb, double *x, const int *ix, const int *jx, const int *descx,
+              double *ferr, double *berr, double *work, const int *lwork, int *iwork,
+              const int *liwork, int *info, size_t trans_len);
 
 /* ------------------------------------------------------------------------ */
 /* Part 2: device-resident interface.                                        */
@@ -95,7 +118,10 @@ int sla_set_stream(sla_handle_t h, void *stream);
  * PDGETF2), "lookahead" (0/1, default 1: block column p+1 is factored on a
  * second stream during p's update), "update_cus" (0 = all; else the trailing
  * update runs as a persistent grid sized for this many CUs, leaving the
- * others to the panel). */
+ * others to the panel), "lacon_lapack" (0/1, default 0: the condition and
+ * forward-error estimates of sla_dgecon / sla_dgerfs / pdgecon_ / pdgerfs_
+ * return what the reference's PDLACON returns; 1: LAPACK DLACON's
+ * iteration, never smaller and usually closer to the true norm). */
 int sla_set_param(sla_handle_t h, const char *name, int value);
 int sla_get_param(sla_handle_t h, const char *name, int *value);
 const char *sla_last_error(void);
@@ -117,6 +143,24 @@ int sla_dgetrs(sla_handle_t h, int n, int nrhs, const double *dA, int lda, const
                double *dB, int ldb);
 int sla_dgesv(sla_handle_t h, int n, int nrhs, double *dA, int lda, int *dipiv, double *dB,
               int ldb, int *info);
+/* Solve op(A) X = B, op(A) = A ('N') or A^T ('T', 'C'), with the factors of
+ * sla_dgetrf (PDGETRS with TRANS, SRC/pdgetrs.f:255-284). */
+int sla_dgetrs_trans(sla_handle_t h, char trans, int n, int nrhs, const double *dA, int lda,
+                     const int *dipiv, double *dB, int ldb);
+/* PDLANGE of an m x n device matrix (SRC/pdlange.f:196-330); norm one of
+ * 'M', '1', 'O', 'I', 'F', 'E'. */
+int sla_dlange(sla_handle_t h, char norm, int m, int n, const double *dA, int lda,
+               double *value);
+/* PDGECON with the factors of sla_dgetrf (SRC/pdgecon.f:287-403): *rcond =
+ * 1 / (anorm * estimate of ||A^{-1}||) in the '1' / 'O' or 'I' norm. */
+int sla_dgecon(sla_handle_t h, char norm, int n, const double *dLU, int lda, double anorm,
+               double *rcond);
+/* PDGERFS (SRC/pdgerfs.f:473-675): refines the solutions dX of op(A) X = dB
+ * given A (dA) and its factors (dAF, dipiv from sla_dgetrf); ferr / berr are
+ * host arrays of nrhs doubles (ferr[k] is left as it was when x_k is zero). */
+int sla_dgerfs(sla_handle_t h, char trans, int n, int nrhs, const double *dA, int lda,
+               const double *dAF, int ldaf, const int *dipiv, const double *dB, int ldb, double *dX,
+               int ldx, double *ferr, double *berr);
 
 /* The update GEMM alone: C[m x n] -= A[m x k] * B[k x n] (FP64,
  * column-major, device pointers) -- PDGEMM's local dgemm_('N','N', ..., -1,

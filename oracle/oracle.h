@@ -17,6 +17,10 @@
  *   oracle_sresid   TESTING/traditional/LIN/pdlaschk.f:185-187,274-299
  *   oracle_fresid   TESTING/traditional/LIN/pdlafchk.f:199-226 with the
  *                   P*L*U rebuild of pdgetrrv.f:210-289
+ *   oracle_pdlange  SRC/pdlange.f:196-330          (aux_oracle.c)
+ *   oracle_pdgecon  SRC/pdgecon.f:287-403 with SRC/pdlacon.f
+ *   oracle_pdgerfs  SRC/pdgerfs.f:440-675 (PDAGEMV as PTZBLAS/dagemv.f)
+ *   oracle_pdgetrs_t SRC/pdgetrs.f:273-284       (TRANS = 'T')
  * Arithmetic goes through the host Fortran BLAS (dgemm_, dtrsm_, idamax_,
  * dswap_, dscal_, dger_) in the order the reference issues those calls, with
  * PDTRSM's 32-row blocking (PBLAS/SRC/PTOOLS/PB_CptrsmAB.c:301-412,
@@ -52,6 +56,18 @@ void oracle_matgen(int M, int N, int seed, int i0, int j0, int mloc, int nloc,
                    double *out, int ldo);
 /* The LCG state after k steps from seed (X_k, 0 <= X_k < 2^31). */
 unsigned oracle_lcg_state(int seed, long long k);
+
+double oracle_pdlange(char norm, int m, int n, const double *a, int ia, int ja,
+                      const int *desca);
+/* Returns RCOND for the L\U factors at sub(A). */
+double oracle_pdgecon(char norm, int n, const double *a, int ia, int ja, const int *desca,
+                      double anorm);
+int oracle_pdgetrs_t(int n, int nrhs, const double *a, int ia, int ja, const int *desca,
+                     const int *ipiv, double *b, int ib, int jb, const int *descb);
+/* IA = JA = IAF = JAF = IB = JB = IX = JX = 1; IPIV 1-based. */
+int oracle_pdgerfs(char trans, int n, int nrhs, const double *a, int lda, const double *af,
+                   int ldaf, const int *ipiv, const double *b, int ldb, double *x, int ldx,
+                   double *ferr, double *berr);
 
 double oracle_norm_inf(int m, int n, const double *a, int lda);
 double oracle_sresid(int n, int nrhs, const double *a, int lda, const double *x,

@@ -14,6 +14,8 @@
  *  max_j ||A x_j - b_j||_inf / (||A||_inf ||x_j||_inf eps N), eps = 2^-53).
  *
  * usage: ref_lu M N NB NRHS P Q SEEDA BMODE OUT [--lu] [--nosolve]
+ *        ref_lu aux N NB NRHS SEEDA SEEDB OUT      (1 x 1 grid, see aux())
+ *        ref_lu argcheck ROUTINE ARGS...           (see argcheck())
  *   BMODE = lcg:<seed>  (PDMATGEN of B, as the tester)
  *         | ones        (b = A * ones(N), summed left to right in double)
  *         | file:<path> (N x NRHS column-major doubles)
@@ -108,11 +110,150 @@ static int argcheck(int argc, char **argv)
     return 0;
 }
 
+extern double pdlange_(const char *, int *, int *, double *, int *, int *, int *, double *,
+                       size_t);
+extern void pdgecon_(const char *, int *, double *, int *, int *, int *, double *, double *,
+                     double *, int *, int *, int *, int *, size_t);
+extern void pdgerfs_(const char *, int *, int *, double *, int *, int *, int *, double *, int *,
+                     int *, int *, int *, double *, int *, int *, int *, double *, int *, int *,
+                     int *, double *, double *, double *, int *, int *, int *, int *, size_t);
+
+/* aux N NB NRHS SEEDA SEEDB OUT: the routines beside the solve on a 1 x 1
+ * grid, A = PDMATGEN(SEEDA) N x N, B = PDMATGEN(SEEDB) N x NRHS.
+ *   PDLANGE 'M','1','I','F' of A and of the submatrix A(2:N-2, 3:N)
+ *   PDGETRF of A, PDGECON '1' and 'I' with the matching PDLANGE norms
+ *   PDGETRS 'T' of B
+ *   PDGERFS 'N' from the PDGETRS 'N' solution and from a perturbed start,
+ *   PDGERFS 'T' from a perturbed start of the 'T' solution
+ * OUT: int32 magic 0x41555852, N, NB, NRHS, info_getrf, info_gecon1,
+ *      info_geconI, info_getrsT, info_rfs[3]
+ *      double norms[4], subnorms[4], rcond1, rcondI
+ *      int32 ipiv[N]
+ *      double xT[N*NRHS], then per refinement case c = 0, 1, 2:
+ *      x0[N*NRHS] (start), x[N*NRHS] (refined), ferr[NRHS], berr[NRHS] */
+static int aux(int argc, char **argv)
+{
+    if (argc < 8) {
+        fprintf(stderr, "usage: ref_lu aux N NB NRHS SEEDA SEEDB OUT\n");
+        return 2;
+    }
+    int N = atoi(argv[2]), NB = atoi(argv[3]), NRHS = atoi(argv[4]);
+    int seedA = atoi(argv[5]), seedB = atoi(argv[6]);
+    const char *out = argv[7];
+    int ictxt, neg1 = -1, zero = 0, one = 1, info = 0;
+    blacs_get_(&neg1, &zero, &ictxt);
+    blacs_gridinit_(&ictxt, "Row-major", &one, &one, 9);
+    int lld = N > 1 ? N : 1;
+    int desca[9], descb[9];
+    descinit_(desca, &N, &N, &NB, &NB, &zero, &zero, &ictxt, &lld, &info);
+    descinit_(descb, &N, &NRHS, &NB, &NB, &zero, &zero, &ictxt, &lld, &info);
+    size_t nn = (size_t)N * N, nr = (size_t)N * NRHS;
+    double *A = malloc(nn * 8), *AF = malloc(nn * 8), *B = malloc(nr * 8);
+    double *X = malloc(nr * 8), *XT = malloc(nr * 8), *X0 = malloc(3 * nr * 8),
+           *XR = malloc(3 * nr * 8);
+    double *ferr = calloc(3 * NRHS, 8), *berr = calloc(3 * NRHS, 8);
+    int *ipiv = calloc((size_t)N + NB, sizeof(int));
+    gen_global(ictxt, N, N, NB, seedA, A);
+    gen_global(ictxt, N, NRHS, NB, seedB, B);
+
+    const char nrm[4] = {'M', '1', 'I', 'F'};
+    double norms[4], subnorms[4];
+    double *lw = malloc(((size_t)N + 64) * 8);
+    for (int k = 0; k < 4; k++) norms[k] = pdlange_(&nrm[k], &N, &N, A, &one, &one, desca, lw, 1);
+    {
+        int ms = N - 3, ns = N - 2, ia = 2, ja = 3;
+        for (int k = 0; k < 4; k++)
+            subnorms[k] = pdlange_(&nrm[k], &ms, &ns, A, &ia, &ja, desca, lw, 1);
+    }
+    memcpy(AF, A, nn * 8);
+    int info_f = 0;
+    pdgetrf_(&N, &N, AF, &one, &one, desca, ipiv, &info_f);
+
+    /* PDGECON with workspace from its own query */
+    double rcond[2] = {0.0, 0.0};
+    int info_c[2] = {0, 0};
+    for (int k = 0; k < 2; k++) {
+        const char cn = k == 0 ? '1' : 'I';
+        double anorm = k == 0 ? norms[1] : norms[2], wq;
+        int iq, lq = -1, inf = 0;
+        pdgecon_(&cn, &N, AF, &one, &one, desca, &anorm, &rcond[k], &wq, &lq, &iq, &lq, &inf, 1);
+        int lwork = (int)wq, liwork = iq;
+        double *work = malloc((size_t)lwork * 8);
+        int *iwork = malloc((size_t)liwork * 4);
+        pdgecon_(&cn, &N, AF, &one, &one, desca, &anorm, &rcond[k], work, &lwork, iwork, &liwork,
+                 &info_c[k], 1);
+        free(work);
+        free(iwork);
+    }
+
+    /* PDGETRS 'N' and 'T' */
+    int info_t = 0, info_n = 0;
+    memcpy(XT, B, nr * 8);
+    pdgetrs_("T", &N, &NRHS, AF, &one, &one, desca, ipiv, XT, &one, &one, descb, &info_t, 1);
+    memcpy(X, B, nr * 8);
+    pdgetrs_("N", &N, &NRHS, AF, &one, &one, desca, ipiv, X, &one, &one, descb, &info_n, 1);
+
+    /* PDGERFS: case 0 'N' from X, case 1 'N' from X perturbed, case 2 'T'
+     * from XT perturbed; perturbation x_i (1 + 1e-7 ((i mod 7) - 3)). */
+    int info_r[3] = {0, 0, 0};
+    for (int c = 0; c < 3; c++) {
+        double *x0 = X0 + (size_t)c * nr, *xr = XR + (size_t)c * nr;
+        const double *src = c == 2 ? XT : X;
+        for (size_t i = 0; i < nr; i++) {
+            const int row = (int)(i % (size_t)N);
+            x0[i] = c == 0 ? src[i] : src[i] * (1.0 + 1e-7 * (double)((row % 7) - 3));
+        }
+        memcpy(xr, x0, nr * 8);
+        const char tr = c == 2 ? 'T' : 'N';
+        double wq;
+        int iq, lq = -1, inf = 0;
+        pdgerfs_(&tr, &N, &NRHS, A, &one, &one, desca, AF, &one, &one, desca, ipiv, B, &one, &one,
+                 descb, xr, &one, &one, descb, ferr + c * NRHS, berr + c * NRHS, &wq, &lq, &iq,
+                 &lq, &inf, 1);
+        int lwork = (int)wq, liwork = iq;
+        double *work = malloc((size_t)lwork * 8);
+        int *iwork = malloc((size_t)liwork * 4);
+        pdgerfs_(&tr, &N, &NRHS, A, &one, &one, desca, AF, &one, &one, desca, ipiv, B, &one, &one,
+                 descb, xr, &one, &one, descb, ferr + c * NRHS, berr + c * NRHS, work, &lwork,
+                 iwork, &liwork, &info_r[c], 1);
+        free(work);
+        free(iwork);
+    }
+
+    FILE *f = fopen(out, "wb");
+    int32_t hdr[11] = {0x41555852, N, NB, NRHS, info_f, info_c[0], info_c[1], info_t,
+                       info_r[0], info_r[1], info_r[2]};
+    fwrite(hdr, sizeof hdr, 1, f);
+    fwrite(norms, 8, 4, f);
+    fwrite(subnorms, 8, 4, f);
+    fwrite(rcond, 8, 2, f);
+    fwrite(ipiv, 4, (size_t)N, f);
+    fwrite(XT, 8, nr, f);
+    for (int c = 0; c < 3; c++) {
+        fwrite(X0 + (size_t)c * nr, 8, nr, f);
+        fwrite(XR + (size_t)c * nr, 8, nr, f);
+        fwrite(ferr + c * NRHS, 8, (size_t)NRHS, f);
+        fwrite(berr + c * NRHS, 8, (size_t)NRHS, f);
+    }
+    fclose(f);
+    printf("{\"N\": %d, \"NB\": %d, \"NRHS\": %d, \"rcond1\": %.17g, \"rcondI\": %.17g, "
+           "\"info\": [%d, %d, %d, %d, %d, %d, %d]}\n",
+           N, NB, NRHS, rcond[0], rcond[1], info_f, info_c[0], info_c[1], info_t, info_r[0],
+           info_r[1], info_r[2]);
+    free(A); free(AF); free(B); free(X); free(XT); free(X0); free(XR); free(ferr); free(berr);
+    free(ipiv); free(lw);
+    blacs_gridexit_(&ictxt);
+    int cont = 0;
+    blacs_exit_(&cont);
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     int iam, nprocs;
     blacs_pinfo_(&iam, &nprocs);
     if (argc > 2 && !strcmp(argv[1], "argcheck")) return argcheck(argc, argv);
+    if (argc > 1 && !strcmp(argv[1], "aux")) return aux(argc, argv);
     if (argc < 10) {
         if (iam == 0)
             fprintf(stderr, "usage: ref_lu M N NB NRHS P Q SEEDA BMODE OUT [--lu] [--nosolve]\n");

@@ -48,6 +48,8 @@ def _declare(lib: ctypes.CDLL) -> None:
         "pdgetf2_": [vp] * 8,
         "pdgetrs_": [vp] * 13 + [sz],
         "pdlaswp_": [vp] * 10 + [sz, sz],
+        "pdgecon_": [vp] * 13 + [sz],
+        "pdgerfs_": [vp] * 27 + [sz],
         "sla_create": [vp],
         "sla_destroy": [vp],
         "sla_set_stream": [vp, vp],
@@ -65,14 +67,19 @@ def _declare(lib: ctypes.CDLL) -> None:
         "sla_gemm_lowp_sub": [vp, i, i, i, i, vp, i, vp, i, vp, i],
         "sla_sgetrs": [vp, i, i, vp, i, vp, vp, i],
         "sla_dresidual": [vp, i, i, vp, i, vp, i, vp, i, vp, i, vp],
+        "sla_dgetrs_trans": [vp, ctypes.c_char, i, i, vp, i, vp, vp, i],
+        "sla_dlange": [vp, ctypes.c_char, i, i, vp, i, vp],
+        "sla_dgecon": [vp, ctypes.c_char, i, vp, i, d, vp],
+        "sla_dgerfs": [vp, ctypes.c_char, i, i, vp, i, vp, i, vp, vp, i, vp, i, vp, vp],
     }
     for name, args in sigs.items():
         f = getattr(lib, name)
         f.argtypes = args
         f.restype = None if name.endswith("_") else ctypes.c_int
+    lib.pdlange_.argtypes = [vp] * 8 + [sz]
+    lib.pdlange_.restype = ctypes.c_double
     lib.sla_last_error.argtypes = []
     lib.sla_last_error.restype = ctypes.c_char_p
-    del d
 
 
 def load() -> ctypes.CDLL:

@@ -13,7 +13,8 @@ libscalapack_amd.so through its C-ABI (the symbols a Fortran program links).
     info = pdgesv(n, nrhs, a, 1, 1, desca, ipiv, b, 1, 1, descb)
 
 References: SRC/pdgesv.f:1-2, SRC/pdgetrf.f:1, SRC/pdgetrs.f:1-2,
-SRC/pdgetf2.f:1, SRC/pdlaswp.f:1-2, TOOLS/descinit.f, TOOLS/numroc.f.
+SRC/pdgetf2.f:1, SRC/pdlaswp.f:1-2, SRC/pdlange.f:1-2, SRC/pdgecon.f:1-3,
+SRC/pdgerfs.f:1-4, TOOLS/descinit.f, TOOLS/numroc.f.
 """
 from __future__ import annotations
 
@@ -129,6 +130,47 @@ def pdgesv(n, nrhs, a, ia, ja, desca, ipiv, b, ib, jb, descb) -> int:
     load().pdgesv_(_ref(_i(n)), _ref(_i(nrhs)), _arr(a, np.float64), _ref(_i(ia)), _ref(_i(ja)),
                    ctypes.c_void_p(da.ctypes.data), _arr(ipiv, np.int32), _arr(b, np.float64),
                    _ref(_i(ib)), _ref(_i(jb)), ctypes.c_void_p(db.ctypes.data), _ref(info))
+    return info.value
+
+
+def pdlange(norm, m, n, a, ia, ja, desca, work=None) -> float:
+    """PDLANGE (SRC/pdlange.f): 'M', '1'/'O', 'I' or 'F'/'E' norm of sub(A)."""
+    d = _desc(desca)
+    w = np.zeros(1) if work is None else work
+    c = ctypes.c_char(norm.encode()[:1])
+    return load().pdlange_(_ref(c), _ref(_i(m)), _ref(_i(n)), _arr(a, np.float64), _ref(_i(ia)),
+                           _ref(_i(ja)), ctypes.c_void_p(d.ctypes.data), _arr(w, np.float64),
+                           ctypes.c_size_t(1))
+
+
+def pdgecon(norm, n, a, ia, ja, desca, anorm, work, lwork, iwork, liwork) -> tuple[float, int]:
+    """PDGECON (SRC/pdgecon.f): returns (RCOND, INFO).  WORK(1) / IWORK(1)
+    receive the minimal LWORK / LIWORK (LWORK = -1 or LIWORK = -1 queries)."""
+    d = _desc(desca)
+    rcond, info = ctypes.c_double(), ctypes.c_int()
+    c = ctypes.c_char(norm.encode()[:1])
+    load().pdgecon_(_ref(c), _ref(_i(n)), _arr(a, np.float64), _ref(_i(ia)), _ref(_i(ja)),
+                    ctypes.c_void_p(d.ctypes.data), _ref(ctypes.c_double(anorm)), _ref(rcond),
+                    _arr(work, np.float64), _ref(_i(lwork)), _arr(iwork, np.int32),
+                    _ref(_i(liwork)), _ref(info), ctypes.c_size_t(1))
+    return rcond.value, info.value
+
+
+def pdgerfs(trans, n, nrhs, a, ia, ja, desca, af, iaf, jaf, descaf, ipiv, b, ib, jb, descb, x,
+            ix, jx, descx, ferr, berr, work, lwork, iwork, liwork) -> int:
+    """PDGERFS (SRC/pdgerfs.f): refines X in place, FERR / BERR per column."""
+    da, daf, db, dx = _desc(desca), _desc(descaf), _desc(descb), _desc(descx)
+    info = ctypes.c_int()
+    t = ctypes.c_char(trans.encode()[:1])
+    vp = ctypes.c_void_p
+    load().pdgerfs_(_ref(t), _ref(_i(n)), _ref(_i(nrhs)), _arr(a, np.float64), _ref(_i(ia)),
+                    _ref(_i(ja)), vp(da.ctypes.data), _arr(af, np.float64), _ref(_i(iaf)),
+                    _ref(_i(jaf)), vp(daf.ctypes.data), _arr(ipiv, np.int32),
+                    _arr(b, np.float64), _ref(_i(ib)), _ref(_i(jb)), vp(db.ctypes.data),
+                    _arr(x, np.float64), _ref(_i(ix)), _ref(_i(jx)), vp(dx.ctypes.data),
+                    _arr(ferr, np.float64), _arr(berr, np.float64), _arr(work, np.float64),
+                    _ref(_i(lwork)), _arr(iwork, np.int32), _ref(_i(liwork)), _ref(info),
+                    ctypes.c_size_t(1))
     return info.value
 
 

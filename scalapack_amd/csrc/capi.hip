@@ -12,6 +12,7 @@
 #include "sla_handle.h"
 #include "sla_kernels.h"
 #include "sla_mixed.h"
+#include "sla_aux.h"
 
 #include "scalapack_amd.h"
 #include "sla_blacs1x1.h"
@@ -19,6 +20,7 @@
 #include <dlfcn.h>
 
 #include <algorithm>
+#include <cctype>
 #include <climits>
 #include <cmath>
 #include <cstring>
@@ -121,6 +123,21 @@ sla::Handle &default_handle()
     return *p;
 }
 
+// The handle caches the row permutation of its last factorization keyed on
+// the device addresses; the Fortran entry points free their device copies on
+// return, so they drop that association when they finish.
+struct PermKeyReset {
+    sla::Handle &h;
+    ~PermKeyReset()
+    {
+        h.perm_key_a = nullptr;
+        h.perm_key_ipiv = nullptr;
+        h.perm_m = 0;
+    }
+};
+
+inline bool lsame(char a, char b) { return std::toupper((unsigned char)a) == b; }
+
 // Device copies of host arrays for the Fortran entry points.
 struct DevBuf {
     void *p = nullptr;
@@ -214,6 +231,7 @@ void pdgetrf_(const int *m, const int *n, double *a, const int *ia, const int *j
     }
     try {
         sla::Handle &h = default_handle();
+        PermKeyReset reset{h};
         const int ldd = pad_ld(*m);
         DevBuf dA((size_t)ldd * *n * 8), dp((size_t)std::min(*m, *n) * 4);
         *info = getrf_host(h, *m, *n, a, *ia, *ja, D(desca, LLD_), ipiv, dA, ldd, dp);
@@ -257,6 +275,7 @@ void pdgetf2_(const int *m, const int *n, double *a, const int *ia, const int *j
     }
     try {
         sla::Handle &h = default_handle();
+        PermKeyReset reset{h};
         const int ldd = pad_ld(*m);
         DevBuf dA((size_t)ldd * *n * 8), dp((size_t)std::min(*m, *n) * 4);
         *info = getrf_host(h, *m, *n, a, *ia, *ja, D(desca, LLD_), ipiv, dA, ldd, dp);
@@ -310,13 +329,10 @@ void pdgetrs_(const char *trans, const int *n, const int *nrhs, const double *a,
         pxerbla(g, "PDGETRS", -*info);
         return;
     }
-    if (!(*trans == 'N' || *trans == 'n')) {
-        std::printf("PDGETRS: TRANS = '%c' is not provided by this library (only 'N')\n", *trans);
-        *info = -1;
-        return;
-    }
+    const bool notran = lsame(*trans, 'N');
     try {
         sla::Handle &h = default_handle();
+        PermKeyReset reset{h};
         const int N = *n, NR = *nrhs, ldd = pad_ld(N), lldb = D(descb, LLD_);
         DevBuf dA((size_t)ldd * N * 8), dB((size_t)ldd * NR * 8), dp((size_t)N * 4);
         to_device(dA.as<double>(), ldd, a + (size_t)(*ia - 1) + (size_t)(*ja - 1) * D(desca, LLD_),
@@ -326,7 +342,8 @@ void pdgetrs_(const char *trans, const int *n, const int *nrhs, const double *a,
         std::vector<int> piv(N);
         for (int i = 0; i < N; i++) piv[i] = ipiv[(*ia - 1) + i] - (*ia - 1);  // pdlapv2.f:229
         SLA_HIP(hipMemcpy(dp.p, piv.data(), sizeof(int) * N, hipMemcpyHostToDevice));
-        sla::dgetrs_device(h, N, NR, dA.as<double>(), ldd, dp.as<int>(), dB.as<double>(), ldd);
+        sla::dgetrs_device(h, N, NR, dA.as<double>(), ldd, dp.as<int>(), dB.as<double>(), ldd,
+                           !notran);
         to_host(bsub, lldb, dB.as<double>(), ldd, N, NR);
     } catch (const std::exception &e) {
         fatal("PDGETRS", e);
@@ -376,6 +393,7 @@ void pdgesv_(const int *n, const int *nrhs, double *a, const int *ia, const int 
     }
     try {
         sla::Handle &h = default_handle();
+        PermKeyReset reset{h};
         const int N = *n, NR = *nrhs, ldd = pad_ld(N), lldb = D(descb, LLD_);
         DevBuf dA((size_t)ldd * N * 8), dp((size_t)N * 4), dB((size_t)ldd * std::max(NR, 1) * 8);
         if (D(desca, M_) != 1)
@@ -457,6 +475,226 @@ void pdlaswp_(const char *direc, const char *rowcol, const int *n, double *a, co
     }
 }
 
+// PDLANGE (SRC/pdlange.f:1-2): a function with no argument checks; WORK is
+// not used on the device path.
+double pdlange_(const char *norm, const int *m, const int *n, const double *a, const int *ia,
+                const int *ja, const int *desca, double *work, size_t)
+{
+    (void)work;
+    const Grid g = gridinfo(D(desca, CTXT_));
+    if (std::min(*m, *n) <= 0) return 0.0;
+    if (g.nprow * g.npcol != 1) {
+        auto next = next_symbol<double (*)(const char *, const int *, const int *, const double *,
+                                           const int *, const int *, const int *, double *,
+                                           size_t)>("pdlange_");
+        if (next) return next(norm, m, n, a, ia, ja, desca, work, 1);
+        pxerbla(g, "PDLANGE", 7);
+        return 0.0;
+    }
+    const char c = (char)std::toupper((unsigned char)*norm);
+    if (!(c == 'M' || c == '1' || c == 'O' || c == 'I' || c == 'F' || c == 'E')) return 0.0;
+    try {
+        sla::Handle &h = default_handle();
+        const int M = *m, N = *n, lld = D(desca, LLD_), ldd = pad_ld(M);
+        DevBuf dA((size_t)ldd * N * 8);
+        to_device(dA.as<double>(), ldd, a + (size_t)(*ia - 1) + (size_t)(*ja - 1) * lld, lld, M, N);
+        return sla::dlange_device(h, c, M, N, dA.as<double>(), ldd);
+    } catch (const std::exception &e) {
+        fatal("PDLANGE", e);
+    }
+}
+
+// PDGECON (SRC/pdgecon.f:1-3); argument checks of pdgecon.f:215-285.
+void pdgecon_(const char *norm, const int *n, const double *a, const int *ia, const int *ja,
+              const int *desca, const double *anorm, double *rcond, double *work, const int *lwork,
+              int *iwork, const int *liwork, int *info, size_t)
+{
+    const Grid g = gridinfo(D(desca, CTXT_));
+    *info = 0;
+    int lwmin = 0, liwmin = 0;
+    bool lquery = false;
+    const bool onenrm = (*norm == '1' || lsame(*norm, 'O'));
+    if (g.nprow == -1) {
+        *info = -(600 + CTXT_);
+    } else {
+        chk1mat(*n, 2, *n, 2, *ia, *ja, desca, 6, *info);
+        if (*info == 0) {
+            const int mb = D(desca, MB_), nb = D(desca, NB_);
+            const int iarow = indxg2p(*ia, mb, D(desca, RSRC_), g.nprow);
+            const int iacol = indxg2p(*ja, nb, D(desca, CSRC_), g.npcol);
+            const int npmod = numroc(*n + (*ia - 1) % mb, mb, g.myrow, iarow, g.nprow);
+            const int nqmod = numroc(*n + (*ja - 1) % nb, nb, g.mycol, iacol, g.npcol);
+            auto iceil = [](int x, int y) { return (x + y - 1) / y; };
+            lwmin = 2 * npmod + 2 * nqmod +
+                    std::max(2, std::max(nb * std::max(1, iceil(g.nprow - 1, g.npcol)),
+                                         nqmod + nb * std::max(1, iceil(g.npcol - 1, g.nprow))));
+            work[0] = (double)lwmin;
+            liwmin = std::max(1, npmod);
+            iwork[0] = liwmin;
+            lquery = (*lwork == -1 || *liwork == -1);
+            if (!onenrm && !lsame(*norm, 'I')) *info = -1;
+            else if (*anorm < 0.0) *info = -7;
+            else if (*lwork < lwmin && !lquery) *info = -10;
+            else if (*liwork < liwmin && !lquery) *info = -12;
+        }
+    }
+    if (*info != 0) {
+        pxerbla(g, "PDGECON", -*info);
+        return;
+    }
+    if (lquery) return;
+    if (g.nprow * g.npcol != 1) {
+        auto next = next_symbol<void (*)(const char *, const int *, const double *, const int *,
+                                         const int *, const int *, const double *, double *,
+                                         double *, const int *, int *, const int *, int *,
+                                         size_t)>("pdgecon_");
+        if (next) return next(norm, n, a, ia, ja, desca, anorm, rcond, work, lwork, iwork, liwork,
+                              info, 1);
+        *info = -(600 + CTXT_);
+        pxerbla(g, "PDGECON", -*info);
+        return;
+    }
+    // quick returns, pdgecon.f:287-297
+    *rcond = 0.0;
+    if (*n == 0) {
+        *rcond = 1.0;
+        return;
+    }
+    if (*anorm == 0.0) return;
+    if (*n == 1) {
+        *rcond = 1.0;
+        return;
+    }
+    try {
+        sla::Handle &h = default_handle();
+        const int N = *n, lld = D(desca, LLD_), ldd = pad_ld(N);
+        DevBuf dA((size_t)ldd * N * 8);
+        to_device(dA.as<double>(), ldd, a + (size_t)(*ia - 1) + (size_t)(*ja - 1) * lld, lld, N, N);
+        *rcond = sla::dgecon_device(h, onenrm ? '1' : 'I', N, dA.as<double>(), ldd, *anorm);
+    } catch (const std::exception &e) {
+        fatal("PDGECON", e);
+    }
+}
+
+// PDGERFS (SRC/pdgerfs.f:1-4); argument checks of pdgerfs.f:308-404.
+void pdgerfs_(const char *trans, const int *n, const int *nrhs, const double *a, const int *ia,
+              const int *ja, const int *desca, const double *af, const int *iaf, const int *jaf,
+              const int *descaf, const int *ipiv, const double *b, const int *ib, const int *jb,
+              const int *descb, double *x, const int *ix, const int *jx, const int *descx,
+              double *ferr, double *berr, double *work, const int *lwork, int *iwork,
+              const int *liwork, int *info, size_t)
+{
+    const Grid g = gridinfo(D(desca, CTXT_));
+    const bool notran = lsame(*trans, 'N');
+    *info = 0;
+    bool lquery = false;
+    if (g.nprow == -1) {
+        *info = -(700 + CTXT_);
+    } else {
+        chk1mat(*n, 2, *n, 2, *ia, *ja, desca, 7, *info);
+        chk1mat(*n, 2, *n, 2, *iaf, *jaf, descaf, 11, *info);
+        chk1mat(*n, 2, *nrhs, 3, *ib, *jb, descb, 16, *info);
+        chk1mat(*n, 2, *nrhs, 3, *ix, *jx, descx, 20, *info);
+        if (*info == 0) {
+            const int iroffa = (*ia - 1) % D(desca, MB_), icoffa = (*ja - 1) % D(desca, NB_);
+            const int iroffaf = (*iaf - 1) % D(descaf, MB_), icoffaf = (*jaf - 1) % D(descaf, NB_);
+            const int iroffb = (*ib - 1) % D(descb, MB_), icoffb = (*jb - 1) % D(descb, NB_);
+            const int iroffx = (*ix - 1) % D(descx, MB_), icoffx = (*jx - 1) % D(descx, NB_);
+            const int iarow = indxg2p(*ia, D(desca, MB_), D(desca, RSRC_), g.nprow);
+            const int iafcol = indxg2p(*jaf, D(descaf, NB_), D(descaf, CSRC_), g.npcol);
+            const int iafrow = indxg2p(*iaf, D(descaf, MB_), D(descaf, RSRC_), g.nprow);
+            const int iacol = indxg2p(*ja, D(desca, NB_), D(desca, CSRC_), g.npcol);
+            const int ixbrow = indxg2p(*ib, D(descb, MB_), D(descb, RSRC_), g.nprow);
+            const int ixbcol = indxg2p(*jb, D(descb, NB_), D(descb, CSRC_), g.npcol);
+            const int ixrow = indxg2p(*ix, D(descx, MB_), D(descx, RSRC_), g.nprow);
+            const int ixcol = indxg2p(*jx, D(descx, NB_), D(descx, CSRC_), g.npcol);
+            const int npmod = numroc(*n + iroffa, D(desca, MB_), g.myrow, iarow, g.nprow);
+            const int lwmin = 3 * npmod, liwmin = npmod;
+            work[0] = (double)lwmin;
+            iwork[0] = liwmin;
+            lquery = (*lwork == -1 || *liwork == -1);
+            const int ictxt = D(desca, CTXT_);
+            if (!notran && !lsame(*trans, 'T') && !lsame(*trans, 'C')) *info = -1;
+            else if (*n < 0) *info = -2;
+            else if (*nrhs < 0) *info = -3;
+            else if (iroffa != 0) *info = -5;
+            else if (icoffa != 0) *info = -6;
+            else if (D(desca, MB_) != D(desca, NB_)) *info = -(700 + NB_);
+            else if (D(desca, MB_) != D(descaf, MB_)) *info = -(1100 + MB_);
+            else if (iroffaf != 0 || iarow != iafrow) *info = -9;
+            else if (D(desca, NB_) != D(descaf, NB_)) *info = -(1100 + NB_);
+            else if (icoffaf != 0 || iacol != iafcol) *info = -10;
+            else if (ictxt != D(descaf, CTXT_)) *info = -(1100 + CTXT_);
+            else if (iroffa != iroffb || iarow != ixbrow) *info = -14;
+            else if (D(desca, MB_) != D(descb, MB_)) *info = -(1600 + MB_);
+            else if (ictxt != D(descb, CTXT_)) *info = -(1600 + CTXT_);
+            else if (D(descb, MB_) != D(descx, MB_)) *info = -(2000 + MB_);
+            else if (iroffx != 0 || ixbrow != ixrow) *info = -18;
+            else if (D(descb, NB_) != D(descx, NB_)) *info = -(2000 + NB_);
+            else if (icoffb != icoffx || ixbcol != ixcol) *info = -19;
+            else if (ictxt != D(descx, CTXT_)) *info = -(2000 + CTXT_);
+            else if (*lwork < lwmin && !lquery) *info = -24;
+            else if (*liwork < liwmin && !lquery) *info = -26;
+        }
+    }
+    if (*info != 0) {
+        pxerbla(g, "PDGERFS", -*info);
+        return;
+    }
+    if (lquery) return;
+    if (g.nprow * g.npcol != 1) {
+        auto next = next_symbol<void (*)(const char *, const int *, const int *, const double *,
+                                         const int *, const int *, const int *, const double *,
+                                         const int *, const int *, const int *, const int *,
+                                         const double *, const int *, const int *, const int *,
+                                         double *, const int *, const int *, const int *, double *,
+                                         double *, double *, const int *, int *, const int *,
+                                         int *, size_t)>("pdgerfs_");
+        if (next) return next(trans, n, nrhs, a, ia, ja, desca, af, iaf, jaf, descaf, ipiv, b, ib,
+                              jb, descb, x, ix, jx, descx, ferr, berr, work, lwork, iwork, liwork,
+                              info, 1);
+        *info = -(700 + CTXT_);
+        pxerbla(g, "PDGERFS", -*info);
+        return;
+    }
+    const int N = *n, NR = *nrhs;
+    if (N <= 1 || NR == 0) {  // pdgerfs.f:440-446
+        for (int k = 0; k < NR; k++) ferr[(*jb - 1) + k] = berr[(*jb - 1) + k] = 0.0;
+        return;
+    }
+    try {
+        sla::Handle &h = default_handle();
+        PermKeyReset reset{h};
+        const int ldd = pad_ld(N);
+        const int llda = D(desca, LLD_), lldaf = D(descaf, LLD_), lldb = D(descb, LLD_),
+                  lldx = D(descx, LLD_);
+        DevBuf dA((size_t)ldd * N * 8), dAF((size_t)ldd * N * 8), dB((size_t)ldd * NR * 8),
+            dX((size_t)ldd * NR * 8), dp((size_t)N * 4);
+        to_device(dA.as<double>(), ldd, a + (size_t)(*ia - 1) + (size_t)(*ja - 1) * llda, llda, N, N);
+        to_device(dAF.as<double>(), ldd, af + (size_t)(*iaf - 1) + (size_t)(*jaf - 1) * lldaf, lldaf,
+                  N, N);
+        to_device(dB.as<double>(), ldd, b + (size_t)(*ib - 1) + (size_t)(*jb - 1) * lldb, lldb, N, NR);
+        double *xsub = x + (size_t)(*ix - 1) + (size_t)(*jx - 1) * lldx;
+        to_device(dX.as<double>(), ldd, xsub, lldx, N, NR);
+        std::vector<int> piv(N);
+        for (int i = 0; i < N; i++) piv[i] = ipiv[(*iaf - 1) + i] - (*iaf - 1);  // as pdlapv2.f:229
+        SLA_HIP(hipMemcpy(dp.p, piv.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+        // FERR / BERR are indexed by the local column of B (pdgerfs.f:437, JJFBE = JJXB)
+        std::vector<double> fe(NR), be(NR);
+        for (int k = 0; k < NR; k++) fe[k] = ferr[(*jb - 1) + k];
+        sla::dgerfs_device(h, !notran, N, NR, dA.as<double>(), ldd, dAF.as<double>(), ldd,
+                           dp.as<int>(), dB.as<double>(), ldd, dX.as<double>(), ldd, fe.data(),
+                           be.data());
+        to_host(xsub, lldx, dX.as<double>(), ldd, N, NR);
+        for (int k = 0; k < NR; k++) {
+            ferr[(*jb - 1) + k] = fe[k];
+            berr[(*jb - 1) + k] = be[k];
+        }
+    } catch (const std::exception &e) {
+        fatal("PDGERFS", e);
+    }
+}
+
 // ============================================================================
 // Part 2: device-resident interface
 // ============================================================================
@@ -503,7 +741,7 @@ int sla_set_param(sla_handle_t h, const char *name, int value)
     const std::string k(name ? name : "");
     static const char *known[] = {"nb",      "leaf_w",  "panel_wgs",        "profile",
                                   "ir_max_iter", "gmres_m", "ir_gmres_on_stall", "gmres_tol_exp",
-                                  "pivoting", "lookahead", "update_cus"};
+                                  "pivoting", "lookahead", "update_cus", "lacon_lapack"};
     if (std::find_if(std::begin(known), std::end(known), [&](const char *x) { return k == x; }) ==
         std::end(known)) {
         g_last_error = "sla_set_param: unknown parameter " + k;
@@ -520,6 +758,7 @@ int sla_set_param(sla_handle_t h, const char *name, int value)
     else if (k == "pivoting" && (value == 0 || value == 1)) H->pivoting = value;
     else if (k == "lookahead" && (value == 0 || value == 1)) H->lookahead = value;
     else if (k == "update_cus" && value >= 0 && value <= H->num_cus) H->update_cus = value;
+    else if (k == "lacon_lapack" && (value == 0 || value == 1)) H->lacon_lapack = value;
     else if (k == "gmres_tol_exp" && value >= 1 && value <= 15) H->gmres_tol = std::pow(10.0, -value);
     else {
         g_last_error = "sla_set_param: value out of range for " + k;
@@ -543,6 +782,7 @@ int sla_get_param(sla_handle_t h, const char *name, int *value)
     else if (k == "pivoting") *value = H->pivoting;
     else if (k == "lookahead") *value = H->lookahead;
     else if (k == "update_cus") *value = H->update_cus;
+    else if (k == "lacon_lapack") *value = H->lacon_lapack;
     else if (k == "gmres_tol_exp") *value = (int)std::lround(-std::log10(H->gmres_tol));
     else {
         g_last_error = "sla_get_param: unknown parameter " + k;
@@ -590,6 +830,74 @@ int sla_dgetrs(sla_handle_t h, int n, int nrhs, const double *dA, int lda, const
         return SLA_ERR_ARG;
     }
     SLA_TRY(sla::dgetrs_device(*reinterpret_cast<sla::Handle *>(h), n, nrhs, dA, lda, dipiv, dB, ldb))
+}
+
+int sla_dgetrs_trans(sla_handle_t h, char trans, int n, int nrhs, const double *dA, int lda,
+                     const int *dipiv, double *dB, int ldb)
+{
+    SLA_NEED_HANDLE(h);
+    const bool notran = lsame(trans, 'N');
+    if (!notran && !lsame(trans, 'T') && !lsame(trans, 'C')) {
+        g_last_error = "sla_dgetrs_trans: TRANS must be 'N', 'T' or 'C'";
+        return SLA_ERR_ARG;
+    }
+    if (n < 0 || nrhs < 0 || lda < std::max(1, n) || ldb < std::max(1, n)) {
+        g_last_error = "sla_dgetrs_trans: illegal n, nrhs, lda or ldb";
+        return SLA_ERR_ARG;
+    }
+    SLA_TRY(sla::dgetrs_device(*reinterpret_cast<sla::Handle *>(h), n, nrhs, dA, lda, dipiv, dB, ldb,
+                               !notran))
+}
+
+int sla_dlange(sla_handle_t h, char norm, int m, int n, const double *dA, int lda, double *value)
+{
+    SLA_NEED_HANDLE(h);
+    const char c = (char)std::toupper((unsigned char)norm);
+    if (!(c == 'M' || c == '1' || c == 'O' || c == 'I' || c == 'F' || c == 'E')) {
+        g_last_error = "sla_dlange: NORM must be one of M 1 O I F E";
+        return SLA_ERR_ARG;
+    }
+    if (m < 0 || n < 0 || lda < std::max(1, m) || !value) {
+        g_last_error = "sla_dlange: illegal m, n, lda or value";
+        return SLA_ERR_ARG;
+    }
+    SLA_TRY(*value = sla::dlange_device(*reinterpret_cast<sla::Handle *>(h), c, m, n, dA, lda))
+}
+
+int sla_dgecon(sla_handle_t h, char norm, int n, const double *dLU, int lda, double anorm,
+               double *rcond)
+{
+    SLA_NEED_HANDLE(h);
+    const char c = (char)std::toupper((unsigned char)norm);
+    if (!(c == '1' || c == 'O' || c == 'I')) {
+        g_last_error = "sla_dgecon: NORM must be '1', 'O' or 'I'";
+        return SLA_ERR_ARG;
+    }
+    if (n < 0 || lda < std::max(1, n) || !(anorm >= 0.0) || !rcond) {
+        g_last_error = "sla_dgecon: illegal n, lda, anorm or rcond";
+        return SLA_ERR_ARG;
+    }
+    SLA_TRY(*rcond = sla::dgecon_device(*reinterpret_cast<sla::Handle *>(h), c, n, dLU, lda, anorm))
+}
+
+int sla_dgerfs(sla_handle_t h, char trans, int n, int nrhs, const double *dA, int lda,
+               const double *dAF, int ldaf, const int *dipiv, const double *dB, int ldb, double *dX,
+               int ldx, double *ferr, double *berr)
+{
+    SLA_NEED_HANDLE(h);
+    const bool notran = lsame(trans, 'N');
+    if (!notran && !lsame(trans, 'T') && !lsame(trans, 'C')) {
+        g_last_error = "sla_dgerfs: TRANS must be 'N', 'T' or 'C'";
+        return SLA_ERR_ARG;
+    }
+    const int ld1 = std::max(1, n);
+    if (n < 0 || nrhs < 0 || lda < ld1 || ldaf < ld1 || ldb < ld1 || ldx < ld1 ||
+        (nrhs > 0 && (!ferr || !berr))) {
+        g_last_error = "sla_dgerfs: illegal n, nrhs, leading dimension or output pointer";
+        return SLA_ERR_ARG;
+    }
+    SLA_TRY(sla::dgerfs_device(*reinterpret_cast<sla::Handle *>(h), !notran, n, nrhs, dA, lda, dAF,
+                               ldaf, dipiv, dB, ldb, dX, ldx, ferr, berr))
 }
 
 int sla_dgemm_sub(sla_handle_t h, int m, int n, int k, const double *dA, int lda,

@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <climits>
 #include <cstring>
+#include <string>
 #include <vector>
 
 namespace sla {
@@ -511,6 +512,9 @@ int sgetrf_device(Handle &h, int m, int n, float *dA, int lda, int *dipiv, int s
 // not follow this handle's own factorization).
 void perm_from_ipiv(Handle &h, int n, const int *dipiv)
 {
+    h.perm_key_a = nullptr;
+    h.perm_key_ipiv = nullptr;
+    h.perm_m = 0;
     h.ensure_perm(n);
     iota_int(h.perm, n, 0, h.stream);
     for (int k = 0; k < n; k += 512) {
@@ -520,40 +524,72 @@ void perm_from_ipiv(Handle &h, int n, const int *dipiv)
     }
 }
 
-int dgetrs_device(Handle &h, int n, int nrhs, const double *dA, int lda, const int *dipiv,
-                  double *dB, int ldb)
+void getrs_perm_device(Handle &h, bool trans, int n, int nrhs, const double *dA, int lda,
+                       double *dB, int ldb)
 {
-    if (n <= 0 || nrhs <= 0) return 0;
-    fill_int(h.bad_pivot, 1, 0, h.stream);
-    if (!(h.perm_key_a == dA && h.perm_key_ipiv == dipiv && h.perm_m == n))
-        perm_from_ipiv(h, n, dipiv);
+    if (n <= 0 || nrhs <= 0) return;
     const int chunk = std::min(nrhs, 16);
     h.ensure_work((size_t)n * chunk * sizeof(double));
     double *W = reinterpret_cast<double *>(h.work);
     TrsvWork tw = h.trsv_work();
-    fill_int(h.abort_flag, 1, 0, h.stream);
     for (int j = 0; j < nrhs; j += 16) {
         const int nr = std::min(16, nrhs - j);
         double *Bj = dB + (size_t)j * ldb;
-        h.timing_begin(CAT_LASWP);
-        gather_rows<double>(Bj, ldb, W, n, h.perm, n, nr, h.stream);
-        h.timing_end(16.0 * n * nr);
-        h.timing_begin(CAT_TRSV);
-        trsv_persistent<double, double>(true, n, nr, dA, lda, W, n, tw, h.stream);
-        h.timing_end(4.0 * (double)n * n);
-        h.timing_begin(CAT_TRSV);
-        trsv_persistent<double, double>(false, n, nr, dA, lda, W, n, tw, h.stream);
-        h.timing_end(4.0 * (double)n * n);
-        SLA_HIP(hipMemcpy2DAsync(Bj, (size_t)ldb * sizeof(double), W, (size_t)n * sizeof(double),
-                                 (size_t)n * sizeof(double), nr, hipMemcpyDeviceToDevice, h.stream));
+        if (!trans) {
+            // P B, then L^{-1}, then U^{-1} (pdgetrs.f:255-266)
+            h.timing_begin(CAT_LASWP);
+            gather_rows<double>(Bj, ldb, W, n, h.perm, n, nr, h.stream);
+            h.timing_end(16.0 * n * nr);
+            h.timing_begin(CAT_TRSV);
+            trsv_persistent<double, double>(TrsvOp::L, n, nr, dA, lda, W, n, tw, h.stream);
+            h.timing_end(4.0 * (double)n * n);
+            h.timing_begin(CAT_TRSV);
+            trsv_persistent<double, double>(TrsvOp::U, n, nr, dA, lda, W, n, tw, h.stream);
+            h.timing_end(4.0 * (double)n * n);
+            SLA_HIP(hipMemcpy2DAsync(Bj, (size_t)ldb * sizeof(double), W, (size_t)n * sizeof(double),
+                                     (size_t)n * sizeof(double), nr, hipMemcpyDeviceToDevice,
+                                     h.stream));
+        } else {
+            // U^{-T}, then L^{-T}, then P^T (pdgetrs.f:273-284)
+            SLA_HIP(hipMemcpy2DAsync(W, (size_t)n * sizeof(double), Bj, (size_t)ldb * sizeof(double),
+                                     (size_t)n * sizeof(double), nr, hipMemcpyDeviceToDevice,
+                                     h.stream));
+            h.timing_begin(CAT_TRSV);
+            trsv_persistent<double, double>(TrsvOp::UT, n, nr, dA, lda, W, n, tw, h.stream);
+            h.timing_end(4.0 * (double)n * n);
+            h.timing_begin(CAT_TRSV);
+            trsv_persistent<double, double>(TrsvOp::LT, n, nr, dA, lda, W, n, tw, h.stream);
+            h.timing_end(4.0 * (double)n * n);
+            h.timing_begin(CAT_LASWP);
+            scatter_rows<double>(W, n, Bj, ldb, h.perm, n, nr, h.stream);
+            h.timing_end(16.0 * n * nr);
+        }
     }
     h.trsv_epoch = tw.epoch;
+}
+
+void check_solve_flags(Handle &h, const char *routine)
+{
     int flags[6];
     SLA_HIP(hipMemcpyAsync(flags, h.ctl, sizeof flags, hipMemcpyDeviceToHost, h.stream));
     SLA_HIP(hipStreamSynchronize(h.stream));
     h.timing_collect();
-    if (flags[5]) throw std::runtime_error("dgetrs: IPIV holds a row outside 1..N");
-    if (flags[1]) throw std::runtime_error("dgetrs: a dependency wait timed out (triangular solve)");
+    if (flags[5]) throw std::runtime_error(std::string(routine) + ": IPIV holds a row outside 1..N");
+    if (flags[1])
+        throw std::runtime_error(std::string(routine) +
+                                 ": a dependency wait timed out (triangular solve)");
+}
+
+int dgetrs_device(Handle &h, int n, int nrhs, const double *dA, int lda, const int *dipiv,
+                  double *dB, int ldb, bool trans)
+{
+    if (n <= 0 || nrhs <= 0) return 0;
+    fill_int(h.bad_pivot, 1, 0, h.stream);
+    fill_int(h.abort_flag, 1, 0, h.stream);
+    if (!(h.perm_key_a == dA && h.perm_key_ipiv == dipiv && h.perm_m == n))
+        perm_from_ipiv(h, n, dipiv);
+    getrs_perm_device(h, trans, n, nrhs, dA, lda, dB, ldb);
+    check_solve_flags(h, "dgetrs");
     return 0;
 }
 

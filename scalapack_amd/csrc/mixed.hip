@@ -333,10 +333,10 @@ struct Mixed {
         h.timing_end(12.0 * n * nr);
         TrsvWork tw = h.trsv_work();
         h.timing_begin(CAT_TRSV);
-        trsv_persistent<float, float>(true, n, nr, S, lds, W, n, tw, st());
+        trsv_persistent<float, float>(TrsvOp::L, n, nr, S, lds, W, n, tw, st());
         h.timing_end(2.0 * (double)n * n);
         h.timing_begin(CAT_TRSV);
-        trsv_persistent<float, float>(false, n, nr, S, lds, W, n, tw, st());
+        trsv_persistent<float, float>(TrsvOp::U, n, nr, S, lds, W, n, tw, st());
         h.timing_end(2.0 * (double)n * n);
         h.trsv_epoch = tw.epoch;
     }
@@ -350,10 +350,10 @@ struct Mixed {
         h.timing_end(16.0 * n);
         TrsvWork tw = h.trsv_work();
         h.timing_begin(CAT_TRSV);
-        trsv_persistent<float, double>(true, n, 1, S, lds, z, n, tw, st());
+        trsv_persistent<float, double>(TrsvOp::L, n, 1, S, lds, z, n, tw, st());
         h.timing_end(2.0 * (double)n * n);
         h.timing_begin(CAT_TRSV);
-        trsv_persistent<float, double>(false, n, 1, S, lds, z, n, tw, st());
+        trsv_persistent<float, double>(TrsvOp::U, n, 1, S, lds, z, n, tw, st());
         h.timing_end(2.0 * (double)n * n);
         h.trsv_epoch = tw.epoch;
     }
@@ -520,8 +520,8 @@ void sgetrs_device(Handle &h, int n, int nrhs, const float *dA, int lda, const i
         const int nr = std::min(16, nrhs - j);
         float *Bj = dB + (size_t)j * ldb;
         gather_rows<float>(Bj, ldb, W, n, h.perm, n, nr, h.stream);
-        trsv_persistent<float, float>(true, n, nr, dA, lda, W, n, tw, h.stream);
-        trsv_persistent<float, float>(false, n, nr, dA, lda, W, n, tw, h.stream);
+        trsv_persistent<float, float>(TrsvOp::L, n, nr, dA, lda, W, n, tw, h.stream);
+        trsv_persistent<float, float>(TrsvOp::U, n, nr, dA, lda, W, n, tw, h.stream);
         SLA_HIP(hipMemcpy2DAsync(Bj, (size_t)ldb * 4, W, (size_t)n * 4, (size_t)n * 4, nr,
                                  hipMemcpyDeviceToDevice, h.stream));
     }

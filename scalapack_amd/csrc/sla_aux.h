@@ -1,0 +1,25 @@
+// sla_aux.h -- norms, condition estimate and iterative refinement with the
+// factors of PDGETRF (aux.hip), on device-resident data.
+#pragma once
+
+#include "sla_handle.h"
+
+namespace sla {
+
+// PDLANGE: 'M' max |a|, '1'/'O' one norm, 'I' infinity norm, 'F'/'E'
+// Frobenius norm of the m x n matrix at dA.
+double dlange_device(Handle &h, char norm, int m, int n, const double *dA, int lda);
+
+// PDGECON: reciprocal condition number estimate in the one ('1'/'O') or
+// infinity ('I') norm from the L\U factors at dLU and the norm of A.
+double dgecon_device(Handle &h, char norm, int n, const double *dLU, int lda, double anorm);
+
+// PDGERFS: refines the nrhs solutions at dX of op(A) X = B (op = transpose
+// when trans) with the factors dAF / dipiv of A; writes the forward error
+// bounds ferr[k] (left unchanged when x_k is zero) and the componentwise
+// backward errors berr[k] (host arrays).
+void dgerfs_device(Handle &h, bool trans, int n, int nrhs, const double *dA, int lda,
+                   const double *dAF, int ldaf, const int *dipiv, const double *dB, int ldb,
+                   double *dX, int ldx, double *ferr, double *berr);
+
+}  // namespace sla

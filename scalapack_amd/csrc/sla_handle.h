@@ -55,6 +55,7 @@ enum BufId {
     BUF_KRYLOV,      // GMRES bases
     BUF_PARTIAL,     // partial sums of the residual / reductions
     BUF_TSLU,        // candidate exchange of the tournament-pivoting leaf
+    BUF_AUX,         // vectors and partial sums of the norms / estimators / PDGERFS
     BUF_COUNT
 };
 
@@ -124,6 +125,8 @@ struct Handle {
     int update_cus = 0;    // > 0: the trailing update runs on at most this many CUs' worth
                            // of workgroups (the rest stay free for the panel)
     int num_cus = 256;     // CUs of the device
+    int lacon_lapack = 0;  // condition / error estimates: 0 the reference's PDLACON result,
+                           // 1 LAPACK DLACON's iteration (estimate kept across steps)
     // mixed-precision refinement
     int ir_max_iter = 30;        // refinement steps before the FP64 fallback (DSGESV's ITERMAX)
     int ir_gmres_on_stall = 1;   // classical refinement that stalls continues as GMRES-IR
@@ -163,10 +166,17 @@ int dgetrf_device(Handle &h, int m, int n, double *dA, int lda, int *dipiv);
 // The same on FP32 storage; `schur` (enum Schur) selects the precision of the
 // outer trailing update.
 int sgetrf_device(Handle &h, int m, int n, float *dA, int lda, int *dipiv, int schur);
+// trans: solve A^T X = B (PDGETRS TRANS = 'T' or 'C') instead of A X = B.
 int dgetrs_device(Handle &h, int n, int nrhs, const double *dA, int lda, const int *dipiv,
-                  double *dB, int ldb);
-// The factorization's net row permutation rebuilt from IPIV alone.
+                  double *dB, int ldb, bool trans = false);
+// The factorization's net row permutation rebuilt from IPIV alone (h.perm;
+// the cached association with a factorization is dropped).
 void perm_from_ipiv(Handle &h, int n, const int *dipiv);
+// Solve with the permutation already in h.perm (no wait for completion;
+// check_solve_flags reports what the kernels flagged).
+void getrs_perm_device(Handle &h, bool trans, int n, int nrhs, const double *dA, int lda,
+                       double *dB, int ldb);
+void check_solve_flags(Handle &h, const char *routine);
 // PDLASWP on a device matrix: row interchanges k1 .. k2-1 (0-based) with
 // rows ipiv[k]-1 over ncols columns, forward or backward; and the column form.
 void laswp_rows_device(Handle &h, double *dA, int lda, int ncols, int k1, int k2,

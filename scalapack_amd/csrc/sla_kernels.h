@@ -122,8 +122,10 @@ void laswp_cols(double *A, int lda, int nrows, int k1, int k2, const int *ipiv, 
 // (PDTRSM 'L','L','N','U').
 template <typename T>
 void trsm_llnu(int m, int n, const T *L, int ldl, T *B, int ldb, hipStream_t s);
-// One launch per triangle for n x nrhs right-hand sides (nrhs <= 16):
-// forward = X <- L^{-1} X (unit lower), else X <- U^{-1} X (upper, non-unit).
+// One launch per triangle for n x nrhs right-hand sides (nrhs <= 16), with L
+// (unit lower) and U (upper, non-unit) stored together as PDGETRF leaves them:
+// X <- L^{-1} X, U^{-1} X, U^{-T} X or L^{-T} X.
+enum class TrsvOp { L, U, UT, LT };
 struct TrsvWork {
     int *ticket;         // workgroup ticket counter (reset before each launch)
     int *flags;          // per 64-row block: == epoch once that block is solved
@@ -135,7 +137,7 @@ struct TrsvWork {
 // TA: storage type of the triangle, T: type of X and of the arithmetic
 // (<double, double>, <float, float>, <float, double>).
 template <typename TA, typename T>
-void trsv_persistent(bool forward, int n, int nrhs, const TA *A, int lda, T *X, int ldx,
+void trsv_persistent(TrsvOp op, int n, int nrhs, const TA *A, int lda, T *X, int ldx,
                      TrsvWork &tw, hipStream_t s);
 
 // ---- small helpers ---------------------------------------------------------
@@ -146,5 +148,9 @@ void iota_int(int *p, int n, int base, hipStream_t s);
 template <typename T>
 void gather_rows(const T *src, int lds, T *dst, int ldd, const int *perm, int n, int nrhs,
                  hipStream_t s);
+// B[perm[i], :] = Bsrc[i, :] (row scatter, the inverse permutation)
+template <typename T>
+void scatter_rows(const T *src, int lds, T *dst, int ldd, const int *perm, int n, int nrhs,
+                  hipStream_t s);
 
 }  // namespace sla

@@ -161,6 +161,16 @@ __global__ void gather_rows_kernel(const T *__restrict__ src, int lds, T *__rest
     for (int j = 0; j < nrhs; j++) dst[i + (size_t)j * ldd] = src[p + (size_t)j * lds];
 }
 
+template <typename T>
+__global__ void scatter_rows_kernel(const T *__restrict__ src, int lds, T *__restrict__ dst,
+                                    int ldd, const int *__restrict__ perm, int n, int nrhs)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int p = perm[i];
+    for (int j = 0; j < nrhs; j++) dst[p + (size_t)j * ldd] = src[i + (size_t)j * lds];
+}
+
 }  // namespace
 
 void laswp_pairs(const int *ipiv, int k1, int k2, int nrows, int *pairs, int *count, int *bad,
@@ -248,5 +258,16 @@ void gather_rows(const T *src, int lds, T *dst, int ldd, const int *perm, int n,
 }
 template void gather_rows<double>(const double *, int, double *, int, const int *, int, int, hipStream_t);
 template void gather_rows<float>(const float *, int, float *, int, const int *, int, int, hipStream_t);
+
+template <typename T>
+void scatter_rows(const T *src, int lds, T *dst, int ldd, const int *perm, int n, int nrhs,
+                  hipStream_t s)
+{
+    if (n <= 0 || nrhs <= 0) return;
+    hipLaunchKernelGGL(scatter_rows_kernel<T>, dim3(ceil_div(n, 256)), dim3(256), 0, s, src, lds,
+                       dst, ldd, perm, n, nrhs);
+    SLA_LAUNCH_CHECK();
+}
+template void scatter_rows<double>(const double *, int, double *, int, const int *, int, int, hipStream_t);
 
 }  // namespace sla

@@ -7,14 +7,18 @@
 //   it in LDS, and walks L11 in 32-column blocks staged through LDS: solve the
 //   32 x 32 diagonal block, then update the rows below with one fma chain.
 //
-// trsv_persistent: the two solves of PDGETRS (SRC/pdgetrs.f:260-266) for a
-//   few right-hand sides.  Those are latency-bound, so one launch covers the
-//   whole triangle: block row b (64 rows) is owned by the workgroup that
-//   takes ticket b; it accumulates -L[b, c] X[c] for every finished block c
-//   (a device-wide flag per block, bounded wait), then solves its 64 x 64
-//   diagonal block in registers of one wave (lane = row, values broadcast by
-//   lane reads) and publishes X[b].  Tickets are taken in start order, so a
-//   workgroup only ever waits on workgroups that are already running.
+// trsv_persistent: the two solves of PDGETRS (SRC/pdgetrs.f:260-266 for
+//   TRANS = 'N', :273-284 for 'T') and of PDLATRS inside PDGECON
+//   (SRC/pdgecon.f:342-370) for a few right-hand sides.  Those are
+//   latency-bound, so one launch covers the whole triangle: block row b
+//   (64 rows) is owned by the workgroup that takes ticket b; it accumulates
+//   -T[b, c] X[c] for every finished block c (a device-wide flag per block,
+//   bounded wait), then solves its 64 x 64 diagonal block in registers of one
+//   wave (lane = row, values broadcast by lane reads) and publishes X[b].
+//   Tickets are taken in start order, so a workgroup only ever waits on
+//   workgroups that are already running.  The transposed solves read the
+//   off-diagonal block by columns (lane = column of the stored triangle,
+//   16 consecutive elements of that column per wave).
 #include "sla_common.h"
 #include "sla_kernels.h"
 
@@ -65,13 +69,15 @@ __global__ __launch_bounds__(256) void trsm_llnu_strip_kernel(int m, int n, cons
 // ---------------------------------------------------------------------------
 constexpr int TV_BS = 64, TV_NR = 16;
 
-// FORWARD: X = L^{-1} X with L unit lower.  Otherwise X = U^{-1} X with U
-// upper, non-unit, scaled by the reciprocals of its diagonal, computed before
-// any wait so they stay off the dependency chain.  X is n x nrhs
+// The stored matrix holds L (unit lower, below the diagonal) and U (upper).
+//   !TRANS,  FORWARD: X = L^{-1} X        !TRANS, !FORWARD: X = U^{-1} X
+//    TRANS,  FORWARD: X = U^{-T} X         TRANS, !FORWARD: X = L^{-T} X
+// A non-unit diagonal is applied as a product with its reciprocals, computed
+// before any wait so they stay off the dependency chain.  X is n x nrhs
 // (nrhs <= NR <= 16) with leading dimension ldx; the matrix is n x n.
 // TA: storage type of the triangle; T: type of X and of the arithmetic (a
 // FP32 factorization may be applied in FP64, as GMRES-IR's preconditioner).
-template <typename TA, typename T, bool FORWARD, int NR>
+template <typename TA, typename T, bool FORWARD, bool TRANS, int NR>
 __global__ __launch_bounds__(256) void trsv_persistent_kernel(
     int n, int nrhs, const TA *__restrict__ A, int lda, T *X, int ldx, int *ticket, int *flags,
     int epoch, int *abort_flag, uint64_t spin_limit)
@@ -96,8 +102,9 @@ __global__ __launch_bounds__(256) void trsv_persistent_kernel(
         const int r = idx % TV_BS, k = idx / TV_BS;
         Ds[r][k] = (r < rows && k < rows) ? (T)A[(size_t)(rb + r) + (size_t)(rb + k) * lda] : T(0);
     }
+    constexpr bool UNIT = (FORWARD != TRANS);  // L and L^T have a unit diagonal
     T rdiag = T(1);
-    if (!FORWARD && wave == 0 && lane < rows) rdiag = T(1) / (T)A[(size_t)(rb + lane) * (lda + 1)];
+    if (!UNIT && wave == 0 && lane < rows) rdiag = T(1) / (T)A[(size_t)(rb + lane) * (lda + 1)];
 
     T part[NR];
 #pragma unroll
@@ -134,13 +141,16 @@ __global__ __launch_bounds__(256) void trsv_persistent_kernel(
         }
         __syncthreads();
         // wave w covers k in [16w, 16w+16) of the 64 x 64 block A[rb.., cb..]
+        // (TRANS: of the block A[cb.., rb..], read down its columns)
         if (lane < rows) {
-            const TA *Ar = A + (size_t)(rb + lane) + (size_t)cb * lda;
+            const TA *Ar = TRANS ? A + (size_t)cb + (size_t)(rb + lane) * lda
+                                 : A + (size_t)(rb + lane) + (size_t)cb * lda;
+            const size_t kstride = TRANS ? 1 : (size_t)lda;
 #pragma unroll 4
             for (int kk = 0; kk < 16; kk++) {
                 const int k = wave * 16 + kk;
                 if (k < crows) {
-                    const T a = (T)Ar[(size_t)k * lda];
+                    const T a = (T)Ar[(size_t)k * kstride];
 #pragma unroll
                     for (int j = 0; j < NR; j++) part[j] = fma(a, Xs[k][j], part[j]);
                 }
@@ -160,21 +170,24 @@ __global__ __launch_bounds__(256) void trsv_persistent_kernel(
             const T s = ((Part[0][lane][j] + Part[1][lane][j]) + Part[2][lane][j]) + Part[3][lane][j];
             x[j] = rhs - s;
         }
+        // Column i of the triangle applied to x: element (lane, i) of the
+        // diagonal block, or (i, lane) for the transposed solves.
         if (FORWARD) {
             for (int i = 0; i < rows; i++) {
-                const T l = Ds[lane][i];
+                const T l = TRANS ? Ds[i][lane] : Ds[lane][i];
 #pragma unroll
                 for (int j = 0; j < NR; j++) {
+                    if (!UNIT && lane == i) x[j] = x[j] * rdiag;
                     const T xi = lane_read(x[j], i);
                     if (lane > i) x[j] = fma(-l, xi, x[j]);
                 }
             }
         } else {
             for (int i = rows - 1; i >= 0; i--) {
-                const T u = Ds[lane][i];
+                const T u = TRANS ? Ds[i][lane] : Ds[lane][i];
 #pragma unroll
                 for (int j = 0; j < NR; j++) {
-                    if (lane == i) x[j] = x[j] * rdiag;
+                    if (!UNIT && lane == i) x[j] = x[j] * rdiag;
                     const T xi = lane_read(x[j], i);
                     if (lane < i) x[j] = fma(-u, xi, x[j]);
                 }
@@ -210,7 +223,7 @@ template void trsm_llnu<double>(int, int, const double *, int, double *, int, hi
 template void trsm_llnu<float>(int, int, const float *, int, float *, int, hipStream_t);
 
 template <typename TA, typename T>
-void trsv_persistent(bool forward, int n, int nrhs, const TA *A, int lda, T *X, int ldx,
+void trsv_persistent(TrsvOp op, int n, int nrhs, const TA *A, int lda, T *X, int ldx,
                      TrsvWork &tw, hipStream_t s)
 {
     if (n <= 0 || nrhs <= 0) return;
@@ -219,29 +232,33 @@ void trsv_persistent(bool forward, int n, int nrhs, const TA *A, int lda, T *X, 
     if (nblk > tw.max_blocks) throw std::logic_error("trsv_persistent: flag array too small");
     fill_int(tw.ticket, 1, 0, s);
     const int epoch = ++tw.epoch;
-#define SLA_TRSV(FW, NRV)                                                                       \
-    hipLaunchKernelGGL((trsv_persistent_kernel<TA, T, FW, NRV>), dim3(nblk), dim3(256), 0, s, n, \
-                       nrhs, A, lda, X, ldx, tw.ticket, tw.flags, epoch, tw.abort_flag,        \
+#define SLA_TRSV(FW, TR, NRV)                                                                  \
+    hipLaunchKernelGGL((trsv_persistent_kernel<TA, T, FW, TR, NRV>), dim3(nblk), dim3(256), 0, \
+                       s, n, nrhs, A, lda, X, ldx, tw.ticket, tw.flags, epoch, tw.abort_flag, \
                        tw.spin_limit)
-#define SLA_TRSV_NR(FW)                           \
-    do {                                          \
-        if (nrhs <= 1) SLA_TRSV(FW, 1);           \
-        else if (nrhs <= 2) SLA_TRSV(FW, 2);      \
-        else if (nrhs <= 4) SLA_TRSV(FW, 4);      \
-        else if (nrhs <= 8) SLA_TRSV(FW, 8);      \
-        else SLA_TRSV(FW, 16);                    \
+#define SLA_TRSV_NR(FW, TR)                           \
+    do {                                              \
+        if (nrhs <= 1) SLA_TRSV(FW, TR, 1);           \
+        else if (nrhs <= 2) SLA_TRSV(FW, TR, 2);      \
+        else if (nrhs <= 4) SLA_TRSV(FW, TR, 4);      \
+        else if (nrhs <= 8) SLA_TRSV(FW, TR, 8);      \
+        else SLA_TRSV(FW, TR, 16);                    \
     } while (0)
-    if (forward) SLA_TRSV_NR(true);
-    else SLA_TRSV_NR(false);
+    switch (op) {
+    case TrsvOp::L: SLA_TRSV_NR(true, false); break;
+    case TrsvOp::U: SLA_TRSV_NR(false, false); break;
+    case TrsvOp::UT: SLA_TRSV_NR(true, true); break;
+    case TrsvOp::LT: SLA_TRSV_NR(false, true); break;
+    }
 #undef SLA_TRSV_NR
 #undef SLA_TRSV
     SLA_LAUNCH_CHECK();
 }
-template void trsv_persistent<double, double>(bool, int, int, const double *, int, double *, int,
+template void trsv_persistent<double, double>(TrsvOp, int, int, const double *, int, double *, int,
                                               TrsvWork &, hipStream_t);
-template void trsv_persistent<float, float>(bool, int, int, const float *, int, float *, int,
+template void trsv_persistent<float, float>(TrsvOp, int, int, const float *, int, float *, int,
                                             TrsvWork &, hipStream_t);
-template void trsv_persistent<float, double>(bool, int, int, const float *, int, double *, int,
+template void trsv_persistent<float, double>(TrsvOp, int, int, const float *, int, double *, int,
                                              TrsvWork &, hipStream_t);
 
 }  // namespace sla

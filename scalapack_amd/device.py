@@ -112,6 +112,35 @@ class Handle:
                                    ctypes.byref(info)))
         return info.value
 
+    def dgetrs_trans(self, trans: str, n: int, nrhs: int, a, lda: int, ipiv, b, ldb: int) -> None:
+        """op(A) X = B with the factors of dgetrf; trans 'N', 'T' or 'C'."""
+        _check(self._lib.sla_dgetrs_trans(self._h, trans.encode()[:1], n, nrhs, _ptr(a), lda,
+                                          _ptr(ipiv), _ptr(b), ldb))
+
+    def dlange(self, norm: str, m: int, n: int, a, lda: int) -> float:
+        """PDLANGE: 'M', '1'/'O', 'I' or 'F'/'E' norm of an m x n matrix."""
+        out = ctypes.c_double()
+        _check(self._lib.sla_dlange(self._h, norm.encode()[:1], m, n, _ptr(a), lda,
+                                    ctypes.byref(out)))
+        return out.value
+
+    def dgecon(self, norm: str, n: int, lu, lda: int, anorm: float) -> float:
+        """PDGECON: reciprocal condition estimate from the factors of dgetrf."""
+        out = ctypes.c_double()
+        _check(self._lib.sla_dgecon(self._h, norm.encode()[:1], n, _ptr(lu), lda, float(anorm),
+                                    ctypes.byref(out)))
+        return out.value
+
+    def dgerfs(self, trans: str, n: int, nrhs: int, a, lda: int, af, ldaf: int, ipiv, b,
+               ldb: int, x, ldx: int, ferr=None) -> tuple[list[float], list[float]]:
+        """PDGERFS: refines x in place; returns (ferr, berr) per column.  ferr
+        keeps the given start values where x_k is zero."""
+        fe = (ctypes.c_double * max(1, nrhs))(*(ferr or [0.0] * nrhs))
+        be = (ctypes.c_double * max(1, nrhs))()
+        _check(self._lib.sla_dgerfs(self._h, trans.encode()[:1], n, nrhs, _ptr(a), lda, _ptr(af),
+                                    ldaf, _ptr(ipiv), _ptr(b), ldb, _ptr(x), ldx, fe, be))
+        return list(fe[:nrhs]), list(be[:nrhs])
+
     # ------------------------------------------------------- mixed precision
     def dxgesv(self, n: int, nrhs: int, a, lda: int, ipiv, b, ldb: int, x, ldx: int,
                prec: str = "fp32", refine: str = "ir") -> tuple[int, int]:

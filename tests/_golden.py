@@ -51,3 +51,21 @@ def example_6x6():
 
 def argcheck_cases() -> list[dict]:
     return json.loads((HERE / "argcheck_cases.json").read_text())
+
+
+def aux_cases() -> list[tuple[int, int, int]]:
+    """(N, NB, NRHS) of the stored cases of the routines beside the solve."""
+    z = np.load(HERE / "aux_cases.npz")
+    out = set()
+    for k in z.files:
+        p = k.split("_")
+        out.add((int(p[0][1:]), int(p[1][2:]), int(p[2][4:])))
+    return sorted(out)
+
+
+def aux(N: int, NB: int, NRHS: int) -> dict:
+    """Outputs of the reference's PDLANGE / PDGECON / PDGETRS('T') / PDGERFS
+    for A = PDMATGEN(N, N, seed 100), B = PDMATGEN(N, NRHS, seed 200)."""
+    z = np.load(HERE / "aux_cases.npz")
+    pre = f"N{N}_NB{NB}_NRHS{NRHS}_"
+    return {k[len(pre):]: z[k] for k in z.files if k.startswith(pre)}

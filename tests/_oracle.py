@@ -33,6 +33,14 @@ def lib() -> ctypes.CDLL:
         _lib.oracle_norm_inf.restype = ctypes.c_double
         _lib.oracle_lcg_state.restype = ctypes.c_uint
         _lib.oracle_lcg_state.argtypes = [ctypes.c_int, ctypes.c_longlong]
+        _lib.oracle_pdlange.restype = ctypes.c_double
+        _lib.oracle_pdlange.argtypes = [ctypes.c_char, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_void_p]
+        _lib.oracle_pdgecon.restype = ctypes.c_double
+        _lib.oracle_pdgecon.argtypes = [ctypes.c_char, ctypes.c_int, ctypes.c_void_p,
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                        ctypes.c_double]
     return _lib
 
 
@@ -107,3 +115,43 @@ def fresid(a0: np.ndarray, lu: np.ndarray, ipiv: np.ndarray) -> float:
     lu = np.asfortranarray(lu)
     return lib().oracle_fresid(m, n, _p(a0), m, _p(lu), lu.shape[0],
                                _p(np.ascontiguousarray(ipiv, np.int32)))
+
+
+def pdlange(norm: str, a: np.ndarray, m=None, n=None, ia=1, ja=1) -> float:
+    """PDLANGE of a(ia:ia+m-1, ja:ja+n-1)."""
+    a = np.asfortranarray(a)
+    M, N = a.shape
+    m = M - ia + 1 if m is None else m
+    n = N - ja + 1 if n is None else n
+    return lib().oracle_pdlange(norm.encode()[:1], m, n, _p(a), ia, ja, _p(desc(M, N, 1, M)))
+
+
+def pdgecon(norm: str, lu: np.ndarray, anorm: float) -> float:
+    """PDGECON's RCOND from L\\U factors (IA = JA = 1)."""
+    lu = np.asfortranarray(lu)
+    n = lu.shape[0]
+    return lib().oracle_pdgecon(norm.encode()[:1], n, _p(lu), 1, 1, _p(desc(n, n, 1, n)), anorm)
+
+
+def pdgetrs_t(lu: np.ndarray, ipiv: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """A^T X = B with the oracle's factors (PDGETRS TRANS = 'T')."""
+    n = lu.shape[0]
+    x = np.asfortranarray(b.copy())
+    nrhs = x.shape[1]
+    lib().oracle_pdgetrs_t(n, nrhs, _p(np.asfortranarray(lu)), 1, 1, _p(desc(n, n, 1, n)),
+                           _p(np.ascontiguousarray(ipiv, np.int32)), _p(x), 1, 1,
+                           _p(desc(n, nrhs, 1, n)))
+    return x
+
+
+def pdgerfs(trans: str, a: np.ndarray, lu: np.ndarray, ipiv: np.ndarray, b: np.ndarray,
+            x0: np.ndarray):
+    """PDGERFS from the start x0; returns (x, ferr, berr)."""
+    a, lu, b = (np.asfortranarray(v) for v in (a, lu, b))
+    x = np.asfortranarray(x0.copy())
+    n, nrhs = b.shape
+    ferr, berr = np.zeros(nrhs), np.zeros(nrhs)
+    lib().oracle_pdgerfs(ctypes.c_char(trans.encode()[:1]), n, nrhs, _p(a), n, _p(lu), n,
+                         _p(np.ascontiguousarray(ipiv, np.int32)), _p(b), n, _p(x), n,
+                         _p(ferr), _p(berr))
+    return x, ferr, berr

@@ -12,7 +12,11 @@ IPIV, INFO, the solution X and the tester's scaled residual.  Cases:
   * config 2: N = 16384, NB = 256, NRHS = 16, B i.i.d. standard normal from
     numpy.random.default_rng(200) (written column-major), grid 1x1;
   * the 6 x 6 example of EXAMPLE/DSCAEXMAT.dat / DSCAEXRHS.dat with its exact
-    rational solution.
+    rational solution;
+  * the routines beside the solve (ref_lu aux): PDLANGE 'M' '1' 'I' 'F' of A
+    and of A(2:N-2, 3:N), PDGECON '1' and 'I', PDGETRS 'T', and PDGERFS 'N'
+    from the PDGETRS solution, 'N' and 'T' from perturbed starts, for
+    (N, NB, NRHS) in AUX_CASES with A = PDMATGEN(100), B = PDMATGEN(200).
 
 Usage (in the build container, where /root/reference exists):
     make -C oracle ref && python tests/golden/gen_golden.py [--skip-c2]
@@ -40,6 +44,7 @@ LU_DAT_MN = [(4, 4), (10, 12), (17, 13), (13, 13)]
 LU_DAT_NB = [2, 3, 4]
 LU_DAT_NRHS = [1, 3, 9]
 LU_DAT_GRIDS = [(1, 1), (2, 2), (1, 4), (4, 1)]
+AUX_CASES = [(37, 4, 2), (200, 32, 3), (513, 64, 1)]
 
 
 def read_out(path: Path) -> dict:
@@ -71,6 +76,46 @@ def run_ref(M, N, NB, NRHS, P, Q, bmode, threads: int = 1) -> dict:
             cmd = [MPIEXEC, "-n", str(P * Q)] + cmd
         subprocess.run(cmd, check=True, env=env, capture_output=True)
         return read_out(out)
+
+
+def run_aux(N, NB, NRHS) -> dict:
+    """ref_lu aux: see oracle/ref_driver.c aux() for the file layout."""
+    with tempfile.TemporaryDirectory() as td:
+        out = Path(td) / "aux.bin"
+        env = dict(os.environ, MKL_THREADING_LAYER="SEQUENTIAL", MKL_NUM_THREADS="1")
+        subprocess.run([str(REF_LU), "aux", str(N), str(NB), str(NRHS), "100", "200", str(out)],
+                       check=True, capture_output=True, env=env)
+        b = out.read_bytes()
+    hdr = np.frombuffer(b[:44], np.int32)
+    assert hdr[0] == 0x41555852 and tuple(hdr[1:4]) == (N, NB, NRHS)
+    off = 44
+    dh = np.frombuffer(b[off:off + 80], np.float64)
+    off += 80
+    ipiv = np.frombuffer(b[off:off + 4 * N], np.int32).copy()
+    off += 4 * N
+    nr = N * NRHS
+
+    def mat():
+        nonlocal off
+        m = np.frombuffer(b[off:off + 8 * nr], np.float64).reshape(NRHS, N).T.copy(order="F")
+        off += 8 * nr
+        return m
+
+    def vec():
+        nonlocal off
+        v = np.frombuffer(b[off:off + 8 * NRHS], np.float64).copy()
+        off += 8 * NRHS
+        return v
+
+    r = dict(info=hdr[4:11].copy(), norms=dh[0:4].copy(), subnorms=dh[4:8].copy(),
+             rcond=dh[8:10].copy(), ipiv=ipiv, xT=mat())
+    for c in range(3):
+        r[f"rfs{c}_x0"] = mat()
+        r[f"rfs{c}_x"] = mat()
+        r[f"rfs{c}_ferr"] = vec()
+        r[f"rfs{c}_berr"] = vec()
+    assert off == len(b)
+    return r
 
 
 def read_dat(path: Path) -> np.ndarray:
@@ -169,9 +214,19 @@ def run_argcheck(case: dict) -> int:
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--skip-c2", action="store_true")
+    ap.add_argument("--only-aux", action="store_true",
+                    help="write only aux_cases.npz (the routines beside the solve)")
     args = ap.parse_args()
     if not REF_LU.exists():
         sys.exit(f"{REF_LU} missing: run `make -C oracle ref` first")
+    if args.only_aux:
+        aux = {}
+        for (N, NB, NRHS) in AUX_CASES:
+            for k, v in run_aux(N, NB, NRHS).items():
+                aux[f"N{N}_NB{NB}_NRHS{NRHS}_{k}"] = v
+        np.savez_compressed(HERE / "aux_cases.npz", **aux)
+        print("aux:", AUX_CASES)
+        return
 
     # LU.dat cases
     lu = {}
@@ -207,6 +262,15 @@ def main() -> None:
         c1[k + "_sresid"] = np.array([r["sresid"]])
     np.savez_compressed(HERE / "c1_n1024_nb64.npz", **c1)
     print("c1: ipiv 1x1 == 2x2:", np.array_equal(c1["P1_Q1_ipiv"], c1["P2_Q2_ipiv"]))
+
+    # routines beside the solve
+    aux = {}
+    for (N, NB, NRHS) in AUX_CASES:
+        r = run_aux(N, NB, NRHS)
+        for k, v in r.items():
+            aux[f"N{N}_NB{NB}_NRHS{NRHS}_{k}"] = v
+    np.savez_compressed(HERE / "aux_cases.npz", **aux)
+    print("aux:", AUX_CASES)
 
     # 6 x 6 example
     a6 = read_dat(REFERENCE / "EXAMPLE" / "DSCAEXMAT.dat")

@@ -26,7 +26,8 @@ def test_library_loads_and_exports_declared_symbols():
 
 def test_reference_fortran_entry_points_are_declared():
     decl = declared_symbols()["scalapack_amd.h"]
-    for name in ("pdgesv_", "pdgetrf_", "pdgetrs_", "pdgetf2_", "pdlaswp_"):
+    for name in ("pdgesv_", "pdgetrf_", "pdgetrs_", "pdgetf2_", "pdlaswp_", "pdlange_",
+                 "pdgecon_", "pdgerfs_"):
         assert name in decl
 
 
@@ -113,13 +114,53 @@ def test_illegal_argument_info_matches_reference(golden, capfd):
     assert "On entry to PDGETRF parameter number  606 had an illegal value" in out
 
 
-def test_pdgetrs_transpose_is_reported():
+def test_pdgetrs_illegal_trans_is_reported(capfd):
+    """TRANS other than N / T / C is argument 1 (pdgetrs.f:208-210); 'T'
+    and 'C' pass the checks and run on the GPU (tests/test_gpu_aux.py)."""
     ctx = api.blacs_gridinit(1, 1)
     d, _ = api.descinit(4, 4, 2, 2, 0, 0, ctx, 4)
     db, _ = api.descinit(4, 1, 2, 2, 0, 0, ctx, 4)
     a = np.asfortranarray(np.eye(4))
     b = np.asfortranarray(np.ones((4, 1)))
-    assert api.pdgetrs("T", 4, 1, a, 1, 1, d, np.arange(1, 5, dtype=np.int32), b, 1, 1, db) == -1
+    assert api.pdgetrs("X", 4, 1, a, 1, 1, d, np.arange(1, 5, dtype=np.int32), b, 1, 1, db) == -1
+    assert "On entry to PDGETRS parameter number    1 had an illegal value" in capfd.readouterr().out
+
+
+def test_aux_entry_argument_checks(capfd):
+    """PDGECON / PDGERFS argument checks and workspace queries return before
+    any GPU work (pdgecon.f:215-285, pdgerfs.f:308-404)."""
+    ctx = api.blacs_gridinit(1, 1)
+    n, nb = 8, 4
+    d, _ = api.descinit(n, n, nb, nb, 0, 0, ctx, n)
+    db, _ = api.descinit(n, 2, nb, nb, 0, 0, ctx, n)
+    a = np.asfortranarray(np.eye(n))
+    work, iwork = np.zeros(64), np.zeros(16, np.int32)
+    rc, info = api.pdgecon("1", n, a, 1, 1, d, 1.0, work, -1, iwork, 1)
+    assert info == 0 and work[0] == 5 * n + nb and iwork[0] == n   # LWMIN, LIWMIN on 1 x 1
+    assert api.pdgecon("Q", n, a, 1, 1, d, 1.0, work, 64, iwork, 16)[1] == -1
+    assert api.pdgecon("I", n, a, 1, 1, d, -2.0, work, 64, iwork, 16)[1] == -7
+    assert api.pdgecon("I", n, a, 1, 1, d, 1.0, work, 5 * n + nb - 1, iwork, 16)[1] == -10
+    assert api.pdgecon("I", n, a, 1, 1, d, 1.0, work, 64, iwork, n - 1)[1] == -12
+    assert api.pdgecon("I", n, a, 2, 1, d, 1.0, work, 64, iwork, 16)[1] == -2  # N+IA-1 > M_A (CHK1MAT)
+    ipiv = np.arange(1, n + 1, dtype=np.int32)
+    b = np.asfortranarray(np.ones((n, 2)))
+    x = b.copy(order="F")
+    fe, be = np.zeros(2), np.zeros(2)
+
+    def rfs(trans="N", lwork=3 * n, liwork=n, ib=1, descx=db, ia=1):
+        return api.pdgerfs(trans, n if ia == 1 else n - nb, 2, a, ia, 1, d, a, ia, 1, d, ipiv, b,
+                           ib, 1, db, x, 1, 1, descx, fe, be, work, lwork, iwork, liwork)
+    assert rfs(lwork=-1) == 0 and work[0] == 3 * n and iwork[0] == n
+    assert rfs("Z") == -1
+    assert rfs(lwork=3 * n - 1) == -24
+    assert rfs(liwork=n - 1) == -26
+    bad_x = db.copy()
+    bad_x[4] = nb + 1
+    assert rfs(descx=bad_x) == -(2000 + 5)     # MB_B != MB_X
+    assert rfs(ia=3) == -5                     # IA not aligned to MB_A
+    out = capfd.readouterr().out
+    assert "On entry to PDGECON parameter number    1 had an illegal value" in out
+    assert "On entry to PDGERFS parameter number   24 had an illegal value" in out
 
 
 def test_python_mirror_rejects_c_ordered_arrays():

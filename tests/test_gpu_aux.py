@@ -1,0 +1,307 @@
+"""GPU parity of the routines beside the solve, through the C-ABI: PDLANGE,
+PDGECON (PDLACON), PDGETRS with TRANS = 'T' and PDGERFS.
+
+Against the reference's golden fixtures (tests/golden/aux_cases.npz) and
+against the oracle at other shapes.  Tolerances: the max, one and infinity
+norms are taken in the reference's order and must agree bit for bit; the
+Frobenius norm scales by max |a| instead of DLASSQ's running scale (1e-14
+relative).  RCOND: 1e-10 relative (one triangular solve pair on the
+alternating-sign vector, rounding of two different solve orders).  The 'T'
+solve: 1e-12 of max |x| (kappa ~ 1e3 matrices).  PDGERFS: refined X within
+1e-13 of max |x|, BERR within a factor 2 and below 4 eps, FERR within
+8 / (n+1) relative: its weights are |r| + (n+1) eps w with |r| <= BERR w
+<= 4 eps w, and r is rounding-level, so two correct residuals move FERR by
+at most 2 * 4 eps / ((n+1) eps).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+EPS = 2.0 ** -53
+
+
+def _note(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def dev():
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    from scalapack_amd.device import Handle
+    h = Handle()
+    yield h
+    h.close()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from scalapack_amd import api
+    return api.blacs_gridinit(1, 1)
+
+
+def to_dev(a: np.ndarray, ld: int | None = None):
+    """Column-major numpy matrix -> device tensor (storage = columns), with
+    leading dimension ld >= rows (rows beyond m are filled with NaN so that
+    a read past the matrix shows)."""
+    a = np.asfortranarray(a)
+    m, n = a.shape
+    ld = m if ld is None else ld
+    buf = np.full((n, ld), np.nan)
+    buf[:, :m] = a.T
+    return torch.from_numpy(buf).cuda(), ld
+
+
+def from_dev(t, m: int) -> np.ndarray:
+    return np.asfortranarray(t.cpu().numpy()[:, :m].T)
+
+
+def _factor(dev, a, ld=None):
+    n = a.shape[0]
+    dA, lda = to_dev(a, ld)
+    dp = torch.zeros(n, dtype=torch.int32, device="cuda")
+    assert dev.dgetrf(n, n, dA, lda, dp) == 0
+    return dA, lda, dp
+
+
+# ------------------------------------------------------------------ PDLANGE
+@pytest.mark.parametrize("idx", range(3))
+def test_dlange_matches_reference_bitwise(dev, oracle, golden, idx):
+    N, NB, NRHS = golden.aux_cases()[idx]
+    _note(f"dlange golden N={N}")
+    r = golden.aux(N, NB, NRHS)
+    a = oracle.matgen(N, N, 100)
+    dA, lda = to_dev(a, N + 3)
+    for k, c in enumerate("M1I"):
+        assert dev.dlange(c, N, N, dA, lda) == r["norms"][k], c
+    assert np.isclose(dev.dlange("F", N, N, dA, lda), r["norms"][3], rtol=1e-14, atol=0)
+    # the submatrix A(2:N-2, 3:N): offset pointer, same leading dimension
+    sub = dA.view(-1)[(3 - 1) * lda + (2 - 1):]
+    for k, c in enumerate("M1I"):
+        assert dev.dlange(c, N - 3, N - 2, sub, lda) == r["subnorms"][k], c
+    assert np.isclose(dev.dlange("F", N - 3, N - 2, sub, lda), r["subnorms"][3], rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("m,n", [(1, 1), (1, 300), (300, 1), (1000, 777), (4097, 130), (64, 64)])
+def test_dlange_shapes_match_oracle(dev, oracle, m, n):
+    _note(f"dlange m={m} n={n}")
+    rng = np.random.default_rng(m * 13 + n)
+    a = np.asfortranarray(rng.uniform(-2, 2, (m, n)) * rng.choice([1e-3, 1, 1e3], size=(1, n)))
+    dA, lda = to_dev(a, m + 5)
+    for c in "M1OI":
+        assert dev.dlange(c, m, n, dA, lda) == oracle.pdlange(c, a), c
+    for c in "FE":
+        assert np.isclose(dev.dlange(c, m, n, dA, lda), oracle.pdlange(c, a), rtol=1e-14, atol=0)
+    assert dev.dlange("M", 0, n, dA, lda) == 0.0
+
+
+def test_dlange_large_exact(dev, oracle):
+    """8192 x 8192 (512 MiB): the sequential-order sums at scale."""
+    n = 8192
+    _note(f"dlange n={n}")
+    a = oracle.matgen(n, n, 77)
+    dA, lda = to_dev(a)
+    for c in "M1I":
+        assert dev.dlange(c, n, n, dA, lda) == oracle.pdlange(c, a), c
+    del dA
+    torch.cuda.empty_cache()
+
+
+# --------------------------------------------------------- PDGETRS 'T'
+@pytest.mark.parametrize("idx", range(3))
+def test_getrs_transposed_matches_reference(dev, oracle, golden, idx):
+    N, NB, NRHS = golden.aux_cases()[idx]
+    _note(f"getrs T golden N={N}")
+    r = golden.aux(N, NB, NRHS)
+    a = oracle.matgen(N, N, 100)
+    b = oracle.matgen(N, NRHS, 200)
+    dA, lda, dp = _factor(dev, a)
+    assert np.array_equal(dp.cpu().numpy(), r["ipiv"])
+    dB, ldb = to_dev(b)
+    dev.dgetrs_trans("T", N, NRHS, dA, lda, dp, dB, ldb)
+    x = from_dev(dB, N)
+    assert np.allclose(x, r["xT"], rtol=0, atol=1e-12 * np.abs(r["xT"]).max())
+    dB, ldb = to_dev(b)
+    dev.dgetrs_trans("C", N, NRHS, dA, lda, dp, dB, ldb)
+    assert np.array_equal(from_dev(dB, N), x)
+
+
+@pytest.mark.parametrize("n,nrhs", [(2, 1), (65, 3), (1000, 17), (4100, 1)])
+def test_getrs_transposed_solves(dev, oracle, n, nrhs):
+    _note(f"getrs T n={n} nrhs={nrhs}")
+    a = oracle.matgen(n, n, 5 + n)
+    b = oracle.matgen(n, nrhs, 6 + n)
+    lu = a.copy(order="F")
+    piv, _ = oracle.pdgetrf(lu, 64)
+    xo = oracle.pdgetrs_t(lu, piv, b)
+    dA, lda, dp = _factor(dev, a, n + 1)
+    dB, ldb = to_dev(b, n + 2)
+    dev.dgetrs_trans("T", n, nrhs, dA, lda, dp, dB, ldb)
+    x = from_dev(dB, n)
+    assert np.allclose(x, xo, rtol=0, atol=1e-10 * np.abs(xo).max())
+    # the 'N' solve through the same entry equals dgetrs
+    dB1, _ = to_dev(b, n + 2)
+    dB2, _ = to_dev(b, n + 2)
+    dev.dgetrs_trans("N", n, nrhs, dA, lda, dp, dB1, ldb)
+    dev.dgetrs(n, nrhs, dA, lda, dp, dB2, ldb)
+    assert torch.equal(dB1[:, :n], dB2[:, :n])
+
+
+# ----------------------------------------------------------------- PDGECON
+@pytest.mark.parametrize("idx", range(3))
+def test_dgecon_matches_reference(dev, oracle, golden, idx):
+    N, NB, NRHS = golden.aux_cases()[idx]
+    _note(f"dgecon golden N={N}")
+    r = golden.aux(N, NB, NRHS)
+    a = oracle.matgen(N, N, 100)
+    dA, lda, dp = _factor(dev, a)
+    rc1 = dev.dgecon("1", N, dA, lda, r["norms"][1])
+    rcI = dev.dgecon("I", N, dA, lda, r["norms"][2])
+    assert np.isclose(rc1, r["rcond"][0], rtol=1e-10, atol=0)
+    assert np.isclose(rcI, r["rcond"][1], rtol=1e-10, atol=0)
+    assert dev.dgecon("O", N, dA, lda, r["norms"][1]) == rc1
+
+
+@pytest.mark.parametrize("n", [2, 129, 1000, 3000])
+def test_dgecon_matches_oracle(dev, oracle, n):
+    _note(f"dgecon n={n}")
+    a = oracle.matgen(n, n, 300 + n)
+    lu = a.copy(order="F")
+    oracle.pdgetrf(lu, 64)
+    dA, lda, dp = _factor(dev, a, n + 7)
+    for c in "1I":
+        anorm = oracle.pdlange(c, a)
+        ro = oracle.pdgecon(c, lu, anorm)
+        assert np.isclose(dev.dgecon(c, n, dA, lda, anorm), ro, rtol=1e-10, atol=0), c
+
+
+def test_dgecon_quick_returns_and_lapack_steps(dev, oracle):
+    _note("dgecon quick returns / lacon_lapack")
+    a = oracle.matgen(300, 300, 9)
+    dA, lda, dp = _factor(dev, a)
+    assert dev.dgecon("1", 0, dA, lda, 1.0) == 1.0
+    assert dev.dgecon("1", 300, dA, lda, 0.0) == 0.0
+    assert dev.dgecon("1", 1, dA, lda, 5.0) == 1.0
+    ainv = np.linalg.inv(a)
+    for c, ax in (("1", 0), ("I", 1)):
+        anorm = np.abs(a).sum(ax).max()
+        true = 1.0 / (anorm * np.abs(ainv).sum(ax).max())
+        ref = dev.dgecon(c, 300, dA, lda, anorm)
+        dev.set("lacon_lapack", 1)
+        try:
+            lap = dev.dgecon(c, 300, dA, lda, anorm)
+        finally:
+            dev.set("lacon_lapack", 0)
+        # every estimate of ||A^-1|| is a lower bound: rcond >= the true one;
+        # LAPACK's steps never give a smaller estimate than label 140 alone
+        assert lap >= true * (1 - 1e-12) and ref >= true * (1 - 1e-12)
+        assert lap <= ref * (1 + 1e-12)
+        assert lap <= 3 * true
+
+
+# ----------------------------------------------------------------- PDGERFS
+@pytest.mark.parametrize("idx", range(3))
+def test_dgerfs_matches_reference(dev, oracle, golden, idx):
+    N, NB, NRHS = golden.aux_cases()[idx]
+    _note(f"dgerfs golden N={N}")
+    r = golden.aux(N, NB, NRHS)
+    a = oracle.matgen(N, N, 100)
+    b = oracle.matgen(N, NRHS, 200)
+    dA0, lda0 = to_dev(a)
+    dAF, ldaf, dp = _factor(dev, a, N + 1)
+    dB, ldb = to_dev(b)
+    for c, tr in enumerate("NNT"):
+        dX, ldx = to_dev(r[f"rfs{c}_x0"], N + 4)
+        ferr, berr = dev.dgerfs(tr, N, NRHS, dA0, lda0, dAF, ldaf, dp, dB, ldb, dX, ldx)
+        x = from_dev(dX, N)
+        xr = r[f"rfs{c}_x"]
+        assert np.allclose(x, xr, rtol=0, atol=1e-13 * np.abs(xr).max()), (c, tr)
+        assert np.allclose(ferr, r[f"rfs{c}_ferr"], rtol=8 / (N + 1), atol=0), (c, tr)
+        rb = r[f"rfs{c}_berr"]
+        assert np.all(np.array(berr) <= 4 * EPS), (c, tr, berr)
+        assert np.all(np.array(berr) <= 2 * rb + EPS) and np.all(rb <= 2 * np.array(berr) + EPS)
+
+
+@pytest.mark.parametrize("n,nrhs,trans", [(300, 2, "N"), (300, 2, "T"), (2049, 1, "N"), (2049, 1, "T")])
+def test_dgerfs_from_far_start_matches_oracle(dev, oracle, n, nrhs, trans):
+    _note(f"dgerfs n={n} trans={trans}")
+    a = np.asfortranarray(oracle.matgen(n, n, 500 + n) * np.geomspace(1, 1e2, n)[:, None])
+    b = oracle.matgen(n, nrhs, 501 + n)
+    lu = a.copy(order="F")
+    piv, _ = oracle.pdgetrf(lu, 64)
+    opa = a if trans == "N" else np.asfortranarray(a.T)
+    xt = np.linalg.solve(opa, b)
+    x0 = np.asfortranarray(xt * (1 + 1e-5 * np.sin(np.arange(n)))[:, None])
+    xo, fo, bo = oracle.pdgerfs(trans, a, lu, piv, b, x0)
+    dA0, lda0 = to_dev(a)
+    dAF, ldaf, dp = _factor(dev, a)
+    dB, ldb = to_dev(b)
+    dX, ldx = to_dev(x0)
+    ferr, berr = dev.dgerfs(trans, n, nrhs, dA0, lda0, dAF, ldaf, dp, dB, ldb, dX, ldx)
+    x = from_dev(dX, n)
+    # each refined solution is within about kappa * eps of the exact one
+    kappa = np.linalg.cond(opa, 1)
+    assert np.allclose(x, xo, rtol=0, atol=4 * kappa * EPS * np.abs(xo).max())
+    assert np.allclose(ferr, fo, rtol=8 / (n + 1), atol=0)
+    assert np.all(np.array(berr) <= 4 * EPS)
+
+
+# ------------------------------------------------------- Fortran drop-ins
+def test_fortran_entries_match_reference(ctx, oracle, golden):
+    from scalapack_amd import api
+    N, NB, NRHS = 200, 32, 3
+    _note("fortran pdlange/pdgecon/pdgetrs T/pdgerfs")
+    r = golden.aux(N, NB, NRHS)
+    a = oracle.matgen(N, N, 100)
+    b = oracle.matgen(N, NRHS, 200)
+    desca, _ = api.descinit(N, N, NB, NB, 0, 0, ctx, N)
+    descb, _ = api.descinit(N, NRHS, NB, NB, 0, 0, ctx, N)
+    for k, c in enumerate("M1I"):
+        assert api.pdlange(c, N, N, a, 1, 1, desca) == r["norms"][k]
+        assert api.pdlange(c, N - 3, N - 2, a, 2, 3, desca) == r["subnorms"][k]
+    af = a.copy(order="F")
+    ipiv = np.zeros(N + NB, np.int32)
+    assert api.pdgetrf(N, N, af, 1, 1, desca, ipiv) == 0
+    assert np.array_equal(ipiv[:N], r["ipiv"])
+    # workspace query, then the call
+    work, iwork = np.zeros(1), np.zeros(1, np.int32)
+    rc, info = api.pdgecon("1", N, af, 1, 1, desca, r["norms"][1], work, -1, iwork, -1)
+    assert info == 0 and int(work[0]) == 2 * N + 2 * N + N + NB and iwork[0] == N
+    work, iwork = np.zeros(int(work[0])), np.zeros(int(iwork[0]), np.int32)
+    rc, info = api.pdgecon("1", N, af, 1, 1, desca, r["norms"][1], work, work.size, iwork, iwork.size)
+    assert info == 0 and np.isclose(rc, r["rcond"][0], rtol=1e-10, atol=0)
+    rc, info = api.pdgecon("I", N, af, 1, 1, desca, r["norms"][2], work, work.size, iwork, iwork.size)
+    assert info == 0 and np.isclose(rc, r["rcond"][1], rtol=1e-10, atol=0)
+    # illegal arguments as the reference's checks (pdgecon.f:239-247)
+    assert api.pdgecon("X", N, af, 1, 1, desca, 1.0, work, work.size, iwork, iwork.size)[1] == -1
+    assert api.pdgecon("1", N, af, 1, 1, desca, -1.0, work, work.size, iwork, iwork.size)[1] == -7
+    assert api.pdgecon("1", N, af, 1, 1, desca, 1.0, work, 3, iwork, iwork.size)[1] == -10
+    assert api.pdgecon("1", N, af, 1, 1, desca, 1.0, work, work.size, iwork, 0)[1] == -12
+    # transposed solve
+    xt = b.copy(order="F")
+    assert api.pdgetrs("T", N, NRHS, af, 1, 1, desca, ipiv, xt, 1, 1, descb) == 0
+    assert np.allclose(xt, r["xT"], rtol=0, atol=1e-12 * np.abs(r["xT"]).max())
+    # refinement, all three fixture cases
+    lw, liw = 3 * N, N
+    for c, tr in enumerate("NNT"):
+        x = r[f"rfs{c}_x0"].copy(order="F")
+        ferr, berr = np.zeros(NRHS), np.zeros(NRHS)
+        info = api.pdgerfs(tr, N, NRHS, a, 1, 1, desca, af, 1, 1, desca, ipiv, b, 1, 1, descb,
+                           x, 1, 1, descb, ferr, berr, np.zeros(lw), lw, np.zeros(liw, np.int32), liw)
+        assert info == 0
+        assert np.allclose(x, r[f"rfs{c}_x"], rtol=0, atol=1e-13 * np.abs(x).max())
+        assert np.allclose(ferr, r[f"rfs{c}_ferr"], rtol=8 / (N + 1), atol=0)
+        assert np.all(berr <= 4 * EPS)
+    x = r["rfs0_x0"].copy(order="F")
+    ferr, berr = np.zeros(NRHS), np.zeros(NRHS)
+    bad = api.pdgerfs("X", N, NRHS, a, 1, 1, desca, af, 1, 1, desca, ipiv, b, 1, 1, descb,
+                      x, 1, 1, descb, ferr, berr, np.zeros(lw), lw, np.zeros(liw, np.int32), liw)
+    assert bad == -1
+    bad = api.pdgerfs("N", N, NRHS, a, 1, 1, desca, af, 1, 1, desca, ipiv, b, 1, 1, descb,
+                      x, 1, 1, descb, ferr, berr, np.zeros(lw), lw - 1, np.zeros(liw, np.int32), liw)
+    assert bad == -24
