@@ -225,6 +225,31 @@ int sla_sgetrs(sla_handle_t h, int n, int nrhs, const float *dA, int lda, const 
  * sum order is fixed, so R is reproducible bit for bit. */
 int sla_dresidual(sla_handle_t h, int n, int nrhs, const double *dA, int lda, const double *dX,
                   int ldx, const double *dB, int ldb, double *dR, int ldr, double *rnorm);
+/* ------------------------------------------------------------------------ */
+/* Part 4: batched small systems (no counterpart in the reference: thousands  */
+/* of independent PDGESV-sized-down solves in one persistent launch).         */
+/* ------------------------------------------------------------------------ */
+/* A_s X_s = B_s for s < batch, each as PDGESV on a 1 x 1 grid (partial
+ * pivoting with IDAMAX's first maximum, rows interchanged across the matrix,
+ * reciprocal scaling: SRC/pdgetf2.f:206-237; solve as SRC/pdgetrs.f:255-266,
+ * skipped when INFO > 0 as SRC/pdgesv.f:231).  prec: SLA_PREC_FP64 (double
+ * matrices) or SLA_PREC_FP32 (float).  n, nrhs, lda, ldb: host int[batch]
+ * (n <= 1024, nrhs <= 16); dA, dB: device arrays of device pointers to the
+ * column-major matrices; dipiv: device array of device pointers to int[n]
+ * (1-based rows); dinfo: device int[batch].  One persistent launch takes the
+ * systems largest first.  Handle knobs: "batched_w" (panel width 8 / 16 /
+ * 32), "batched_wgs_per_cu" (1..8), "batched_graph" (1: the launch is
+ * captured into a hipGraph once per set of pointer arrays and sizes, then
+ * replayed). */
+int sla_gesv_batched(sla_handle_t h, int prec, int batch, const int *n, const int *nrhs,
+                     void *const *dA, const int *lda, int *const *dipiv, void *const *dB,
+                     const int *ldb, int *dinfo);
+/* Autotuner: times every panel width and workgroups-per-CU on synthetic
+ * systems of these sizes (its own device memory), keeps the fastest in the
+ * handle's knobs and returns it (*ms: the batch's time in milliseconds). */
+int sla_gesv_batched_tune(sla_handle_t h, int prec, int batch, const int *n, const int *nrhs,
+                          int *w, int *wgs_per_cu, double *ms);
+
 /* The low-precision trailing-update kernel alone: C -= A * B (FP32
  * column-major operands, products in `prec`, FP32 accumulation). */
 int sla_gemm_lowp_sub(sla_handle_t h, int prec, int m, int n, int k, const float *dA, int lda,

@@ -71,6 +71,8 @@ def _declare(lib: ctypes.CDLL) -> None:
         "sla_dlange": [vp, ctypes.c_char, i, i, vp, i, vp],
         "sla_dgecon": [vp, ctypes.c_char, i, vp, i, d, vp],
         "sla_dgerfs": [vp, ctypes.c_char, i, i, vp, i, vp, i, vp, vp, i, vp, i, vp, vp],
+        "sla_gesv_batched": [vp, i, i, vp, vp, vp, vp, vp, vp, vp, vp],
+        "sla_gesv_batched_tune": [vp, i, i, vp, vp, vp, vp, vp],
     }
     for name, args in sigs.items():
         f = getattr(lib, name)

@@ -1,0 +1,538 @@
+// batched.hip -- many independent small systems A_s X_s = B_s (n_s <= 1024,
+// nrhs_s <= 16, uneven sizes) factored and solved in ONE persistent launch.
+//
+// Each system follows the LU path of a 1 x 1 grid: partial pivoting with the
+// first maximum |a| of each column (IDAMAX), rows interchanged across the
+// whole matrix, the pivot column scaled by the reciprocal of the pivot
+// (SRC/pdgetf2.f:206-237), a zero pivot recorded as INFO = column and the
+// factorization continued (pdgetrf.f:297-302); then PDGETRS's interchanges
+// and two triangular solves (pdgetrs.f:255-266), skipped when INFO > 0
+// (pdgesv.f:231).
+//
+// Work distribution: workgroups (256 threads) take systems from a device
+// queue, largest first (the host sorts the sizes once), until the queue is
+// empty; every workgroup leaves when the ticket passes the batch size.
+// Inside a system the factorization is blocked right-looking with W-column
+// panels:
+//   panel    columns kb .. kb+W-1 unblocked: pivot search as a wave-shuffle
+//            argmax folded across the four waves in LDS, row interchange over
+//            all n columns, reciprocal scaling, rank-1 update of the panel;
+//   U12      L11^{-1} A12, one thread per column of A12;
+//   update   A22 -= L21 U12 in 64 x 64 tiles, L21 / U12 staged through LDS,
+//            4 x 4 outputs per thread.
+// The solves take 64-row blocks: one wave solves the diagonal block in
+// registers (lane = row, values broadcast by lane reads), then all four
+// waves update the rows below (above) with the 64 new values.
+#include "sla_common.h"
+#include "sla_handle.h"
+#include "sla_batched.h"
+
+#include <algorithm>
+#include <climits>
+#include <cstring>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace sla {
+
+namespace {
+
+constexpr int BT = 256;       // threads per workgroup
+constexpr int UT = 64;        // update tile edge
+
+struct SysDesc {
+    int n, nrhs, lda, ldb;
+};
+
+template <typename T>
+__device__ __forceinline__ void argmax_fold(T &v, int &i, T v2, int i2)
+{
+    if (v2 > v || (v2 == v && i2 < i)) {
+        v = v2;
+        i = i2;
+    }
+}
+
+template <typename T, int W>
+__device__ void factor_system(int n, T *__restrict__ a, int lda, int *__restrict__ piv, int &info,
+                              T (*Ls)[W + 1], T (*Us)[UT + 1], T *s_v, int *s_i)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int kb = 0; kb < n; kb += W) {
+        const int w = min(W, n - kb);
+        // ---- panel (unblocked), columns kb .. kb+w-1
+        for (int j = kb; j < kb + w; j++) {
+            T best = T(-1);
+            int bi = INT_MAX;
+            const T *col = a + (size_t)j * lda;
+            for (int i = j + tid; i < n; i += BT) {
+                const T v = fabs(col[i]);
+                if (v > best) {
+                    best = v;
+                    bi = i;
+                }
+            }
+            for (int o = 32; o > 0; o >>= 1) argmax_fold(best, bi, __shfl_xor(best, o), __shfl_xor(bi, o));
+            if (lane == 0) {
+                s_v[wave] = best;
+                s_i[wave] = bi;
+            }
+            __syncthreads();
+            best = s_v[0];
+            bi = s_i[0];
+            for (int q = 1; q < 4; q++) argmax_fold(best, bi, s_v[q], s_i[q]);
+            const int p = bi < n ? bi : j;  // no comparable entry (all NaN): keep row j
+            const T pv = col[p];
+            __syncthreads();  // s_v / s_i are reused by the next column
+            if (tid == 0) piv[j] = p + 1;
+            if (pv != T(0)) {
+                if (p != j)
+                    for (int c = tid; c < n; c += BT) {
+                        T *cc = a + (size_t)c * lda;
+                        const T t = cc[j];
+                        cc[j] = cc[p];
+                        cc[p] = t;
+                    }
+                __syncthreads();
+                const T r = T(1) / pv;
+                T *cj = a + (size_t)j * lda;
+                for (int i = j + 1 + tid; i < n; i += BT) cj[i] *= r;
+            } else if (info == 0) {
+                info = j + 1;
+            }
+            __syncthreads();
+            // rank-1 update of the panel's columns right of j
+            const int rows = n - j - 1, cols = kb + w - j - 1;
+            if (rows > 0 && cols > 0) {
+                const T *lj = a + (size_t)j * lda;
+                for (int idx = tid; idx < rows * cols; idx += BT) {
+                    const int i = j + 1 + idx % rows, c = j + 1 + idx / rows;
+                    T *cc = a + (size_t)c * lda;
+                    cc[i] = fma(-lj[i], cc[j], cc[i]);
+                }
+            }
+            __syncthreads();
+        }
+        const int c0 = kb + w;
+        if (c0 >= n) break;
+        // ---- U12 = L11^{-1} A12 (unit lower L11), one thread per column
+        for (int c = c0 + tid; c < n; c += BT) {
+            T *cc = a + (size_t)c * lda;
+            for (int r = kb; r < kb + w; r++) {
+                const T x = cc[r];
+                for (int r2 = r + 1; r2 < kb + w; r2++) cc[r2] = fma(-a[r2 + (size_t)r * lda], x, cc[r2]);
+            }
+        }
+        __syncthreads();
+        // ---- A22 -= L21 U12 in 64 x 64 tiles
+        const int m2 = n - c0;
+        const int tr = tid & 15, tc = tid >> 4;  // 4 x 4 outputs at rows 4tr.., cols 4tc..
+        for (int t0 = 0; t0 < m2; t0 += UT)
+            for (int u0 = 0; u0 < m2; u0 += UT) {
+                const int rows = min(UT, m2 - t0), cols = min(UT, m2 - u0);
+                for (int idx = tid; idx < UT * w; idx += BT) {
+                    const int i = idx % UT, k = idx / UT;
+                    Ls[i][k] = i < rows ? a[(size_t)(c0 + t0 + i) + (size_t)(kb + k) * lda] : T(0);
+                }
+                for (int idx = tid; idx < UT * w; idx += BT) {
+                    const int k = idx % w, c = idx / w;
+                    Us[k][c] = c < cols ? a[(size_t)(kb + k) + (size_t)(c0 + u0 + c) * lda] : T(0);
+                }
+                __syncthreads();
+                T acc[4][4];
+#pragma unroll
+                for (int x = 0; x < 4; x++)
+#pragma unroll
+                    for (int y = 0; y < 4; y++) acc[x][y] = T(0);
+                for (int k = 0; k < w; k++) {
+                    T l[4], u[4];
+#pragma unroll
+                    for (int x = 0; x < 4; x++) l[x] = Ls[4 * tr + x][k];
+#pragma unroll
+                    for (int y = 0; y < 4; y++) u[y] = Us[k][4 * tc + y];
+#pragma unroll
+                    for (int x = 0; x < 4; x++)
+#pragma unroll
+                        for (int y = 0; y < 4; y++) acc[x][y] = fma(l[x], u[y], acc[x][y]);
+                }
+#pragma unroll
+                for (int y = 0; y < 4; y++) {
+                    const int c = 4 * tc + y;
+                    if (c < cols) {
+                        T *cc = a + (size_t)(c0 + u0 + c) * lda + c0 + t0;
+#pragma unroll
+                        for (int x = 0; x < 4; x++)
+                            if (4 * tr + x < rows) cc[4 * tr + x] -= acc[x][y];
+                    }
+                }
+                __syncthreads();  // Ls / Us are reloaded by the next tile
+            }
+    }
+}
+
+// X = U^{-1} L^{-1} P B for one right-hand side column x (length n).
+template <typename T>
+__device__ void solve_system(int n, const T *__restrict__ a, int lda, const int *__restrict__ piv,
+                             T *__restrict__ x)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // P: the interchanges in order (pdlapv2.f:208-242)
+    if (tid == 0)
+        for (int j = 0; j < n; j++) {
+            const int p = piv[j] - 1;
+            if (p != j) {
+                const T t = x[j];
+                x[j] = x[p];
+                x[p] = t;
+            }
+        }
+    __syncthreads();
+    // L (unit lower), 64-row blocks top down
+    for (int b0 = 0; b0 < n; b0 += 64) {
+        const int rows = min(64, n - b0);
+        if (wave == 0) {
+            T v = lane < rows ? x[b0 + lane] : T(0);
+            for (int i = 0; i < rows; i++) {
+                const T xi = lane_read(v, i);
+                if (lane > i && lane < rows) v = fma(-a[(size_t)(b0 + lane) + (size_t)(b0 + i) * lda], xi, v);
+            }
+            if (lane < rows) x[b0 + lane] = v;
+        }
+        __syncthreads();
+        for (int i = b0 + rows + tid; i < n; i += BT) {
+            T acc = T(0);
+            for (int k = 0; k < rows; k++) acc = fma(a[(size_t)i + (size_t)(b0 + k) * lda], x[b0 + k], acc);
+            x[i] -= acc;
+        }
+        __syncthreads();
+    }
+    // U (upper, non-unit), 64-row blocks bottom up
+    const int nb = (n + 63) / 64;
+    for (int bb = nb - 1; bb >= 0; bb--) {
+        const int b0 = bb * 64, rows = min(64, n - b0);
+        if (wave == 0) {
+            T v = lane < rows ? x[b0 + lane] : T(0);
+            const T d = lane < rows ? a[(size_t)(b0 + lane) * (lda + 1)] : T(1);
+            for (int i = rows - 1; i >= 0; i--) {
+                if (lane == i) v = v / d;
+                const T xi = lane_read(v, i);
+                if (lane < i) v = fma(-a[(size_t)(b0 + lane) + (size_t)(b0 + i) * lda], xi, v);
+            }
+            if (lane < rows) x[b0 + lane] = v;
+        }
+        __syncthreads();
+        for (int i = tid; i < b0; i += BT) {
+            T acc = T(0);
+            for (int k = 0; k < rows; k++) acc = fma(a[(size_t)i + (size_t)(b0 + k) * lda], x[b0 + k], acc);
+            x[i] -= acc;
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T, int W>
+__global__ __launch_bounds__(BT) void gesv_batched_kernel(int batch, const int *__restrict__ order,
+                                                          const SysDesc *__restrict__ desc,
+                                                          T *const *__restrict__ A,
+                                                          int *const *__restrict__ ipiv,
+                                                          T *const *__restrict__ B,
+                                                          int *__restrict__ info, int *queue)
+{
+    __shared__ T Ls[UT][W + 1];
+    __shared__ T Us[W][UT + 1];
+    __shared__ T s_v[4];
+    __shared__ int s_i[4];
+    __shared__ int s_q;
+    for (;;) {
+        if (threadIdx.x == 0) s_q = atomicAdd(queue, 1);
+        __syncthreads();
+        const int q = s_q;
+        __syncthreads();  // s_q is rewritten by the next ticket
+        if (q >= batch) break;
+        const int s = order[q];
+        const SysDesc d = desc[s];
+        T *a = A[s];
+        int *piv = ipiv[s];
+        int inf = 0;
+        if (d.n > 0) factor_system<T, W>(d.n, a, d.lda, piv, inf, Ls, Us, s_v, s_i);
+        if (threadIdx.x == 0) info[s] = inf;
+        if (inf == 0 && d.n > 0)
+            for (int r = 0; r < d.nrhs; r++) solve_system<T>(d.n, a, d.lda, piv, B[s] + (size_t)r * d.ldb);
+        __syncthreads();
+    }
+}
+
+__global__ void zero_queue_kernel(int *q)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *q = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Synthetic systems for the tuner: A uniform[-1, 1) plus n I, B uniform.
+__device__ __forceinline__ double hash_uniform(uint64_t x)
+{
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdull;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ull;
+    x ^= x >> 33;
+    return (double)(x >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+template <typename T>
+__global__ void fill_synthetic_kernel(int batch, const SysDesc *__restrict__ desc, T *const *A,
+                                      T *const *B)
+{
+    for (int s = blockIdx.y; s < batch; s += gridDim.y) {
+        const SysDesc d = desc[s];
+        const size_t na = (size_t)d.lda * d.n, nbv = (size_t)d.ldb * d.nrhs;
+        for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < na + nbv;
+             e += (size_t)gridDim.x * 256) {
+            const double u = hash_uniform(((uint64_t)s << 32) ^ e);
+            if (e < na) {
+                const int i = (int)(e % d.lda), j = (int)(e / d.lda);
+                A[s][e] = (T)(u + (i == j ? (double)d.n : 0.0));
+            } else {
+                B[s][e - na] = (T)u;
+            }
+        }
+    }
+}
+
+}  // namespace
+
+// ============================================================================
+// Host side
+// ============================================================================
+struct BatchedPlan {
+    int batch = 0;
+    std::vector<int> n, nrhs, lda, ldb;
+    int *d_order = nullptr;
+    SysDesc *d_desc = nullptr;
+    int *d_queue = nullptr;
+    void *graph_exec = nullptr;  // hipGraphExec_t of the last captured launch
+    const void *g_key[4] = {nullptr, nullptr, nullptr, nullptr};
+    int g_prec = -1, g_w = 0, g_wgs = 0;
+    ~BatchedPlan()
+    {
+        if (graph_exec) (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph_exec));
+        if (d_order) (void)hipFree(d_order);
+        if (d_desc) (void)hipFree(d_desc);
+        if (d_queue) (void)hipFree(d_queue);
+    }
+};
+
+namespace {
+
+bool same_sizes(const BatchedPlan &p, int batch, const int *n, const int *nrhs, const int *lda,
+                const int *ldb)
+{
+    if (p.batch != batch) return false;
+    for (int s = 0; s < batch; s++)
+        if (p.n[s] != n[s] || p.nrhs[s] != nrhs[s] || p.lda[s] != lda[s] || p.ldb[s] != ldb[s])
+            return false;
+    return true;
+}
+
+template <typename T>
+void launch(const BatchedPlan &p, int w, int wgs, T *const *A, int *const *ipiv, T *const *B,
+            int *info, hipStream_t s)
+{
+    hipLaunchKernelGGL(zero_queue_kernel, dim3(1), dim3(64), 0, s, p.d_queue);
+#define SLA_BATCHED(WV)                                                                          \
+    hipLaunchKernelGGL((gesv_batched_kernel<T, WV>), dim3(wgs), dim3(BT), 0, s, p.batch,         \
+                       p.d_order, p.d_desc, A, ipiv, B, info, p.d_queue)
+    switch (w) {
+    case 8: SLA_BATCHED(8); break;
+    case 16: SLA_BATCHED(16); break;
+    case 32: SLA_BATCHED(32); break;
+    default: throw std::invalid_argument("batched: panel width must be 8, 16 or 32");
+    }
+#undef SLA_BATCHED
+    SLA_LAUNCH_CHECK();
+}
+
+}  // namespace
+
+BatchedPlan &batched_plan(Handle &h, int batch, const int *n, const int *nrhs, const int *lda,
+                          const int *ldb)
+{
+    if (!h.batched) h.batched = new BatchedPlan();
+    BatchedPlan &p = *h.batched;
+    if (same_sizes(p, batch, n, nrhs, lda, ldb)) return p;
+    for (int s = 0; s < batch; s++) {
+        if (n[s] < 0 || n[s] > BATCHED_NMAX || nrhs[s] < 0 || nrhs[s] > BATCHED_NRHS_MAX ||
+            lda[s] < std::max(1, n[s]) || ldb[s] < std::max(1, n[s]))
+            throw std::invalid_argument("batched: system " + std::to_string(s) +
+                                        " has n, nrhs, lda or ldb out of range");
+    }
+    SLA_HIP(hipStreamSynchronize(h.stream));
+    if (p.graph_exec) {
+        SLA_HIP(hipGraphExecDestroy(static_cast<hipGraphExec_t>(p.graph_exec)));
+        p.graph_exec = nullptr;
+    }
+    if (p.d_order) SLA_HIP(hipFree(p.d_order));
+    if (p.d_desc) SLA_HIP(hipFree(p.d_desc));
+    if (!p.d_queue) SLA_HIP(hipMalloc(&p.d_queue, 64));
+    p.batch = batch;
+    p.n.assign(n, n + batch);
+    p.nrhs.assign(nrhs, nrhs + batch);
+    p.lda.assign(lda, lda + batch);
+    p.ldb.assign(ldb, ldb + batch);
+    // largest work first (n^3 + n^2 nrhs), ties by index
+    std::vector<int> order(batch);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        const double wx = (double)n[x] * n[x] * (n[x] + 3.0 * nrhs[x]);
+        const double wy = (double)n[y] * n[y] * (n[y] + 3.0 * nrhs[y]);
+        return wx > wy;
+    });
+    std::vector<SysDesc> desc(batch);
+    for (int s = 0; s < batch; s++) desc[s] = SysDesc{n[s], nrhs[s], lda[s], ldb[s]};
+    SLA_HIP(hipMalloc(&p.d_order, sizeof(int) * std::max(1, batch)));
+    SLA_HIP(hipMalloc(&p.d_desc, sizeof(SysDesc) * std::max(1, batch)));
+    SLA_HIP(hipMemcpy(p.d_order, order.data(), sizeof(int) * batch, hipMemcpyHostToDevice));
+    SLA_HIP(hipMemcpy(p.d_desc, desc.data(), sizeof(SysDesc) * batch, hipMemcpyHostToDevice));
+    return p;
+}
+
+void batched_plan_free(BatchedPlan *p) { delete p; }
+
+template <typename T>
+void gesv_batched_device(Handle &h, int batch, const int *n, const int *nrhs, T *const *dA,
+                         const int *lda, int *const *dipiv, T *const *dB, const int *ldb, int *dinfo)
+{
+    if (batch <= 0) return;
+    BatchedPlan &p = batched_plan(h, batch, n, nrhs, lda, ldb);
+    const int w = h.batched_w;
+    const int wgs = std::max(1, h.batched_wgs_per_cu) * h.num_cus;
+    const int prec = sizeof(T) == 8 ? 0 : 1;
+    if (!h.batched_graph) {
+        launch<T>(p, w, wgs, dA, dipiv, dB, dinfo, h.stream);
+        SLA_HIP(hipStreamSynchronize(h.stream));
+        return;
+    }
+    // hipGraph path: the queue reset (a kernel, not a memset) and the
+    // persistent launch captured once per (pointer arrays, sizes, shape)
+    // and replayed afterwards.
+    const void *key[4] = {dA, dipiv, dB, dinfo};
+    const bool hit = p.graph_exec && std::equal(key, key + 4, p.g_key) && p.g_prec == prec &&
+                     p.g_w == w && p.g_wgs == wgs;
+    if (!hit) {
+        if (p.graph_exec) {
+            SLA_HIP(hipGraphExecDestroy(static_cast<hipGraphExec_t>(p.graph_exec)));
+            p.graph_exec = nullptr;
+        }
+        hipStream_t cs = h.own_stream;  // capture on the handle's own stream
+        SLA_HIP(hipStreamSynchronize(h.stream));
+        hipGraph_t g = nullptr;
+        SLA_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        try {
+            launch<T>(p, w, wgs, dA, dipiv, dB, dinfo, cs);
+        } catch (...) {
+            (void)hipStreamEndCapture(cs, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        SLA_HIP(hipStreamEndCapture(cs, &g));
+        hipGraphExec_t ge = nullptr;
+        SLA_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+        SLA_HIP(hipGraphDestroy(g));
+        p.graph_exec = ge;
+        std::copy(key, key + 4, p.g_key);
+        p.g_prec = prec;
+        p.g_w = w;
+        p.g_wgs = wgs;
+    }
+    SLA_HIP(hipGraphLaunch(static_cast<hipGraphExec_t>(p.graph_exec), h.stream));
+    SLA_HIP(hipStreamSynchronize(h.stream));
+}
+
+template void gesv_batched_device<double>(Handle &, int, const int *, const int *, double *const *,
+                                          const int *, int *const *, double *const *, const int *,
+                                          int *);
+template void gesv_batched_device<float>(Handle &, int, const int *, const int *, float *const *,
+                                         const int *, int *const *, float *const *, const int *,
+                                         int *);
+
+// Autotuner: times every (panel width, workgroups per CU) on synthetic
+// systems of the given sizes (diagonally dominant, so every candidate does
+// the same work) and keeps the fastest in the handle.
+template <typename T>
+BatchedTune tune_batched_device(Handle &h, int batch, const int *n, const int *nrhs, int reps)
+{
+    BatchedTune out;
+    if (batch <= 0) return out;
+    std::vector<int> lda(batch), ldb(batch);
+    size_t total = 0, total_piv = 0;
+    for (int s = 0; s < batch; s++) {
+        lda[s] = std::max(1, n[s]);
+        ldb[s] = std::max(1, n[s]);
+        total += (size_t)lda[s] * n[s] + (size_t)ldb[s] * nrhs[s];
+        total_piv += (size_t)std::max(1, n[s]);
+    }
+    BatchedPlan &p = batched_plan(h, batch, n, nrhs, lda.data(), ldb.data());
+    T *data = nullptr;
+    int *piv = nullptr, *info = nullptr;
+    void **ptrs = nullptr;
+    SLA_HIP(hipMalloc(&data, total * sizeof(T) + 256));
+    SLA_HIP(hipMalloc(&piv, total_piv * sizeof(int) + 256));
+    SLA_HIP(hipMalloc(&info, sizeof(int) * batch));
+    SLA_HIP(hipMalloc(&ptrs, sizeof(void *) * 3 * batch));
+    std::vector<void *> hp(3 * (size_t)batch);
+    size_t off = 0, offp = 0;
+    for (int s = 0; s < batch; s++) {
+        hp[s] = data + off;
+        off += (size_t)lda[s] * n[s];
+        hp[2 * (size_t)batch + s] = data + off;
+        off += (size_t)ldb[s] * nrhs[s];
+        hp[(size_t)batch + s] = piv + offp;
+        offp += (size_t)std::max(1, n[s]);
+    }
+    SLA_HIP(hipMemcpy(ptrs, hp.data(), sizeof(void *) * hp.size(), hipMemcpyHostToDevice));
+    T *const *dA = reinterpret_cast<T *const *>(ptrs);
+    int *const *dP = reinterpret_cast<int *const *>(ptrs + batch);
+    T *const *dB = reinterpret_cast<T *const *>(ptrs + 2 * (size_t)batch);
+    hipEvent_t e0, e1;
+    SLA_HIP(hipEventCreate(&e0));
+    SLA_HIP(hipEventCreate(&e1));
+    double best = 1e30;
+    for (int w : {8, 16, 32})
+        for (int k : {1, 2, 4}) {
+            double ms_sum = 0.0;
+            for (int r = 0; r < reps + 1; r++) {
+                hipLaunchKernelGGL(fill_synthetic_kernel<T>, dim3(64, std::min(batch, 65535)), dim3(256), 0, h.stream,
+                                   batch, p.d_desc, dA, dB);
+                SLA_HIP(hipEventRecord(e0, h.stream));
+                launch<T>(p, w, k * h.num_cus, dA, dP, dB, info, h.stream);
+                SLA_HIP(hipEventRecord(e1, h.stream));
+                SLA_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                SLA_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (r > 0) ms_sum += ms;  // the first run warms up
+            }
+            const double ms = ms_sum / std::max(1, reps);
+            out.tried.push_back({w, k, ms});
+            if (ms < best) {
+                best = ms;
+                out.w = w;
+                out.wgs_per_cu = k;
+                out.ms = ms;
+            }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(data);
+    (void)hipFree(piv);
+    (void)hipFree(info);
+    (void)hipFree(ptrs);
+    h.batched_w = out.w;
+    h.batched_wgs_per_cu = out.wgs_per_cu;
+    return out;
+}
+
+template BatchedTune tune_batched_device<double>(Handle &, int, const int *, const int *, int);
+template BatchedTune tune_batched_device<float>(Handle &, int, const int *, const int *, int);
+
+}  // namespace sla

@@ -13,6 +13,7 @@
 #include "sla_kernels.h"
 #include "sla_mixed.h"
 #include "sla_aux.h"
+#include "sla_batched.h"
 
 #include "scalapack_amd.h"
 #include "sla_blacs1x1.h"
@@ -741,7 +742,8 @@ int sla_set_param(sla_handle_t h, const char *name, int value)
     const std::string k(name ? name : "");
     static const char *known[] = {"nb",      "leaf_w",  "panel_wgs",        "profile",
                                   "ir_max_iter", "gmres_m", "ir_gmres_on_stall", "gmres_tol_exp",
-                                  "pivoting", "lookahead", "update_cus", "lacon_lapack"};
+                                  "pivoting", "lookahead", "update_cus", "lacon_lapack",
+                                  "batched_w", "batched_wgs_per_cu", "batched_graph"};
     if (std::find_if(std::begin(known), std::end(known), [&](const char *x) { return k == x; }) ==
         std::end(known)) {
         g_last_error = "sla_set_param: unknown parameter " + k;
@@ -759,6 +761,9 @@ int sla_set_param(sla_handle_t h, const char *name, int value)
     else if (k == "lookahead" && (value == 0 || value == 1)) H->lookahead = value;
     else if (k == "update_cus" && value >= 0 && value <= H->num_cus) H->update_cus = value;
     else if (k == "lacon_lapack" && (value == 0 || value == 1)) H->lacon_lapack = value;
+    else if (k == "batched_w" && (value == 8 || value == 16 || value == 32)) H->batched_w = value;
+    else if (k == "batched_wgs_per_cu" && value >= 1 && value <= 8) H->batched_wgs_per_cu = value;
+    else if (k == "batched_graph" && (value == 0 || value == 1)) H->batched_graph = value;
     else if (k == "gmres_tol_exp" && value >= 1 && value <= 15) H->gmres_tol = std::pow(10.0, -value);
     else {
         g_last_error = "sla_set_param: value out of range for " + k;
@@ -783,6 +788,9 @@ int sla_get_param(sla_handle_t h, const char *name, int *value)
     else if (k == "lookahead") *value = H->lookahead;
     else if (k == "update_cus") *value = H->update_cus;
     else if (k == "lacon_lapack") *value = H->lacon_lapack;
+    else if (k == "batched_w") *value = H->batched_w;
+    else if (k == "batched_wgs_per_cu") *value = H->batched_wgs_per_cu;
+    else if (k == "batched_graph") *value = H->batched_graph;
     else if (k == "gmres_tol_exp") *value = (int)std::lround(-std::log10(H->gmres_tol));
     else {
         g_last_error = "sla_get_param: unknown parameter " + k;
@@ -847,6 +855,64 @@ int sla_dgetrs_trans(sla_handle_t h, char trans, int n, int nrhs, const double *
     }
     SLA_TRY(sla::dgetrs_device(*reinterpret_cast<sla::Handle *>(h), n, nrhs, dA, lda, dipiv, dB, ldb,
                                !notran))
+}
+
+int sla_gesv_batched(sla_handle_t h, int prec, int batch, const int *n, const int *nrhs,
+                     void *const *dA, const int *lda, int *const *dipiv, void *const *dB,
+                     const int *ldb, int *dinfo)
+{
+    SLA_NEED_HANDLE(h);
+    if (prec != SLA_PREC_FP64 && prec != SLA_PREC_FP32) {
+        g_last_error = "sla_gesv_batched: prec must be SLA_PREC_FP64 or SLA_PREC_FP32";
+        return SLA_ERR_ARG;
+    }
+    if (batch < 0 || (batch > 0 && (!n || !nrhs || !lda || !ldb || !dA || !dipiv || !dB || !dinfo))) {
+        g_last_error = "sla_gesv_batched: illegal batch or null array";
+        return SLA_ERR_ARG;
+    }
+    for (int s = 0; s < batch; s++)
+        if (n[s] < 0 || n[s] > sla::BATCHED_NMAX || nrhs[s] < 0 || nrhs[s] > sla::BATCHED_NRHS_MAX ||
+            lda[s] < std::max(1, n[s]) || ldb[s] < std::max(1, n[s])) {
+            g_last_error = "sla_gesv_batched: system " + std::to_string(s) +
+                           ": n (<= 1024), nrhs (<= 16), lda or ldb out of range";
+            return SLA_ERR_ARG;
+        }
+    auto &H = *reinterpret_cast<sla::Handle *>(h);
+    if (prec == SLA_PREC_FP64)
+        SLA_TRY(sla::gesv_batched_device<double>(H, batch, n, nrhs,
+                                                 reinterpret_cast<double *const *>(dA), lda, dipiv,
+                                                 reinterpret_cast<double *const *>(dB), ldb, dinfo))
+    SLA_TRY(sla::gesv_batched_device<float>(H, batch, n, nrhs, reinterpret_cast<float *const *>(dA),
+                                            lda, dipiv, reinterpret_cast<float *const *>(dB), ldb,
+                                            dinfo))
+}
+
+int sla_gesv_batched_tune(sla_handle_t h, int prec, int batch, const int *n, const int *nrhs,
+                          int *w, int *wgs_per_cu, double *ms)
+{
+    SLA_NEED_HANDLE(h);
+    if ((prec != SLA_PREC_FP64 && prec != SLA_PREC_FP32) || batch < 1 || !n || !nrhs) {
+        g_last_error = "sla_gesv_batched_tune: illegal prec, batch or sizes";
+        return SLA_ERR_ARG;
+    }
+    for (int s = 0; s < batch; s++)
+        if (n[s] < 0 || n[s] > sla::BATCHED_NMAX || nrhs[s] < 0 || nrhs[s] > sla::BATCHED_NRHS_MAX) {
+            g_last_error = "sla_gesv_batched_tune: size out of range";
+            return SLA_ERR_ARG;
+        }
+    auto &H = *reinterpret_cast<sla::Handle *>(h);
+    try {
+        const sla::BatchedTune t = prec == SLA_PREC_FP64
+                                       ? sla::tune_batched_device<double>(H, batch, n, nrhs, 3)
+                                       : sla::tune_batched_device<float>(H, batch, n, nrhs, 3);
+        if (w) *w = t.w;
+        if (wgs_per_cu) *wgs_per_cu = t.wgs_per_cu;
+        if (ms) *ms = t.ms;
+        return SLA_OK;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return SLA_ERR_RUNTIME;
+    }
 }
 
 int sla_dlange(sla_handle_t h, char norm, int m, int n, const double *dA, int lda, double *value)

@@ -87,7 +87,8 @@ Handle::Handle()
 
 Handle::~Handle()
 {
-    hipStreamSynchronize(stream);
+    (void)hipStreamSynchronize(stream);
+    batched_plan_free(batched);
     for (void *p : {(void *)ctl, (void *)slot_cand_v, (void *)slot_cand_r, (void *)slot_rows,
                     (void *)slot_rowj, (void *)pairs, (void *)pair_counts, (void *)trsv_flags,
                     (void *)perm, (void *)work})

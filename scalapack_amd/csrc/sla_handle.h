@@ -65,6 +65,9 @@ struct Stat {
     long long launches = 0;
 };
 
+struct BatchedPlan;
+void batched_plan_free(BatchedPlan *p);
+
 struct Handle {
     Handle();
     ~Handle();
@@ -127,6 +130,11 @@ struct Handle {
     int num_cus = 256;     // CUs of the device
     int lacon_lapack = 0;  // condition / error estimates: 0 the reference's PDLACON result,
                            // 1 LAPACK DLACON's iteration (estimate kept across steps)
+    // batched solves
+    int batched_w = 16;           // panel width of the per-system factorization (8, 16, 32)
+    int batched_wgs_per_cu = 2;   // persistent workgroups per CU
+    int batched_graph = 0;        // 1: launch through a captured, replayed hipGraph
+    BatchedPlan *batched = nullptr;
     // mixed-precision refinement
     int ir_max_iter = 30;        // refinement steps before the FP64 fallback (DSGESV's ITERMAX)
     int ir_gmres_on_stall = 1;   // classical refinement that stalls continues as GMRES-IR

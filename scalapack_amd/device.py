@@ -141,6 +141,27 @@ class Handle:
                                     ldaf, _ptr(ipiv), _ptr(b), ldb, _ptr(x), ldx, fe, be))
         return list(fe[:nrhs]), list(be[:nrhs])
 
+    # ------------------------------------------------------------ batched
+    def gesv_batched(self, prec: str, n, nrhs, a_ptrs, lda, ipiv_ptrs, b_ptrs, ldb, info) -> None:
+        """Batched small solves: n / nrhs / lda / ldb are sequences of ints;
+        a_ptrs / ipiv_ptrs / b_ptrs are device int64 tensors of device
+        addresses; info a device int32 tensor (INFO per system)."""
+        batch = len(n)
+        arr = lambda v: (ctypes.c_int * max(1, batch))(*[int(x) for x in v])  # noqa: E731
+        _check(self._lib.sla_gesv_batched(self._h, PRECISIONS[prec], batch, arr(n), arr(nrhs),
+                                          _ptr(a_ptrs), arr(lda), _ptr(ipiv_ptrs), _ptr(b_ptrs),
+                                          arr(ldb), _ptr(info)))
+
+    def gesv_batched_tune(self, prec: str, n, nrhs) -> tuple[int, int, float]:
+        """Autotune (panel width, workgroups per CU) for these sizes; the
+        handle keeps the result.  Returns (w, wgs_per_cu, ms)."""
+        batch = len(n)
+        arr = lambda v: (ctypes.c_int * max(1, batch))(*[int(x) for x in v])  # noqa: E731
+        w, k, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _check(self._lib.sla_gesv_batched_tune(self._h, PRECISIONS[prec], batch, arr(n), arr(nrhs),
+                                               ctypes.byref(w), ctypes.byref(k), ctypes.byref(ms)))
+        return w.value, k.value, ms.value
+
     # ------------------------------------------------------- mixed precision
     def dxgesv(self, n: int, nrhs: int, a, lda: int, ipiv, b, ldb: int, x, ldx: int,
                prec: str = "fp32", refine: str = "ir") -> tuple[int, int]:
