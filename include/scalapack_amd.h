@@ -250,6 +250,41 @@ int sla_gesv_batched(sla_handle_t h, int prec, int batch, const int *n, const in
 int sla_gesv_batched_tune(sla_handle_t h, int prec, int batch, const int *n, const int *nrhs,
                           int *w, int *wgs_per_cu, double *ms);
 
+/* ------------------------------------------------------------------------ */
+/* Part 5: learned precision selector (no counterpart in the reference).      */
+/* ------------------------------------------------------------------------ */
+/* A network 12 -> 64 -> 64 -> 4 (tanh, softmax; FP32 weights) picks the
+ * factorization precision of sla_dxgesv -- class 0 FP8, 1 BF16, 2 FP32, 3 plain
+ * FP64 -- from 12 features of A computed on the GPU (norms, row / column
+ * scaling spread, sampled pivot growth and a PDGECON-style condition
+ * estimate of sampled principal submatrices).  Parameters: W1[64][12], b1,
+ * W2[64][64], b2, W3[4][64], b3, then the input normalisation mu[12],
+ * sigma[12]: SLA_MLP_NPARAM floats. */
+#define SLA_MLP_NPARAM 5276
+/* The 12 features of an n x n FP64 device matrix into feat (host float[12]). */
+int sla_selector_features(sla_handle_t h, int n, const double *dA, int lda, float *feat);
+/* Mean cost-weighted cross-entropy over `batch` samples (dX device
+ * float[batch][12] raw features, dy device int labels 0..3, dcost device
+ * float[4] class weights or NULL) and its gradient with respect to the
+ * trained parameters (the first SLA_MLP_NPARAM - 24 floats of dgrad). */
+int sla_mlp_grad(sla_handle_t h, int batch, const float *dX, const int *dy, const float *dcost,
+                 const float *dparams, float *dgrad, float *loss);
+/* Full-batch Adam (beta 0.9 / 0.999) for `epochs` steps on dparams in place;
+ * loss_history: host float[epochs] or NULL. */
+int sla_mlp_train(sla_handle_t h, int batch, const float *dX, const int *dy, const float *dcost,
+                  float *dparams, int epochs, float lr, float *loss_history);
+/* Classes (device int[batch], or NULL) and probabilities (device
+ * float[batch][4], or NULL). */
+int sla_mlp_predict(sla_handle_t h, int batch, const float *dX, const float *dparams, int *dpred,
+                    float *dprob);
+/* Install the selector's parameters (host float[SLA_MLP_NPARAM]). */
+int sla_selector_set_model(sla_handle_t h, const float *params);
+/* sla_dxgesv with the precision the selector picks (*choice, 0..3); the
+ * refinement and its FP64 fallback are sla_dxgesv's (SLA_REFINE_IR). */
+int sla_dxgesv_auto(sla_handle_t h, int n, int nrhs, double *dA, int lda, int *dipiv,
+                    const double *dB, int ldb, double *dX, int ldx, int *choice, int *iter,
+                    int *info);
+
 /* The low-precision trailing-update kernel alone: C -= A * B (FP32
  * column-major operands, products in `prec`, FP32 accumulation). */
 int sla_gemm_lowp_sub(sla_handle_t h, int prec, int m, int n, int k, const float *dA, int lda,

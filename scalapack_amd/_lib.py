@@ -73,6 +73,12 @@ def _declare(lib: ctypes.CDLL) -> None:
         "sla_dgerfs": [vp, ctypes.c_char, i, i, vp, i, vp, i, vp, vp, i, vp, i, vp, vp],
         "sla_gesv_batched": [vp, i, i, vp, vp, vp, vp, vp, vp, vp, vp],
         "sla_gesv_batched_tune": [vp, i, i, vp, vp, vp, vp, vp],
+        "sla_selector_features": [vp, i, vp, i, vp],
+        "sla_mlp_grad": [vp, i, vp, vp, vp, vp, vp, vp],
+        "sla_mlp_train": [vp, i, vp, vp, vp, vp, i, ctypes.c_float, vp],
+        "sla_mlp_predict": [vp, i, vp, vp, vp, vp],
+        "sla_selector_set_model": [vp, vp],
+        "sla_dxgesv_auto": [vp, i, i, vp, i, vp, vp, i, vp, i, vp, vp, vp],
     }
     for name, args in sigs.items():
         f = getattr(lib, name)

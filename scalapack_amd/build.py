@@ -25,7 +25,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SLA_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["gemm_f64.hip", "gemm_lowp.hip", "gemm_small.hip", "panel.hip", "panel_tslu.hip", "swap.hip",
-               "trsm.hip", "lu.hip", "mixed.hip", "aux.hip", "batched.hip", "capi.hip"]
+               "trsm.hip", "lu.hip", "mixed.hip", "aux.hip", "batched.hip", "mlp.hip", "capi.hip"]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}",

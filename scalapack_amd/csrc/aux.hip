@@ -28,6 +28,8 @@
 #include "sla_handle.h"
 #include "sla_kernels.h"
 #include "sla_aux.h"
+#include "sla_batched.h"
+#include "sla_mlp.h"
 
 #include <algorithm>
 #include <cctype>
@@ -37,6 +39,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 namespace sla {
 
@@ -135,12 +138,17 @@ __global__ __launch_bounds__(256) void lange_ssq_kernel(int m, int n, const doub
 }
 
 // ---- vector reductions (two stages, fixed order) ---------------------------
-enum RedOp { RED_SUM_ABS = 0, RED_MAX_ABS = 1, RED_SUM = 2 };
+enum RedOp { RED_SUM_ABS = 0, RED_MAX_ABS = 1, RED_SUM = 2, RED_MIN_ABS = 3 };
 
 template <int OP>
 __device__ __forceinline__ double red_combine(double a, double b)
 {
-    return OP == RED_MAX_ABS ? fmax(a, b) : a + b;
+    return OP == RED_MAX_ABS ? fmax(a, b) : OP == RED_MIN_ABS ? fmin(a, b) : a + b;
+}
+template <int OP>
+__device__ __forceinline__ double red_init()
+{
+    return OP == RED_MIN_ABS ? INFINITY : 0.0;
 }
 template <int OP>
 __device__ __forceinline__ double red_load(double v)
@@ -154,7 +162,7 @@ __global__ __launch_bounds__(256) void reduce1_kernel(int n, const double *__res
                                                       double *__restrict__ part)
 {
     __shared__ double ws[4];
-    double v = 0.0;
+    double v = red_init<OP>();
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
         v = red_combine<OP>(v, red_load<OP>(x[i]));
     for (int o = 32; o > 0; o >>= 1) v = red_combine<OP>(v, __shfl_xor(v, o));
@@ -169,7 +177,7 @@ __global__ __launch_bounds__(256) void reduce2_kernel(int nb, const double *__re
                                                       double alpha, double *out)
 {
     __shared__ double ws[4];
-    double v = 0.0;
+    double v = red_init<OP>();
     for (int i = threadIdx.x; i < nb; i += 256) v = red_combine<OP>(v, part[i]);
     for (int o = 32; o > 0; o >>= 1) v = red_combine<OP>(v, __shfl_xor(v, o));
     if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
@@ -365,6 +373,75 @@ __global__ void ferr_weights_kernel(int n, const double *__restrict__ r, double 
     w[i] = wi > safe2 ? fabs(r[i]) + nzeps * wi : fabs(r[i]) + nzeps * wi + safe1;
 }
 
+// ---- selector features: sampled principal submatrices ---------------------
+// Sample t (blockIdx.y) is A(idx_t, idx_t), k x k, copied twice (orig and
+// work, ld k); its right-hand side is the alternating-sign vector of
+// pdlacon.f's label 120.
+__global__ void gather_sample_kernel(const double *__restrict__ A, int lda, int k,
+                                     const int *__restrict__ idx, double *__restrict__ orig,
+                                     double *__restrict__ work, double *__restrict__ rhs)
+{
+    const int t = blockIdx.y;
+    const int *ix = idx + (size_t)t * k;
+    const size_t kk = (size_t)k * k;
+    for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < kk; e += (size_t)gridDim.x * 256) {
+        const int i = (int)(e % k), j = (int)(e / k);
+        const double v = A[(size_t)ix[i] + (size_t)ix[j] * lda];
+        orig[t * kk + e] = v;
+        work[t * kk + e] = v;
+    }
+    if (blockIdx.x == 0)
+        for (int i = threadIdx.x; i < k; i += 256)
+            rhs[(size_t)t * k + i] = ((i + 1) % 2 == 0 ? -1.0 : 1.0) *
+                                     (1.0 + (k > 1 ? (double)i / (double)(k - 1) : 0.0));
+}
+
+// Per sample (one block each): max |A_s|, ||A_s||_1, max |L\U|, max and min
+// |u_ii|, sum |x| (x = A_s^{-1} x_alt) into st[6 t ..].
+__global__ __launch_bounds__(256) void sample_stats_kernel(int k, const double *__restrict__ orig,
+                                                           const double *__restrict__ lu,
+                                                           const double *__restrict__ x,
+                                                           double *__restrict__ st)
+{
+    __shared__ double r[6][256];
+    const int t = blockIdx.x, tid = threadIdx.x;
+    const size_t kk = (size_t)k * k;
+    const double *a = orig + t * kk, *f = lu + t * kk, *xv = x + (size_t)t * k;
+    double mA = 0.0, n1 = 0.0, mLU = 0.0, dmax = 0.0, dmin = INFINITY, sx = 0.0;
+    for (size_t e = tid; e < kk; e += 256) {
+        mA = fmax(mA, fabs(a[e]));
+        mLU = fmax(mLU, fabs(f[e]));
+    }
+    for (int j = tid; j < k; j += 256) {
+        double c = 0.0;
+        for (int i = 0; i < k; i++) c += fabs(a[(size_t)i + (size_t)j * k]);
+        n1 = fmax(n1, c);
+        const double d = fabs(f[(size_t)j * (k + 1)]);
+        dmax = fmax(dmax, d);
+        dmin = fmin(dmin, d);
+        sx += fabs(xv[j]);
+    }
+    r[0][tid] = mA;
+    r[1][tid] = n1;
+    r[2][tid] = mLU;
+    r[3][tid] = dmax;
+    r[4][tid] = dmin;
+    r[5][tid] = sx;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) {
+            r[0][tid] = fmax(r[0][tid], r[0][tid + w]);
+            r[1][tid] = fmax(r[1][tid], r[1][tid + w]);
+            r[2][tid] = fmax(r[2][tid], r[2][tid + w]);
+            r[3][tid] = fmax(r[3][tid], r[3][tid + w]);
+            r[4][tid] = fmin(r[4][tid], r[4][tid + w]);
+            r[5][tid] += r[5][tid + w];
+        }
+        __syncthreads();
+    }
+    if (tid < 6) st[6 * t + tid] = r[tid][0];
+}
+
 unsigned blocks(long long n) { return (unsigned)std::max<long long>(1, (n + 255) / 256); }
 
 }  // namespace
@@ -423,6 +500,14 @@ struct AuxWork {
         const int nb = red_blocks(len);
         hipLaunchKernelGGL(reduce1_kernel<RED_MAX_ABS>, dim3(nb), dim3(256), 0, st(), len, y, red_v);
         hipLaunchKernelGGL(reduce2_kernel<RED_MAX_ABS>, dim3(1), dim3(256), 0, st(), nb, red_v, 1.0,
+                           res + slot);
+        SLA_LAUNCH_CHECK();
+    }
+    void minabs(const double *y, int len, int slot)
+    {
+        const int nb = red_blocks(len);
+        hipLaunchKernelGGL(reduce1_kernel<RED_MIN_ABS>, dim3(nb), dim3(256), 0, st(), len, y, red_v);
+        hipLaunchKernelGGL(reduce2_kernel<RED_MIN_ABS>, dim3(1), dim3(256), 0, st(), nb, red_v, 1.0,
                            res + slot);
         SLA_LAUNCH_CHECK();
     }
@@ -720,6 +805,130 @@ void dgerfs_device(Handle &h, bool trans, int n, int nrhs, const double *dA, int
         if (xmax != 0.0) ferr[k] = est / xmax;  // pdgerfs.f:655-664
     }
     check_solve_flags(h, "dgerfs");
+}
+
+// ============================================================================
+// Features of the learned precision selector (12 floats):
+//   0 log10 n
+//   1 log10 ||A||_1     2 log10 ||A||_inf     3 log10 max |a|
+//   4 log10 (||A||_F / sqrt n)
+//   5 log10 (max / min row sum of |A|)   6 log10 (max / min column sum)
+//   7 mean over samples of log10 (max |L\U| / max |A_s|)   (sampled growth)
+//   8 max over samples of log10 (max |u_ii| / min |u_ii|)
+//   9 mean over samples of log10 of the PDGECON-style estimate
+//     ||A_s||_1 * 2 ||A_s^{-1} x_alt||_1 / (3k) (pdlacon.f label 140)
+//  10 max over samples of the same
+//  11 log10 (n / k)
+// Samples: four principal k x k submatrices (k = min(n, 128); one when n <=
+// 128) on index sets drawn by a fixed LCG, factored and solved in one
+// batched launch.
+// ============================================================================
+namespace {
+double lg(double x) { return std::log10(std::min(std::max(x, 1e-300), 1e300)); }
+}  // namespace
+
+void selector_features_device(Handle &h, int n, const double *dA, int lda, float *f)
+{
+    if (n < 1) throw std::invalid_argument("selector features: n >= 1");
+    f[0] = (float)std::log10((double)n);
+    const double n1 = dlange_device(h, '1', n, n, dA, lda), ni = dlange_device(h, 'I', n, n, dA, lda);
+    const double nm = dlange_device(h, 'M', n, n, dA, lda), nf = dlange_device(h, 'F', n, n, dA, lda);
+    f[1] = (float)lg(n1);
+    f[2] = (float)lg(ni);
+    f[3] = (float)lg(nm);
+    f[4] = (float)lg(nf / std::sqrt((double)n));
+    {
+        AuxWork a(h, n, 0);
+        hipLaunchKernelGGL(lange_rowsum_kernel, dim3(ceil_div(n, 64)), dim3(64), 0, h.stream, n, n, dA, lda,
+                           a.x);
+        hipLaunchKernelGGL(lange_colsum_kernel, dim3(ceil_div(n, 64)), dim3(64), 0, h.stream, n, n, dA, lda,
+                           a.v);
+        SLA_LAUNCH_CHECK();
+        a.maxabs(a.x, n, 0);
+        a.minabs(a.x, n, 1);
+        a.maxabs(a.v, n, 2);
+        a.minabs(a.v, n, 3);
+        double r[4];
+        a.fetch(r, a.res, sizeof r);
+        f[5] = (float)(lg(r[0]) - lg(r[1]));
+        f[6] = (float)(lg(r[2]) - lg(r[3]));
+    }
+    const int k = std::min(n, 128), S = n > 128 ? 4 : 1;
+    const size_t kk = (size_t)k * k;
+    const size_t bytes = 2 * S * kk * 8 + S * (size_t)k * 8 + S * (size_t)k * 4 + S * 4 +
+                         S * (size_t)k * 4 + 3 * S * sizeof(void *) + S * 6 * 8 + 1024;
+    char *buf = static_cast<char *>(h.buffer(BUF_SELECT, bytes));
+    double *orig = reinterpret_cast<double *>(buf);
+    double *work = orig + S * kk;
+    double *rhs = work + S * kk;
+    double *stats = rhs + (size_t)S * k;
+    int *piv = reinterpret_cast<int *>(stats + 6 * S);
+    int *info = piv + (size_t)S * k;
+    int *idx = info + S;
+    void **ptrs = reinterpret_cast<void **>((reinterpret_cast<uintptr_t>(idx + (size_t)S * k) + 15) & ~uintptr_t(15));
+    // index sets: partial Fisher-Yates with a fixed LCG, sorted
+    std::vector<int> hidx((size_t)S * k);
+    {
+        std::vector<int> perm(n);
+        for (int t = 0; t < S; t++) {
+            for (int i = 0; i < n; i++) perm[i] = i;
+            uint64_t st = 0x9E3779B97F4A7C15ull * (uint64_t)(t + 1) + (uint64_t)n;
+            for (int i = 0; i < k; i++) {
+                st = st * 6364136223846793005ull + 1442695040888963407ull;
+                const int j = i + (int)((st >> 33) % (uint64_t)(n - i));
+                std::swap(perm[i], perm[j]);
+            }
+            std::sort(perm.begin(), perm.begin() + k);
+            std::copy(perm.begin(), perm.begin() + k, hidx.begin() + (size_t)t * k);
+        }
+    }
+    std::vector<void *> hp(3 * (size_t)S);
+    for (int t = 0; t < S; t++) {
+        hp[t] = work + t * kk;
+        hp[S + t] = piv + (size_t)t * k;
+        hp[2 * S + t] = rhs + (size_t)t * k;
+    }
+    SLA_HIP(hipMemcpyAsync(idx, hidx.data(), sizeof(int) * hidx.size(), hipMemcpyHostToDevice, h.stream));
+    SLA_HIP(hipMemcpyAsync(ptrs, hp.data(), sizeof(void *) * hp.size(), hipMemcpyHostToDevice, h.stream));
+    hipLaunchKernelGGL(gather_sample_kernel, dim3(std::max(1, (int)std::min<size_t>(64, (kk + 255) / 256)), S),
+                       dim3(256), 0, h.stream, dA, lda, k, idx, orig, work, rhs);
+    SLA_LAUNCH_CHECK();
+    std::vector<int> nv(S, k), rv(S, 1);
+    const int save_graph = h.batched_graph;
+    h.batched_graph = 0;
+    try {
+        gesv_batched_device<double>(h, S, nv.data(), rv.data(), reinterpret_cast<double *const *>(ptrs),
+                                    nv.data(), reinterpret_cast<int *const *>(ptrs + S),
+                                    reinterpret_cast<double *const *>(ptrs + 2 * S), nv.data(), info);
+    } catch (...) {
+        h.batched_graph = save_graph;
+        throw;
+    }
+    h.batched_graph = save_graph;
+    hipLaunchKernelGGL(sample_stats_kernel, dim3(S), dim3(256), 0, h.stream, k, orig, work, rhs, stats);
+    SLA_LAUNCH_CHECK();
+    std::vector<double> hs(6 * (size_t)S);
+    std::vector<int> hinfo(S);
+    SLA_HIP(hipMemcpyAsync(hs.data(), stats, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, h.stream));
+    SLA_HIP(hipMemcpyAsync(hinfo.data(), info, sizeof(int) * S, hipMemcpyDeviceToHost, h.stream));
+    SLA_HIP(hipStreamSynchronize(h.stream));
+    double g_mean = 0.0, d_max = -1e300, c_mean = 0.0, c_max = -1e300;
+    for (int t = 0; t < S; t++) {
+        const double *q = &hs[6 * (size_t)t];
+        const double growth = lg(q[2]) - lg(q[0]);
+        const double dr = lg(q[3]) - lg(q[4]);
+        // a sample with an exactly zero pivot has no finite estimate: 1e20
+        const double c = hinfo[t] != 0 ? 20.0 : std::min(20.0, lg(q[1] * 2.0 * q[5] / (3.0 * k)));
+        g_mean += growth / S;
+        d_max = std::max(d_max, std::min(20.0, dr));
+        c_mean += c / S;
+        c_max = std::max(c_max, c);
+    }
+    f[7] = (float)g_mean;
+    f[8] = (float)d_max;
+    f[9] = (float)c_mean;
+    f[10] = (float)c_max;
+    f[11] = (float)std::log10((double)n / (double)k);
 }
 
 }  // namespace sla

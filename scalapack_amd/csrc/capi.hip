@@ -14,6 +14,7 @@
 #include "sla_mixed.h"
 #include "sla_aux.h"
 #include "sla_batched.h"
+#include "sla_mlp.h"
 
 #include "scalapack_amd.h"
 #include "sla_blacs1x1.h"
@@ -1018,6 +1019,117 @@ int sla_dxgesv(sla_handle_t h, int prec, int refine, int n, int nrhs, double *dA
         auto &H = *reinterpret_cast<sla::Handle *>(h);
         sla::MixedInfo mi;
         *info = sla::dxgesv_device(H, prec, refine, n, nrhs, dA, lda, dipiv, dB, ldb, dX, ldx, mi);
+        *iter = mi.iter;
+        H.last_mixed = mi;
+    })
+}
+
+// ---------------------------------------------------------------- selector
+static_assert(sla::MLP_NPARAM == SLA_MLP_NPARAM, "parameter count of the selector network");
+
+// The selector's class -> factorization precision of sla_dxgesv.
+static int selector_precision(int choice)
+{
+    switch (choice) {
+    case 0: return SLA_PREC_FP8;
+    case 1: return SLA_PREC_BF16;
+    case 2: return SLA_PREC_FP32;
+    default: return SLA_PREC_FP64;
+    }
+}
+
+int sla_selector_features(sla_handle_t h, int n, const double *dA, int lda, float *feat)
+{
+    SLA_NEED_HANDLE(h);
+    if (n < 1 || lda < n || !dA || !feat) {
+        g_last_error = "sla_selector_features: illegal n, lda or pointer";
+        return SLA_ERR_ARG;
+    }
+    SLA_TRY(sla::selector_features_device(*reinterpret_cast<sla::Handle *>(h), n, dA, lda, feat))
+}
+
+int sla_mlp_grad(sla_handle_t h, int batch, const float *dX, const int *dy, const float *dcost,
+                 const float *dparams, float *dgrad, float *loss)
+{
+    SLA_NEED_HANDLE(h);
+    if (batch < 1 || !dX || !dy || !dparams || !dgrad || !loss) {
+        g_last_error = "sla_mlp_grad: illegal batch or pointer";
+        return SLA_ERR_ARG;
+    }
+    SLA_TRY(*loss = sla::mlp_grad_device(*reinterpret_cast<sla::Handle *>(h), batch, dX, dy, dcost,
+                                         dparams, dgrad))
+}
+
+int sla_mlp_train(sla_handle_t h, int batch, const float *dX, const int *dy, const float *dcost,
+                  float *dparams, int epochs, float lr, float *loss_history)
+{
+    SLA_NEED_HANDLE(h);
+    if (batch < 1 || epochs < 0 || !(lr > 0.f) || !dX || !dy || !dparams) {
+        g_last_error = "sla_mlp_train: illegal batch, epochs, lr or pointer";
+        return SLA_ERR_ARG;
+    }
+    SLA_TRY(sla::mlp_train_device(*reinterpret_cast<sla::Handle *>(h), batch, dX, dy, dcost, dparams,
+                                  epochs, lr, loss_history))
+}
+
+int sla_mlp_predict(sla_handle_t h, int batch, const float *dX, const float *dparams, int *dpred,
+                    float *dprob)
+{
+    SLA_NEED_HANDLE(h);
+    if (batch < 0 || (batch > 0 && (!dX || !dparams))) {
+        g_last_error = "sla_mlp_predict: illegal batch or pointer";
+        return SLA_ERR_ARG;
+    }
+    SLA_TRY(sla::mlp_predict_device(*reinterpret_cast<sla::Handle *>(h), batch, dX, dparams, dpred,
+                                    dprob))
+}
+
+int sla_selector_set_model(sla_handle_t h, const float *params)
+{
+    SLA_NEED_HANDLE(h);
+    if (!params) {
+        g_last_error = "sla_selector_set_model: null parameters";
+        return SLA_ERR_ARG;
+    }
+    SLA_TRY({
+        auto &H = *reinterpret_cast<sla::Handle *>(h);
+        for (int i = 0; i < sla::MLP_IN; i++)
+            if (!(params[sla::MLP_NPARAM - sla::MLP_IN + i] > 0.f))
+                throw std::invalid_argument("sla_selector_set_model: normalisation sigma must be > 0");
+        float *d = static_cast<float *>(H.buffer(sla::BUF_MODEL, 4 * (sla::MLP_NPARAM + 64)));
+        SLA_HIP(hipMemcpyAsync(d, params, 4 * sla::MLP_NPARAM, hipMemcpyHostToDevice, H.stream));
+        SLA_HIP(hipStreamSynchronize(H.stream));
+        H.has_model = true;
+    })
+}
+
+int sla_dxgesv_auto(sla_handle_t h, int n, int nrhs, double *dA, int lda, int *dipiv,
+                    const double *dB, int ldb, double *dX, int ldx, int *choice, int *iter,
+                    int *info)
+{
+    SLA_NEED_HANDLE(h);
+    if (n < 1 || nrhs < 0 || nrhs > 16 || lda < n || ldb < n || ldx < n || !choice || !iter ||
+        !info) {
+        g_last_error = "sla_dxgesv_auto: illegal n, nrhs (<= 16), lda, ldb, ldx or output";
+        return SLA_ERR_ARG;
+    }
+    auto &H = *reinterpret_cast<sla::Handle *>(h);
+    if (!H.has_model) {
+        g_last_error = "sla_dxgesv_auto: no selector model (sla_selector_set_model)";
+        return SLA_ERR_ARG;
+    }
+    SLA_TRY({
+        float feat[sla::MLP_IN];
+        sla::selector_features_device(H, n, dA, lda, feat);
+        float *d = static_cast<float *>(H.buffer(sla::BUF_MODEL, 4 * (sla::MLP_NPARAM + 64)));
+        float *df = d + sla::MLP_NPARAM;
+        int *dc = reinterpret_cast<int *>(df + 16);
+        SLA_HIP(hipMemcpyAsync(df, feat, sizeof feat, hipMemcpyHostToDevice, H.stream));
+        sla::mlp_predict_device(H, 1, df, d, dc, nullptr);
+        SLA_HIP(hipMemcpy(choice, dc, sizeof(int), hipMemcpyDeviceToHost));
+        sla::MixedInfo mi;
+        *info = sla::dxgesv_device(H, selector_precision(*choice), SLA_REFINE_IR, n, nrhs, dA, lda,
+                                   dipiv, dB, ldb, dX, ldx, mi);
         *iter = mi.iter;
         H.last_mixed = mi;
     })

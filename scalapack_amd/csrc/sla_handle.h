@@ -56,6 +56,9 @@ enum BufId {
     BUF_PARTIAL,     // partial sums of the residual / reductions
     BUF_TSLU,        // candidate exchange of the tournament-pivoting leaf
     BUF_AUX,         // vectors and partial sums of the norms / estimators / PDGERFS
+    BUF_MLP,         // activations and gradients of the selector network
+    BUF_SELECT,      // sampled submatrices of the selector's features
+    BUF_MODEL,       // the selector's parameters (sla_selector_set_model)
     BUF_COUNT
 };
 
@@ -135,6 +138,7 @@ struct Handle {
     int batched_wgs_per_cu = 2;   // persistent workgroups per CU
     int batched_graph = 0;        // 1: launch through a captured, replayed hipGraph
     BatchedPlan *batched = nullptr;
+    bool has_model = false;       // a selector model is in BUF_MODEL
     // mixed-precision refinement
     int ir_max_iter = 30;        // refinement steps before the FP64 fallback (DSGESV's ITERMAX)
     int ir_gmres_on_stall = 1;   // classical refinement that stalls continues as GMRES-IR

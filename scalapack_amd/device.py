@@ -162,6 +162,49 @@ class Handle:
                                                ctypes.byref(w), ctypes.byref(k), ctypes.byref(ms)))
         return w.value, k.value, ms.value
 
+    # ----------------------------------------------------- learned selector
+    def selector_features(self, n: int, a, lda: int):
+        """The selector's 12 features of an n x n FP64 device matrix."""
+        out = (ctypes.c_float * 12)()
+        _check(self._lib.sla_selector_features(self._h, n, _ptr(a), lda, out))
+        return list(out)
+
+    def mlp_grad(self, x, y, params, grad, cost=None) -> float:
+        """Loss and gradient (into grad) of the selector network; device tensors."""
+        loss = ctypes.c_float()
+        _check(self._lib.sla_mlp_grad(self._h, x.shape[0], _ptr(x), _ptr(y),
+                                      _ptr(cost) if cost is not None else None, _ptr(params),
+                                      _ptr(grad), ctypes.byref(loss)))
+        return loss.value
+
+    def mlp_train(self, x, y, params, epochs: int, lr: float, cost=None) -> list[float]:
+        """Full-batch Adam on params (device, in place); returns the loss history."""
+        hist = (ctypes.c_float * max(1, epochs))()
+        _check(self._lib.sla_mlp_train(self._h, x.shape[0], _ptr(x), _ptr(y),
+                                       _ptr(cost) if cost is not None else None, _ptr(params),
+                                       epochs, ctypes.c_float(lr), hist))
+        return list(hist[:epochs])
+
+    def mlp_predict(self, x, params, pred=None, prob=None) -> None:
+        _check(self._lib.sla_mlp_predict(self._h, x.shape[0], _ptr(x), _ptr(params),
+                                         _ptr(pred) if pred is not None else None,
+                                         _ptr(prob) if prob is not None else None))
+
+    def selector_set_model(self, params) -> None:
+        """params: a host float32 array of SLA_MLP_NPARAM values."""
+        import numpy as np
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        _check(self._lib.sla_selector_set_model(self._h, ctypes.c_void_p(p.ctypes.data)))
+
+    def dxgesv_auto(self, n: int, nrhs: int, a, lda: int, ipiv, b, ldb: int, x,
+                    ldx: int) -> tuple[int, int, int]:
+        """sla_dxgesv with the selector's precision: (choice, iter, info)."""
+        c, it, info = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(self._lib.sla_dxgesv_auto(self._h, n, nrhs, _ptr(a), lda, _ptr(ipiv), _ptr(b), ldb,
+                                         _ptr(x), ldx, ctypes.byref(c), ctypes.byref(it),
+                                         ctypes.byref(info)))
+        return c.value, it.value, info.value
+
     # ------------------------------------------------------- mixed precision
     def dxgesv(self, n: int, nrhs: int, a, lda: int, ipiv, b, ldb: int, x, ldx: int,
                prec: str = "fp32", refine: str = "ir") -> tuple[int, int]:
