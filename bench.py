@@ -74,6 +74,9 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true", help="mixed mode: skip the FP64 comparison")
     ap.add_argument("--param", action="append", default=[],
                     help="handle knob name=value (sla_set_param), may repeat")
+    ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="GIB",
+                    help="only copy GIB GiB device-to-device 4 times (the known-size kernel that "
+                         "scales the FETCH_SIZE / WRITE_SIZE counters) and exit")
     return ap.parse_args()
 
 
@@ -331,6 +334,17 @@ def main() -> int:
     import torch
 
     from scalapack_amd.device import Handle
+
+    if args.calibrate_copy > 0:
+        nbytes = int(args.calibrate_copy * 2 ** 30)
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(4):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": nbytes, "copies": 4,
+                          "kib_per_copy": nbytes / 1024}), flush=True)
+        return 0
 
     n, nrhs, nb = args.n, args.nrhs, args.nb
     dev = torch.device("cuda:0")
