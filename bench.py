@@ -140,7 +140,22 @@ KERNEL_OF = {  # kernel symbol behind each timing class (for matching kernel tra
 }
 
 
-def roofline_from(stats: dict, steps: int, gemm_prec: str) -> dict:
+def measured_traffic(kernel: str, workload: dict):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (profiles/traffic.json: rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this
+    workload, scaled by a copy of known size), or None when that kernel and
+    workload were not measured."""
+    path = ROOT / "profiles" / "traffic.json"
+    if not path.exists():
+        return None
+    table = json.loads(path.read_text())
+    for sym, rec in table.items():
+        if kernel.startswith(sym) and all(workload.get(k) == v for k, v in rec["workload"].items()):
+            return rec["bytes_per_launch"]
+    return None
+
+
+def roofline_from(stats: dict, steps: int, gemm_prec: str, workload: dict | None = None) -> dict:
     """Roofline of the kernel class with the most summed time."""
     cls, st = max(stats.items(), key=lambda kv: kv[1]["ms"])
     secs = st["ms"] / 1e3
@@ -163,6 +178,8 @@ def roofline_from(stats: dict, steps: int, gemm_prec: str) -> dict:
         out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None}
     out.update(base)
+    if workload is not None:
+        out["traffic"] = measured_traffic(base["kernel"], workload)
     return out
 
 
@@ -380,7 +397,8 @@ def main() -> int:
                        "precision": "fp64", "params": args.param},
             "scaled_residual": max(sres),
             "kernel_ms_per_step": {k: round(v["ms"] / args.steps, 2) for k, v in stats.items()},
-            "roofline": roofline_from(stats, args.steps, "fp64"),
+            "roofline": roofline_from(stats, args.steps, "fp64",
+                                      {"mode": "fp64", "n": n, "nb": nb}),
         })
         ok = max(sres) < 16.0
     else:
@@ -427,7 +445,8 @@ def main() -> int:
                              "kernel_ms_per_step": {k: round(v["ms"] / args.steps, 2)
                                                     for k, v in fstats.items()},
                              "gemm_roofline": roofline_from({"gemm": fstats["gemm"]},
-                                                            args.steps, "fp64")}
+                                                            args.steps, "fp64",
+                                                            {"mode": "fp64", "n": n, "nb": nb})}
             del AT, B0
             t_gen = time.perf_counter()
         ms = 1e3 * total_t / total_steps
@@ -447,7 +466,8 @@ def main() -> int:
             "fp64_lu": fp64_line,
             "speedup_vs_fp64_lu": (round(value / fp64_line["tflops_equiv"], 3)
                                    if fp64_line else None),
-            "roofline": roofline_from(all_stats, total_steps, args.prec),
+            "roofline": roofline_from(all_stats, total_steps, args.prec,
+                                      {"mode": "mixed", "n": n, "nb": nb, "prec": args.prec}),
         })
     line["cpu_baseline"] = None if args.no_cpu else cpu_baseline(args.cpu_n)
     print(json.dumps(line), flush=True)

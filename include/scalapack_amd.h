@@ -87,6 +87,31 @@ void pdgerfs_(const char *trans, const int *n, const int *nrhs, const double *a,
               double *ferr, double *berr, double *work, const int *lwork, int *iwork,
               const int *liwork, int *info, size_t trans_len);
 
+/* PDGEEQU: row and column scalings that equilibrate sub(A).  Replaces
+ * SRC/pdgeequ.f:1-2; R(IA..IA+M-1), C(JA..JA+N-1); INFO = i (row i of sub(A)
+ * exactly zero) or M + j (column j). */
+void pdgeequ_(const int *m, const int *n, const double *a, const int *ia, const int *ja,
+              const int *desca, double *r, double *c, double *rowcnd, double *colcnd, double *amax,
+              int *info);
+
+/* PDLAQGE: applies the scalings of PDGEEQU when they are needed (THRESH =
+ * 0.1).  Replaces SRC/pdlaqge.f:1-2; EQUED = 'N', 'R', 'C' or 'B'. */
+void pdlaqge_(const int *m, const int *n, double *a, const int *ia, const int *ja, const int *desca,
+              const double *r, const double *c, const double *rowcnd, const double *colcnd,
+              const double *amax, char *equed, size_t equed_len);
+
+/* PDGESVX: the expert driver -- equilibration (FACT = 'E'), PDGETRF (unless
+ * FACT = 'F'), PDGECON, PDGETRS, PDGERFS and the scaling of X and FERR back.
+ * Replaces SRC/pdgesvx.f:1-5 (checks :452-620, steps :624-780); INFO = IA+N
+ * when RCOND < eps (X not computed). */
+void pdgesvx_(const char *fact, const char *trans, const int *n, const int *nrhs, double *a,
+              const int *ia, const int *ja, const int *desca, double *af, const int *iaf,
+              const int *jaf, const int *descaf, int *ipiv, char *equed, double *r, double *c,
+              double *b, const int *ib, const int *jb, const int *descb, double *x, const int *ix,
+              const int *jx, const int *descx, double *rcond, double *ferr, double *berr,
+              double *work, const int *lwork, int *iwork, const int *liwork, int *info,
+              size_t fact_len, size_t trans_len, size_t equed_len);
+
 /* ------------------------------------------------------------------------ */
 /* Part 2: device-resident interface.                                        */
 /* ------------------------------------------------------------------------ */
@@ -147,6 +172,16 @@ int sla_dgesv(sla_handle_t h, int n, int nrhs, double *dA, int lda, int *dipiv, 
  * sla_dgetrf (PDGETRS with TRANS, SRC/pdgetrs.f:255-284). */
 int sla_dgetrs_trans(sla_handle_t h, char trans, int n, int nrhs, const double *dA, int lda,
                      const int *dipiv, double *dB, int ldb);
+/* PDGEEQU / PDLAQGE / PDGESVX on device data (dR, dC: device doubles). */
+int sla_dgeequ(sla_handle_t h, int m, int n, const double *dA, int lda, double *dR, double *dC,
+               double *rowcnd, double *colcnd, double *amax, int *info);
+int sla_dlaqge(sla_handle_t h, int m, int n, double *dA, int lda, const double *dR,
+               const double *dC, double rowcnd, double colcnd, double amax, char *equed);
+/* fact 'N', 'E' or 'F'; *equed in ('F') / out; ferr, berr host arrays;
+ * *info = n + 1 when RCOND < eps. */
+int sla_dgesvx(sla_handle_t h, char fact, char trans, int n, int nrhs, double *dA, int lda,
+               double *dAF, int ldaf, int *dipiv, char *equed, double *dR, double *dC, double *dB,
+               int ldb, double *dX, int ldx, double *rcond, double *ferr, double *berr, int *info);
 /* PDLANGE of an m x n device matrix (SRC/pdlange.f:196-330); norm one of
  * 'M', '1', 'O', 'I', 'F', 'E'. */
 int sla_dlange(sla_handle_t h, char norm, int m, int n, const double *dA, int lda,

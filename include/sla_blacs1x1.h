@@ -53,6 +53,19 @@ int indxl2g_(const int *indxloc, const int *nb, const int *iproc, const int *isr
 /* TOOLS/iceil.f:31 */
 int iceil_(const int *inum, const int *idenom);
 
+/* PDLAWRITE (TOOLS/pdlawrite.f:1-2): sub(A) to FILNAM as the reference
+ * writes it -- a list-directed header " M N", then the values down the
+ * columns one per record in D30.18 (pdlawrite.f:217).  IRWRIT / ICWRIT and
+ * WORK are not needed on a 1 x 1 grid. */
+void pdlawrite_(const char *filnam, const int *m, const int *n, const double *a, const int *ia,
+                const int *ja, const int *desca, const int *irwrit, const int *icwrit, double *work,
+                size_t filnam_len);
+/* PDLAREAD (TOOLS/pdlaread.f:1): the matrix of FILNAM ("M N" header, values
+ * down the columns, one per record, D or E exponents) into A(1:M, 1:N);
+ * aborts when M > M_A or N > N_A as the reference does. */
+void pdlaread_(const char *filnam, double *a, const int *desca, const int *irread, const int *icread,
+               double *work, size_t filnam_len);
+
 #ifdef __cplusplus
 }
 #endif

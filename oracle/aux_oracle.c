@@ -330,3 +330,129 @@ int oracle_pdgerfs(char trans, int n, int nrhs, const double *a, int lda, const 
     free(w);
     return 0;
 }
+
+/* PDGEEQU, SRC/pdgeequ.f (1 x 1, IA = JA = 1): row scales R, column scales
+ * C, ROWCND, COLCND, AMAX; INFO = i (row i exactly zero) or M + j. */
+int oracle_pdgeequ(int m, int n, const double *a, int lda, double *r, double *c, double *rowcnd,
+                   double *colcnd, double *amax)
+{
+    const double smlnum = DBL_MIN, bignum = 1.0 / smlnum;
+    if (m == 0 || n == 0) {
+        *rowcnd = 1.0;
+        *colcnd = 1.0;
+        *amax = 0.0;
+        return 0;
+    }
+    for (int i = 0; i < m; i++) r[i] = 0.0;
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < m; i++) r[i] = fmax(r[i], fabs(a[i + (size_t)j * lda]));
+    double rcmin = bignum, rcmax = 0.0;
+    for (int i = 0; i < m; i++) {
+        rcmax = fmax(rcmax, r[i]);
+        rcmin = fmin(rcmin, r[i]);
+    }
+    *amax = rcmax;
+    if (rcmin == 0.0) {
+        for (int i = 0; i < m; i++)
+            if (r[i] == 0.0) return i + 1;
+    }
+    for (int i = 0; i < m; i++) r[i] = 1.0 / fmin(fmax(r[i], smlnum), bignum);
+    *rowcnd = fmax(rcmin, smlnum) / fmin(rcmax, bignum);
+    for (int j = 0; j < n; j++) {
+        c[j] = 0.0;
+        for (int i = 0; i < m; i++) c[j] = fmax(c[j], fabs(a[i + (size_t)j * lda]) * r[i]);
+    }
+    rcmin = bignum;
+    rcmax = 0.0;
+    for (int j = 0; j < n; j++) {
+        rcmin = fmin(rcmin, c[j]);
+        rcmax = fmax(rcmax, c[j]);
+    }
+    if (rcmin == 0.0) {
+        for (int j = 0; j < n; j++)
+            if (c[j] == 0.0) return m + j + 1;
+    }
+    for (int j = 0; j < n; j++) c[j] = 1.0 / fmin(fmax(c[j], smlnum), bignum);
+    *colcnd = fmax(rcmin, smlnum) / fmin(rcmax, bignum);
+    return 0;
+}
+
+/* PDLAQGE, SRC/pdlaqge.f:205-262 (THRESH = 0.1, SMALL = SAFMIN / PRECISION
+ * with PRECISION = EPS * BASE = 2^-52); returns EQUED. */
+char oracle_pdlaqge(int m, int n, double *a, int lda, const double *r, const double *c,
+                    double rowcnd, double colcnd, double amax)
+{
+    const double thresh = 0.1, small = DBL_MIN / ldexp(1.0, -52), large = 1.0 / small;
+    if (m <= 0 || n <= 0) return 'N';
+    if (rowcnd >= thresh && amax >= small && amax <= large) {
+        if (colcnd >= thresh) return 'N';
+        for (int j = 0; j < n; j++)
+            for (int i = 0; i < m; i++) a[i + (size_t)j * lda] = c[j] * a[i + (size_t)j * lda];
+        return 'C';
+    }
+    if (colcnd >= thresh) {
+        for (int j = 0; j < n; j++)
+            for (int i = 0; i < m; i++) a[i + (size_t)j * lda] = r[i] * a[i + (size_t)j * lda];
+        return 'R';
+    }
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < m; i++) a[i + (size_t)j * lda] = c[j] * r[i] * a[i + (size_t)j * lda];
+    return 'B';
+}
+
+/* PDGESVX, SRC/pdgesvx.f:624-737 for FACT = 'E' or 'N' (IA = JA = IB = JB =
+ * IX = JX = 1, NB the block size of PDGETRF): equilibrate, factor, RCOND by
+ * PDGECON, INFO = N + 1 when RCOND < EPS, solve, refine, undo the scaling
+ * of X and FERR.  A, B are overwritten as the reference does. */
+int oracle_pdgesvx(char fact, char trans, int n, int nrhs, int nb, double *a, double *af,
+                   int *ipiv, char *equed, double *r, double *c, double *b, double *x,
+                   double *rcond, double *ferr, double *berr)
+{
+    const int notran = up(trans) == 'N';
+    int rowequ = 0, colequ = 0;
+    double rowcnd = 1.0, colcnd = 1.0, amax = 0.0;
+    *equed = 'N';
+    if (up(fact) == 'E') {
+        if (oracle_pdgeequ(n, n, a, n, r, c, &rowcnd, &colcnd, &amax) == 0) {
+            *equed = oracle_pdlaqge(n, n, a, n, r, c, rowcnd, colcnd, amax);
+            rowequ = *equed == 'R' || *equed == 'B';
+            colequ = *equed == 'C' || *equed == 'B';
+        }
+    }
+    if (notran) {
+        if (rowequ)
+            for (int j = 0; j < nrhs; j++)
+                for (int i = 0; i < n; i++) b[i + (size_t)j * n] = r[i] * b[i + (size_t)j * n];
+    } else if (colequ) {
+        for (int j = 0; j < nrhs; j++)
+            for (int i = 0; i < n; i++) b[i + (size_t)j * n] = c[i] * b[i + (size_t)j * n];
+    }
+    int desca[9] = {1, 0, n, n, nb, nb, 0, 0, n > 1 ? n : 1};
+    memcpy(af, a, (size_t)n * n * sizeof(double));
+    int info = oracle_pdgetrf(n, n, af, 1, 1, desca, ipiv, NULL);
+    if (info != 0) {
+        if (info > 0) *rcond = 0.0;
+        return info;
+    }
+    const char norm = notran ? '1' : 'I';
+    const double anorm = oracle_pdlange(norm, n, n, a, 1, 1, desca);
+    *rcond = oracle_pdgecon(norm, n, af, 1, 1, desca, anorm);
+    if (*rcond < ldexp(1.0, -53)) return n + 1;
+    memcpy(x, b, (size_t)n * nrhs * sizeof(double));
+    int descb[9] = {1, 0, n, nrhs, nb, nb, 0, 0, n > 1 ? n : 1};
+    if (notran) oracle_pdgetrs('N', n, nrhs, af, 1, 1, desca, ipiv, x, 1, 1, descb);
+    else oracle_pdgetrs_t(n, nrhs, af, 1, 1, desca, ipiv, x, 1, 1, descb);
+    oracle_pdgerfs(trans, n, nrhs, a, n, af, n, ipiv, b, n, x, n, ferr, berr);
+    if (notran) {
+        if (colequ) {
+            for (int j = 0; j < nrhs; j++)
+                for (int i = 0; i < n; i++) x[i + (size_t)j * n] = c[i] * x[i + (size_t)j * n];
+            for (int j = 0; j < nrhs; j++) ferr[j] = ferr[j] / colcnd;
+        }
+    } else if (rowequ) {
+        for (int j = 0; j < nrhs; j++)
+            for (int i = 0; i < n; i++) x[i + (size_t)j * n] = r[i] * x[i + (size_t)j * n];
+        for (int j = 0; j < nrhs; j++) ferr[j] = ferr[j] / rowcnd;
+    }
+    return 0;
+}

@@ -21,6 +21,8 @@
  *   oracle_pdgecon  SRC/pdgecon.f:287-403 with SRC/pdlacon.f
  *   oracle_pdgerfs  SRC/pdgerfs.f:440-675 (PDAGEMV as PTZBLAS/dagemv.f)
  *   oracle_pdgetrs_t SRC/pdgetrs.f:273-284       (TRANS = 'T')
+ *   oracle_pdgeequ  SRC/pdgeequ.f; oracle_pdlaqge SRC/pdlaqge.f:205-262
+ *   oracle_pdgesvx  SRC/pdgesvx.f:624-737 (FACT = 'E' or 'N')
  * Arithmetic goes through the host Fortran BLAS (dgemm_, dtrsm_, idamax_,
  * dswap_, dscal_, dger_) in the order the reference issues those calls, with
  * PDTRSM's 32-row blocking (PBLAS/SRC/PTOOLS/PB_CptrsmAB.c:301-412,
@@ -68,6 +70,15 @@ int oracle_pdgetrs_t(int n, int nrhs, const double *a, int ia, int ja, const int
 int oracle_pdgerfs(char trans, int n, int nrhs, const double *a, int lda, const double *af,
                    int ldaf, const int *ipiv, const double *b, int ldb, double *x, int ldx,
                    double *ferr, double *berr);
+
+/* PDGEEQU / PDLAQGE / PDGESVX (aux_oracle.c), IA = JA = 1. */
+int oracle_pdgeequ(int m, int n, const double *a, int lda, double *r, double *c, double *rowcnd,
+                   double *colcnd, double *amax);
+char oracle_pdlaqge(int m, int n, double *a, int lda, const double *r, const double *c,
+                    double rowcnd, double colcnd, double amax);
+int oracle_pdgesvx(char fact, char trans, int n, int nrhs, int nb, double *a, double *af,
+                   int *ipiv, char *equed, double *r, double *c, double *b, double *x,
+                   double *rcond, double *ferr, double *berr);
 
 double oracle_norm_inf(int m, int n, const double *a, int lda);
 double oracle_sresid(int n, int nrhs, const double *a, int lda, const double *x,

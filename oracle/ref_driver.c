@@ -15,6 +15,8 @@
  *
  * usage: ref_lu M N NB NRHS P Q SEEDA BMODE OUT [--lu] [--nosolve]
  *        ref_lu aux N NB NRHS SEEDA SEEDB OUT      (1 x 1 grid, see aux())
+ *        ref_lu svx N NB NRHS OUT                  (1 x 1 grid, see svx())
+ *        ref_lu io M N MB FILE IN OUT              (1 x 1 grid, see io())
  *        ref_lu argcheck ROUTINE ARGS...           (see argcheck())
  *   BMODE = lcg:<seed>  (PDMATGEN of B, as the tester)
  *         | ones        (b = A * ones(N), summed left to right in double)
@@ -248,12 +250,182 @@ static int aux(int argc, char **argv)
     return 0;
 }
 
+extern void pdgeequ_(int *, int *, double *, int *, int *, int *, double *, double *, double *,
+                     double *, double *, int *);
+extern void pdlaqge_(int *, int *, double *, int *, int *, int *, double *, double *, double *,
+                     double *, double *, char *, size_t);
+extern void pdgesvx_(const char *, const char *, int *, int *, double *, int *, int *, int *,
+                     double *, int *, int *, int *, int *, char *, double *, double *, double *,
+                     int *, int *, int *, double *, int *, int *, int *, double *, double *,
+                     double *, double *, int *, int *, int *, int *, size_t, size_t, size_t);
+
+/* svx N NB NRHS OUT: equilibration and the expert driver on a 1 x 1 grid.
+ * A = PDMATGEN(N, N, 100) with A(i,j) scaled by 2^(3 (i mod 7) - 4 (j mod 5))
+ * (0-based, exact), B = PDMATGEN(N, NRHS, 200).
+ *   PDGEEQU of A; PDLAQGE of A with its output
+ *   PDGESVX cases: 0 FACT='E' TRANS='N', 1 'E' 'T', 2 'N' 'N'
+ * OUT: int32 magic 0x53565852, N, NB, NRHS, info_equ, equed_laqge,
+ *      per case: info, equed
+ *      double rowcnd, colcnd, amax, R[N], C[N], A_laqge[N*N]
+ *      per case: rcond, R[N], C[N], X[N*NRHS], FERR[NRHS], BERR[NRHS],
+ *                B_out[N*NRHS] */
+static int svx(int argc, char **argv)
+{
+    if (argc < 6) {
+        fprintf(stderr, "usage: ref_lu svx N NB NRHS OUT\n");
+        return 2;
+    }
+    int N = atoi(argv[2]), NB = atoi(argv[3]), NRHS = atoi(argv[4]);
+    const char *out = argv[5];
+    int ictxt, neg1 = -1, zero = 0, one = 1, info = 0;
+    blacs_get_(&neg1, &zero, &ictxt);
+    blacs_gridinit_(&ictxt, "Row-major", &one, &one, 9);
+    int lld = N > 1 ? N : 1;
+    int desca[9], descb[9];
+    descinit_(desca, &N, &N, &NB, &NB, &zero, &zero, &ictxt, &lld, &info);
+    descinit_(descb, &N, &NRHS, &NB, &NB, &zero, &zero, &ictxt, &lld, &info);
+    size_t nn = (size_t)N * N, nr = (size_t)N * NRHS;
+    double *A0 = malloc(nn * 8), *A = malloc(nn * 8), *AF = malloc(nn * 8);
+    double *B0 = malloc(nr * 8), *B = malloc(nr * 8), *X = malloc(nr * 8);
+    double *R = malloc((size_t)N * 8), *C = malloc((size_t)N * 8);
+    int *ipiv = calloc((size_t)N + NB, sizeof(int));
+    gen_global(ictxt, N, N, NB, 100, A0);
+    for (int j = 0; j < N; j++)
+        for (int i = 0; i < N; i++) A0[i + (size_t)j * N] = ldexp(A0[i + (size_t)j * N], 3 * (i % 7) - 4 * (j % 5));
+    gen_global(ictxt, N, NRHS, NB, 200, B0);
+
+    double rowcnd = 0, colcnd = 0, amax = 0;
+    int info_eq = 0;
+    memcpy(A, A0, nn * 8);
+    pdgeequ_(&N, &N, A, &one, &one, desca, R, C, &rowcnd, &colcnd, &amax, &info_eq);
+    char equed_l = 'X';
+    pdlaqge_(&N, &N, A, &one, &one, desca, R, C, &rowcnd, &colcnd, &amax, &equed_l, 1);
+
+    FILE *f = fopen(out, "wb");
+    int32_t hdr[6] = {0x53565852, N, NB, NRHS, info_eq, (int)equed_l};
+    fwrite(hdr, sizeof hdr, 1, f);
+    /* the cases first into memory, so the header can carry their codes */
+    const char facts[3] = {'E', 'E', 'N'}, transs[3] = {'N', 'T', 'N'};
+    int32_t cinfo[6];
+    double *blob[3];
+    size_t blen = 1 + 2 * (size_t)N + nr + 2 * (size_t)NRHS + nr;
+    for (int c = 0; c < 3; c++) {
+        double *Rc = malloc((size_t)N * 8), *Cc = malloc((size_t)N * 8);
+        double *ferr = calloc((size_t)NRHS, 8), *berr = calloc((size_t)NRHS, 8);
+        double rcond = -1.0;
+        char equed = 'N';
+        memcpy(A, A0, nn * 8);
+        memcpy(B, B0, nr * 8);
+        memset(X, 0, nr * 8);
+        double wq;
+        int iq, lq = -1, inf = 0;
+        pdgesvx_(&facts[c], &transs[c], &N, &NRHS, A, &one, &one, desca, AF, &one, &one, desca, ipiv,
+                 &equed, Rc, Cc, B, &one, &one, descb, X, &one, &one, descb, &rcond, ferr, berr,
+                 &wq, &lq, &iq, &lq, &inf, 1, 1, 1);
+        int lwork = (int)wq, liwork = iq;
+        double *work = malloc((size_t)lwork * 8);
+        int *iwork = malloc((size_t)liwork * 4);
+        pdgesvx_(&facts[c], &transs[c], &N, &NRHS, A, &one, &one, desca, AF, &one, &one, desca, ipiv,
+                 &equed, Rc, Cc, B, &one, &one, descb, X, &one, &one, descb, &rcond, ferr, berr,
+                 work, &lwork, iwork, &liwork, &inf, 1, 1, 1);
+        cinfo[2 * c] = inf;
+        cinfo[2 * c + 1] = (int)equed;
+        double *bl = malloc(blen * 8), *q = bl;
+        *q++ = rcond;
+        memcpy(q, Rc, (size_t)N * 8); q += N;
+        memcpy(q, Cc, (size_t)N * 8); q += N;
+        memcpy(q, X, nr * 8); q += nr;
+        memcpy(q, ferr, (size_t)NRHS * 8); q += NRHS;
+        memcpy(q, berr, (size_t)NRHS * 8); q += NRHS;
+        memcpy(q, B, nr * 8);
+        blob[c] = bl;
+        free(work); free(iwork); free(Rc); free(Cc); free(ferr); free(berr);
+    }
+    fwrite(cinfo, sizeof cinfo, 1, f);
+    double eqd[3] = {rowcnd, colcnd, amax};
+    fwrite(eqd, 8, 3, f);
+    fwrite(R, 8, (size_t)N, f);
+    fwrite(C, 8, (size_t)N, f);
+    fclose(f);
+    /* A after PDLAQGE: recompute on a fresh copy (A was reused by the cases) */
+    memcpy(A, A0, nn * 8);
+    {
+        double r2, c2, am2;
+        int i2;
+        pdgeequ_(&N, &N, A, &one, &one, desca, R, C, &r2, &c2, &am2, &i2);
+        char e2;
+        pdlaqge_(&N, &N, A, &one, &one, desca, R, C, &r2, &c2, &am2, &e2, 1);
+    }
+    f = fopen(out, "ab");
+    fwrite(A, 8, nn, f);
+    for (int c = 0; c < 3; c++) {
+        fwrite(blob[c], 8, blen, f);
+        free(blob[c]);
+    }
+    fclose(f);
+    printf("{\"N\": %d, \"info_equ\": %d, \"equed\": \"%c\", \"cases\": [[%d, \"%c\"], [%d, \"%c\"], "
+           "[%d, \"%c\"]]}\n", N, info_eq, equed_l, cinfo[0], (char)cinfo[1], cinfo[2],
+           (char)cinfo[3], cinfo[4], (char)cinfo[5]);
+    free(A0); free(A); free(AF); free(B0); free(B); free(X); free(R); free(C); free(ipiv);
+    blacs_gridexit_(&ictxt);
+    int cont = 0;
+    blacs_exit_(&cont);
+    return 0;
+}
+
+extern void pdlawrite_(const char *, int *, int *, double *, int *, int *, int *, int *, int *,
+                       double *, size_t);
+extern void pdlaread_(const char *, double *, int *, int *, int *, double *, size_t);
+
+/* io M N MB FILE IN OUT: PDLAWRITE of the M x N matrix PDMATGEN(M, N, 100)
+ * with a few special values (0, 1, -0.5, 1e-300, -3e300, 123456789) at
+ * fixed places, as sub(A) = A(1:M, 1:N) of an (M+2) x (N+1) array, to FILE;
+ * then PDLAREAD of the file IN into an M_A x N_A = (M+3) x (N+2) array
+ * (zero filled) written to OUT as raw doubles (column-major, LLD = M+3). */
+static int io(int argc, char **argv)
+{
+    if (argc < 8) {
+        fprintf(stderr, "usage: ref_lu io M N MB FILE IN OUT\n");
+        return 2;
+    }
+    int M = atoi(argv[2]), N = atoi(argv[3]), MB = atoi(argv[4]);
+    const char *file = argv[5], *in = argv[6], *out = argv[7];
+    int ictxt, neg1 = -1, zero = 0, one = 1, info = 0;
+    blacs_get_(&neg1, &zero, &ictxt);
+    blacs_gridinit_(&ictxt, "Row-major", &one, &one, 9);
+    int Mg = M + 2, Ng = N + 1, lld = Mg;
+    int desca[9];
+    descinit_(desca, &Mg, &Ng, &MB, &MB, &zero, &zero, &ictxt, &lld, &info);
+    double *A = calloc((size_t)Mg * Ng, 8), *work = calloc((size_t)MB + 1, 8);
+    double *G = malloc((size_t)M * N * 8);
+    gen_global(ictxt, M, N, MB, 100, G);
+    const double special[6] = {0.0, 1.0, -0.5, 1e-300, -3e300, 123456789.0};
+    for (int k = 0; k < 6 && k < M * N; k++) G[(size_t)(k * 7) % ((size_t)M * N)] = special[k];
+    for (int j = 0; j < N; j++)
+        for (int i = 0; i < M; i++) A[i + (size_t)j * lld] = G[i + (size_t)j * M];
+    pdlawrite_(file, &M, &N, A, &one, &one, desca, &zero, &zero, work, strlen(file));
+    int Mr = M + 3, Nr = N + 2, lldr = Mr, descr[9];
+    descinit_(descr, &Mr, &Nr, &MB, &MB, &zero, &zero, &ictxt, &lldr, &info);
+    double *Ar = calloc((size_t)Mr * Nr, 8), *wr = calloc((size_t)MB + 1, 8);
+    pdlaread_(in, Ar, descr, &zero, &zero, wr, strlen(in));
+    FILE *f = fopen(out, "wb");
+    fwrite(Ar, 8, (size_t)Mr * Nr, f);
+    fclose(f);
+    free(A); free(work); free(G); free(Ar); free(wr);
+    blacs_gridexit_(&ictxt);
+    int cont = 0;
+    blacs_exit_(&cont);
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     int iam, nprocs;
     blacs_pinfo_(&iam, &nprocs);
     if (argc > 2 && !strcmp(argv[1], "argcheck")) return argcheck(argc, argv);
     if (argc > 1 && !strcmp(argv[1], "aux")) return aux(argc, argv);
+    if (argc > 1 && !strcmp(argv[1], "svx")) return svx(argc, argv);
+    if (argc > 1 && !strcmp(argv[1], "io")) return io(argc, argv);
     if (argc < 10) {
         if (iam == 0)
             fprintf(stderr, "usage: ref_lu M N NB NRHS P Q SEEDA BMODE OUT [--lu] [--nosolve]\n");

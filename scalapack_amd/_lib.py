@@ -50,6 +50,9 @@ def _declare(lib: ctypes.CDLL) -> None:
         "pdlaswp_": [vp] * 10 + [sz, sz],
         "pdgecon_": [vp] * 13 + [sz],
         "pdgerfs_": [vp] * 27 + [sz],
+        "pdgeequ_": [vp] * 12,
+        "pdlaqge_": [vp] * 12 + [sz],
+        "pdgesvx_": [vp] * 32 + [sz, sz, sz],
         "sla_create": [vp],
         "sla_destroy": [vp],
         "sla_set_stream": [vp, vp],
@@ -74,6 +77,10 @@ def _declare(lib: ctypes.CDLL) -> None:
         "sla_gesv_batched": [vp, i, i, vp, vp, vp, vp, vp, vp, vp, vp],
         "sla_gesv_batched_tune": [vp, i, i, vp, vp, vp, vp, vp],
         "sla_selector_features": [vp, i, vp, i, vp],
+        "sla_dgeequ": [vp, i, i, vp, i, vp, vp, vp, vp, vp, vp],
+        "sla_dlaqge": [vp, i, i, vp, i, vp, vp, d, d, d, vp],
+        "sla_dgesvx": [vp, ctypes.c_char, ctypes.c_char, i, i, vp, i, vp, i, vp, vp, vp, vp, vp,
+                       i, vp, i, vp, vp, vp, vp],
         "sla_mlp_grad": [vp, i, vp, vp, vp, vp, vp, vp],
         "sla_mlp_train": [vp, i, vp, vp, vp, vp, i, ctypes.c_float, vp],
         "sla_mlp_predict": [vp, i, vp, vp, vp, vp],

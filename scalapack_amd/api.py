@@ -174,6 +174,49 @@ def pdgerfs(trans, n, nrhs, a, ia, ja, desca, af, iaf, jaf, descaf, ipiv, b, ib,
     return info.value
 
 
+def pdgeequ(m, n, a, ia, ja, desca, r, c) -> tuple[float, float, float, int]:
+    """PDGEEQU (SRC/pdgeequ.f): returns (ROWCND, COLCND, AMAX, INFO)."""
+    d = _desc(desca)
+    rc, cc, am, info = ctypes.c_double(), ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+    load().pdgeequ_(_ref(_i(m)), _ref(_i(n)), _arr(a, np.float64), _ref(_i(ia)), _ref(_i(ja)),
+                    ctypes.c_void_p(d.ctypes.data), _arr(r, np.float64), _arr(c, np.float64),
+                    _ref(rc), _ref(cc), _ref(am), _ref(info))
+    return rc.value, cc.value, am.value, info.value
+
+
+def pdlaqge(m, n, a, ia, ja, desca, r, c, rowcnd, colcnd, amax) -> str:
+    """PDLAQGE (SRC/pdlaqge.f): returns EQUED."""
+    d = _desc(desca)
+    e = ctypes.create_string_buffer(1)
+    load().pdlaqge_(_ref(_i(m)), _ref(_i(n)), _arr(a, np.float64), _ref(_i(ia)), _ref(_i(ja)),
+                    ctypes.c_void_p(d.ctypes.data), _arr(r, np.float64), _arr(c, np.float64),
+                    _ref(ctypes.c_double(rowcnd)), _ref(ctypes.c_double(colcnd)),
+                    _ref(ctypes.c_double(amax)), e, ctypes.c_size_t(1))
+    return e.raw.decode()
+
+
+def pdgesvx(fact, trans, n, nrhs, a, ia, ja, desca, af, iaf, jaf, descaf, ipiv, equed, r, c, b,
+            ib, jb, descb, x, ix, jx, descx, ferr, berr, work, lwork, iwork,
+            liwork) -> tuple[int, str, float]:
+    """PDGESVX (SRC/pdgesvx.f): returns (INFO, EQUED, RCOND)."""
+    da, daf, db, dx = _desc(desca), _desc(descaf), _desc(descb), _desc(descx)
+    e = ctypes.create_string_buffer(equed.encode()[:1], 1)
+    rc, info = ctypes.c_double(), ctypes.c_int()
+    vp = ctypes.c_void_p
+    f = ctypes.c_char(fact.encode()[:1])
+    t = ctypes.c_char(trans.encode()[:1])
+    load().pdgesvx_(_ref(f), _ref(t), _ref(_i(n)), _ref(_i(nrhs)), _arr(a, np.float64),
+                    _ref(_i(ia)), _ref(_i(ja)), vp(da.ctypes.data), _arr(af, np.float64),
+                    _ref(_i(iaf)), _ref(_i(jaf)), vp(daf.ctypes.data), _arr(ipiv, np.int32), e,
+                    _arr(r, np.float64), _arr(c, np.float64), _arr(b, np.float64), _ref(_i(ib)),
+                    _ref(_i(jb)), vp(db.ctypes.data), _arr(x, np.float64), _ref(_i(ix)),
+                    _ref(_i(jx)), vp(dx.ctypes.data), _ref(rc), _arr(ferr, np.float64),
+                    _arr(berr, np.float64), _arr(work, np.float64), _ref(_i(lwork)),
+                    _arr(iwork, np.int32), _ref(_i(liwork)), _ref(info), ctypes.c_size_t(1),
+                    ctypes.c_size_t(1), ctypes.c_size_t(1))
+    return info.value, e.raw.decode(), rc.value
+
+
 def pdlaswp(direc, rowcol, n, a, ia, ja, desca, k1, k2, ipiv) -> None:
     """PDLASWP (SRC/pdlaswp.f): row or column interchanges K1..K2."""
     d = _desc(desca)

@@ -442,6 +442,63 @@ __global__ __launch_bounds__(256) void sample_stats_kernel(int k, const double *
     if (tid < 6) st[6 * t + tid] = r[tid][0];
 }
 
+// ---- PDGEEQU / PDLAQGE / PDGESVX ------------------------------------------
+// r_i = max_j |a_ij| (bit-pattern max into rbits, zero on entry)
+__global__ __launch_bounds__(256) void rowmax_kernel(int m, int n, const double *__restrict__ A,
+                                                     int lda, unsigned long long *rbits)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= m) return;
+    double v = 0.0;
+    for (int j = blockIdx.y; j < n; j += gridDim.y) v = fmax(v, fabs(A[i + (size_t)j * lda]));
+    atomic_max_nonneg(rbits + i, v);
+}
+// c_j = max_i |a_ij| r_i, one wave per column
+__global__ __launch_bounds__(256) void colmax_scaled_kernel(int m, int n, const double *__restrict__ A,
+                                                            int lda, const double *__restrict__ r,
+                                                            double *__restrict__ c)
+{
+    const int j = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (j >= n) return;
+    const double *a = A + (size_t)j * lda;
+    double v = 0.0;
+    for (int i = lane; i < m; i += 64) v = fmax(v, fabs(a[i]) * r[i]);
+    v = wave_max(v);
+    if (lane == 0) c[j] = v;
+}
+// v = 1 / min(max(v, smlnum), bignum)
+__global__ void recip_clamp_kernel(int n, double *v, double smlnum, double bignum)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) v[i] = 1.0 / fmin(fmax(v[i], smlnum), bignum);
+}
+// *idx = min { i : v[i] == 0 } (INT_MAX on entry)
+__global__ void first_zero_kernel(int n, const double *v, int *idx)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n && v[i] == 0.0) atomicMin(idx, i);
+}
+// A = diag(r)^mr A diag(c)^mc, the product order of pdlaqge.f:226-258
+__global__ void laqge_kernel(int m, int n, double *A, int lda, const double *__restrict__ r,
+                             const double *__restrict__ c, int mode)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= m) return;
+    for (int j = blockIdx.y; j < n; j += gridDim.y) {
+        double *a = A + i + (size_t)j * lda;
+        if (mode == 1) *a = c[j] * *a;             // 'C'
+        else if (mode == 2) *a = r[i] * *a;        // 'R'
+        else *a = c[j] * r[i] * *a;                // 'B'
+    }
+}
+// B(i, :) = v_i B(i, :)
+__global__ void scale_rows_kernel(int n, int nrhs, const double *__restrict__ v, double *B, int ldb)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    for (int j = 0; j < nrhs; j++) B[i + (size_t)j * ldb] = v[i] * B[i + (size_t)j * ldb];
+}
+
 unsigned blocks(long long n) { return (unsigned)std::max<long long>(1, (n + 255) / 256); }
 
 }  // namespace
@@ -805,6 +862,167 @@ void dgerfs_device(Handle &h, bool trans, int n, int nrhs, const double *dA, int
         if (xmax != 0.0) ferr[k] = est / xmax;  // pdgerfs.f:655-664
     }
     check_solve_flags(h, "dgerfs");
+}
+
+// ============================================================================
+// PDGEEQU
+// ============================================================================
+int dgeequ_device(Handle &h, int m, int n, const double *dA, int lda, double *dR, double *dC,
+                  double *rowcnd, double *colcnd, double *amax)
+{
+    const double smlnum = DBL_MIN, bignum = 1.0 / smlnum;
+    if (m == 0 || n == 0) {  // pdgeequ.f quick return
+        *rowcnd = 1.0;
+        *colcnd = 1.0;
+        *amax = 0.0;
+        return 0;
+    }
+    const hipStream_t s = h.stream;
+    AuxWork a(h, std::max(m, n), 0);
+    fill_u32(reinterpret_cast<unsigned *>(dR), 2 * m, 0u, s);
+    const unsigned gx = blocks(m), gy = (unsigned)std::min<long long>(n, std::max(1u, 4096u / gx));
+    hipLaunchKernelGGL(rowmax_kernel, dim3(gx, gy), dim3(256), 0, s, m, n, dA, lda,
+                       reinterpret_cast<unsigned long long *>(dR));
+    SLA_LAUNCH_CHECK();
+    a.minabs(dR, m, 0);
+    a.maxabs(dR, m, 1);
+    double mm[2];
+    a.fetch(mm, a.res, sizeof mm);
+    *amax = mm[1];
+    if (mm[0] == 0.0) {
+        fill_int(a.ires + 6, 1, INT_MAX, s);
+        hipLaunchKernelGGL(first_zero_kernel, dim3(blocks(m)), dim3(256), 0, s, m, dR, a.ires + 6);
+        int idx = 0;
+        a.fetch(&idx, a.ires + 6, 4);
+        return idx + 1;
+    }
+    hipLaunchKernelGGL(recip_clamp_kernel, dim3(blocks(m)), dim3(256), 0, s, m, dR, smlnum, bignum);
+    *rowcnd = std::max(mm[0], smlnum) / std::min(mm[1], bignum);
+    hipLaunchKernelGGL(colmax_scaled_kernel, dim3(ceil_div(n, 4)), dim3(256), 0, s, m, n, dA, lda, dR,
+                       dC);
+    SLA_LAUNCH_CHECK();
+    a.minabs(dC, n, 0);
+    a.maxabs(dC, n, 1);
+    a.fetch(mm, a.res, sizeof mm);
+    if (mm[0] == 0.0) {
+        fill_int(a.ires + 6, 1, INT_MAX, s);
+        hipLaunchKernelGGL(first_zero_kernel, dim3(blocks(n)), dim3(256), 0, s, n, dC, a.ires + 6);
+        int idx = 0;
+        a.fetch(&idx, a.ires + 6, 4);
+        return m + idx + 1;
+    }
+    hipLaunchKernelGGL(recip_clamp_kernel, dim3(blocks(n)), dim3(256), 0, s, n, dC, smlnum, bignum);
+    SLA_LAUNCH_CHECK();
+    *colcnd = std::max(mm[0], smlnum) / std::min(mm[1], bignum);
+    SLA_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+// ============================================================================
+// PDLAQGE (THRESH = 0.1; SMALL = SAFMIN / PRECISION, PRECISION = 2^-52)
+// ============================================================================
+char dlaqge_device(Handle &h, int m, int n, double *dA, int lda, const double *dR, const double *dC,
+                   double rowcnd, double colcnd, double amax)
+{
+    const double thresh = 0.1, small = DBL_MIN / std::ldexp(1.0, -52), large = 1.0 / small;
+    if (m <= 0 || n <= 0) return 'N';
+    int mode;
+    char equed;
+    if (rowcnd >= thresh && amax >= small && amax <= large) {
+        if (colcnd >= thresh) return 'N';
+        mode = 1;
+        equed = 'C';
+    } else if (colcnd >= thresh) {
+        mode = 2;
+        equed = 'R';
+    } else {
+        mode = 3;
+        equed = 'B';
+    }
+    const unsigned gx = blocks(m), gy = (unsigned)std::min<long long>(n, std::max(1u, 8192u / gx));
+    hipLaunchKernelGGL(laqge_kernel, dim3(gx, gy), dim3(256), 0, h.stream, m, n, dA, lda, dR, dC, mode);
+    SLA_LAUNCH_CHECK();
+    SLA_HIP(hipStreamSynchronize(h.stream));
+    return equed;
+}
+
+// ============================================================================
+// PDGESVX (SRC/pdgesvx.f:624-737)
+// ============================================================================
+int dgesvx_device(Handle &h, char fact, bool trans, int n, int nrhs, double *dA, int lda, double *dAF,
+                  int ldaf, int *dipiv, char *equed, double *dR, double *dC, double *dB, int ldb,
+                  double *dX, int ldx, double *rcond, double *ferr, double *berr)
+{
+    const hipStream_t s = h.stream;
+    const double smlnum = DBL_MIN, bignum = 1.0 / smlnum;
+    const char f = (char)std::toupper((unsigned char)fact);
+    bool rowequ = false, colequ = false;
+    double rowcnd = 1.0, colcnd = 1.0, amax = 0.0;
+    if (f == 'F') {
+        const char e = (char)std::toupper((unsigned char)*equed);
+        rowequ = e == 'R' || e == 'B';
+        colequ = e == 'C' || e == 'B';
+        AuxWork a(h, std::max(n, 1), 0);
+        if (rowequ && n > 0) {  // pdgesvx.f:548-565
+            a.minabs(dR, n, 0);
+            a.maxabs(dR, n, 1);
+            double mm[2];
+            a.fetch(mm, a.res, sizeof mm);
+            rowcnd = std::max(mm[0], smlnum) / std::min(mm[1], bignum);
+        }
+        if (colequ && n > 0) {
+            a.minabs(dC, n, 0);
+            a.maxabs(dC, n, 1);
+            double mm[2];
+            a.fetch(mm, a.res, sizeof mm);
+            colcnd = std::max(mm[0], smlnum) / std::min(mm[1], bignum);
+        }
+    } else {
+        *equed = 'N';
+    }
+    if (f == 'E') {
+        if (dgeequ_device(h, n, n, dA, lda, dR, dC, &rowcnd, &colcnd, &amax) == 0) {
+            *equed = dlaqge_device(h, n, n, dA, lda, dR, dC, rowcnd, colcnd, amax);
+            rowequ = *equed == 'R' || *equed == 'B';
+            colequ = *equed == 'C' || *equed == 'B';
+        }
+    }
+    // scale B (pdgesvx.f:672-697)
+    if (!trans && rowequ)
+        hipLaunchKernelGGL(scale_rows_kernel, dim3(blocks(n)), dim3(256), 0, s, n, nrhs, dR, dB, ldb);
+    else if (trans && colequ)
+        hipLaunchKernelGGL(scale_rows_kernel, dim3(blocks(n)), dim3(256), 0, s, n, nrhs, dC, dB, ldb);
+    SLA_LAUNCH_CHECK();
+    if (f == 'N' || f == 'E') {
+        SLA_HIP(hipMemcpy2DAsync(dAF, (size_t)ldaf * 8, dA, (size_t)lda * 8, (size_t)n * 8, n,
+                                 hipMemcpyDeviceToDevice, s));
+        const int info = dgetrf_device(h, n, n, dAF, ldaf, dipiv);
+        if (info != 0) {
+            if (info > 0) *rcond = 0.0;
+            return info;
+        }
+    }
+    const char norm = trans ? 'I' : '1';
+    const double anorm = dlange_device(h, norm, n, n, dA, lda);
+    *rcond = dgecon_device(h, norm, n, dAF, ldaf, anorm);
+    if (*rcond < EPS64) return n + 1;  // pdgesvx.f:715-718 (INFO = IA + N)
+    SLA_HIP(hipMemcpy2DAsync(dX, (size_t)ldx * 8, dB, (size_t)ldb * 8, (size_t)n * 8, nrhs,
+                             hipMemcpyDeviceToDevice, s));
+    dgetrs_device(h, n, nrhs, dAF, ldaf, dipiv, dX, ldx, trans);
+    dgerfs_device(h, trans, n, nrhs, dA, lda, dAF, ldaf, dipiv, dB, ldb, dX, ldx, ferr, berr);
+    // undo the scaling of X and FERR (pdgesvx.f:737-780)
+    if (!trans) {
+        if (colequ) {
+            hipLaunchKernelGGL(scale_rows_kernel, dim3(blocks(n)), dim3(256), 0, s, n, nrhs, dC, dX, ldx);
+            for (int j = 0; j < nrhs; j++) ferr[j] = ferr[j] / colcnd;
+        }
+    } else if (rowequ) {
+        hipLaunchKernelGGL(scale_rows_kernel, dim3(blocks(n)), dim3(256), 0, s, n, nrhs, dR, dX, ldx);
+        for (int j = 0; j < nrhs; j++) ferr[j] = ferr[j] / rowcnd;
+    }
+    SLA_LAUNCH_CHECK();
+    SLA_HIP(hipStreamSynchronize(s));
+    return 0;
 }
 
 // ============================================================================

@@ -3,9 +3,13 @@
 // include/sla_blacs1x1.h.
 #include "sla_blacs1x1.h"
 
+#include <cctype>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <mutex>
+#include <string>
 #include <vector>
 
 namespace {
@@ -14,6 +18,63 @@ std::mutex g_mu;
 std::vector<int> g_ctx;  // per handed-out context: 1 = live 1 x 1 grid, 0 = exited/illegal
 
 int mymax(int a, int b) { return a > b ? a : b; }
+
+// Fortran CHARACTER*(*) argument -> file name (trailing blanks dropped).
+std::string fortran_string(const char *s, size_t len)
+{
+    std::string out(s, len);
+    while (!out.empty() && (out.back() == ' ' || out.back() == '\0')) out.pop_back();
+    return out;
+}
+
+// One value in the Fortran edit descriptor D30.18 as the reference's runtime
+// writes it (TOOLS/pdlawrite.f:217): 0.d1..d18 x 10^e, exponent "D+ee" when
+// |e| <= 99 and "+eee" without the letter otherwise, right-justified in 30.
+std::string format_d30_18(double v)
+{
+    char body[64];
+    if (std::isnan(v)) {
+        std::snprintf(body, sizeof body, "NaN");
+    } else if (std::isinf(v)) {
+        std::snprintf(body, sizeof body, "%s", v < 0 ? "-Infinity" : "Infinity");
+    } else {
+        char e[64];
+        std::snprintf(e, sizeof e, "%.17e", v);  // [-]d.ddddddddddddddddde[+-]XX, 18 digits
+        const bool neg = e[0] == '-';
+        const char *d = e + (neg ? 1 : 0);
+        char digits[19];
+        digits[0] = d[0];
+        std::memcpy(digits + 1, d + 2, 17);
+        digits[18] = '\0';
+        int ex = std::atoi(std::strchr(d, 'e') + 1) + 1;
+        if (v == 0.0) ex = 0;
+        const int ax = ex < 0 ? -ex : ex;
+        if (ax <= 99)
+            std::snprintf(body, sizeof body, "%s0.%sD%c%02d", neg ? "-" : "", digits, ex < 0 ? '-' : '+', ax);
+        else
+            std::snprintf(body, sizeof body, "%s0.%s%c%03d", neg ? "-" : "", digits, ex < 0 ? '-' : '+', ax);
+    }
+    char out[64];
+    std::snprintf(out, sizeof out, "%30s", body);
+    return out;
+}
+
+// The first list-directed value of a record: separators blank or comma, a
+// D exponent read as E.
+bool parse_first_value(const std::string &line, double *v)
+{
+    size_t i = 0;
+    while (i < line.size() && (std::isspace((unsigned char)line[i]) || line[i] == ',')) i++;
+    if (i == line.size()) return false;
+    size_t j = i;
+    while (j < line.size() && !std::isspace((unsigned char)line[j]) && line[j] != ',') j++;
+    std::string tok = line.substr(i, j - i);
+    for (char &c : tok)
+        if (c == 'D' || c == 'd') c = 'E';
+    char *end = nullptr;
+    *v = std::strtod(tok.c_str(), &end);
+    return end && *end == '\0';
+}
 int mymin(int a, int b) { return a < b ? a : b; }
 
 }  // namespace
@@ -131,5 +192,84 @@ int indxl2g_(const int *indxloc, const int *nb, const int *iproc, const int *isr
 }
 
 int iceil_(const int *inum, const int *idenom) { return (*inum + *idenom - 1) / *idenom; }
+
+void pdlawrite_(const char *filnam, const int *m, const int *n, const double *a, const int *ia,
+                const int *ja, const int *desca, const int *irwrit, const int *icwrit, double *,
+                size_t filnam_len)
+{
+    (void)irwrit;
+    (void)icwrit;
+    const std::string name = fortran_string(filnam, filnam_len);
+    FILE *f = std::fopen(name.c_str(), "w");
+    if (!f) {
+        std::fprintf(stderr, "PDLAWRITE: cannot open %s\n", name.c_str());
+        return;
+    }
+    std::fprintf(f, " %d %d\n", *m, *n);
+    const int lld = desca[8];
+    for (int j = 0; j < *n; j++)
+        for (int i = 0; i < *m; i++) {
+            const double v = a[(size_t)(*ia - 1 + i) + (size_t)(*ja - 1 + j) * lld];
+            std::fprintf(f, "%s\n", format_d30_18(v).c_str());
+        }
+    std::fclose(f);
+}
+
+void pdlaread_(const char *filnam, double *a, const int *desca, const int *irread, const int *icread,
+               double *, size_t filnam_len)
+{
+    (void)irread;
+    (void)icread;
+    const std::string name = fortran_string(filnam, filnam_len);
+    FILE *f = std::fopen(name.c_str(), "r");
+    if (!f) {
+        std::fprintf(stderr, "PDLAREAD: cannot open %s\n", name.c_str());
+        int ctx = desca[1], code = 0;
+        blacs_abort_(&ctx, &code);
+        return;
+    }
+    std::vector<std::string> lines;
+    char buf[4096];
+    while (std::fgets(buf, sizeof buf, f)) lines.emplace_back(buf);
+    std::fclose(f);
+    // header: the first two integers, over as many records as they take
+    int hdr[2] = {0, 0}, got = 0;
+    size_t li = 0;
+    for (; li < lines.size() && got < 2; li++) {
+        const char *p = lines[li].c_str();
+        char *end = nullptr;
+        while (got < 2) {
+            while (*p && (std::isspace((unsigned char)*p) || *p == ',')) p++;
+            if (!*p) break;
+            const long v = std::strtol(p, &end, 10);
+            if (end == p) break;
+            hdr[got++] = (int)v;
+            p = end;
+        }
+    }
+    const int M = hdr[0], N = hdr[1];
+    if (M <= 0 || N <= 0) return;  // pdlaread.f: quick return
+    if (M > desca[2] || N > desca[3]) {  // pdlaread.f: too big, abort
+        std::printf(" PDLAREAD: Matrix too big to fit in\n Abort ...\n");
+        std::fflush(stdout);
+        int ctx = desca[1], code = 0;
+        blacs_abort_(&ctx, &code);
+        return;
+    }
+    const int lld = desca[8];
+    for (int j = 0; j < N; j++)
+        for (int i = 0; i < M; i++) {
+            double v = 0.0;
+            while (li < lines.size() && !parse_first_value(lines[li], &v)) li++;
+            if (li >= lines.size()) {
+                std::fprintf(stderr, "PDLAREAD: %s ends before %d x %d values\n", name.c_str(), M, N);
+                int ctx = desca[1], code = 0;
+                blacs_abort_(&ctx, &code);
+                return;
+            }
+            li++;
+            a[(size_t)i + (size_t)j * lld] = v;
+        }
+}
 
 }  // extern "C"

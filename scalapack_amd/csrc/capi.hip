@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cctype>
+#include <cfloat>
 #include <climits>
 #include <cmath>
 #include <cstring>
@@ -697,6 +698,242 @@ void pdgerfs_(const char *trans, const int *n, const int *nrhs, const double *a,
     }
 }
 
+// PDGEEQU (SRC/pdgeequ.f:1-2); checks as CHK1MAT (pdgeequ.f:190-198).
+void pdgeequ_(const int *m, const int *n, const double *a, const int *ia, const int *ja,
+              const int *desca, double *r, double *c, double *rowcnd, double *colcnd, double *amax,
+              int *info)
+{
+    const Grid g = gridinfo(D(desca, CTXT_));
+    *info = 0;
+    if (g.nprow == -1) *info = -(600 + CTXT_);
+    else chk1mat(*m, 1, *n, 2, *ia, *ja, desca, 6, *info);
+    if (*info != 0) {
+        pxerbla(g, "PDGEEQU", -*info);
+        return;
+    }
+    if (*m == 0 || *n == 0) {
+        *rowcnd = 1.0;
+        *colcnd = 1.0;
+        *amax = 0.0;
+        return;
+    }
+    if (g.nprow * g.npcol != 1) {
+        auto next = next_symbol<void (*)(const int *, const int *, const double *, const int *,
+                                         const int *, const int *, double *, double *, double *,
+                                         double *, double *, int *)>("pdgeequ_");
+        if (next) return next(m, n, a, ia, ja, desca, r, c, rowcnd, colcnd, amax, info);
+        *info = -(600 + CTXT_);
+        pxerbla(g, "PDGEEQU", -*info);
+        return;
+    }
+    try {
+        sla::Handle &h = default_handle();
+        const int M = *m, N = *n, lld = D(desca, LLD_), ldd = pad_ld(M);
+        DevBuf dA((size_t)ldd * N * 8), dR((size_t)M * 8), dC((size_t)N * 8);
+        to_device(dA.as<double>(), ldd, a + (size_t)(*ia - 1) + (size_t)(*ja - 1) * lld, lld, M, N);
+        *info = sla::dgeequ_device(h, M, N, dA.as<double>(), ldd, dR.as<double>(), dC.as<double>(),
+                                   rowcnd, colcnd, amax);
+        // R(IA..), C(JA..) as far as the reference computes them before a
+        // zero row (R holds the row maxima) or column returns
+        SLA_HIP(hipMemcpy(r + (*ia - 1), dR.p, (size_t)M * 8, hipMemcpyDeviceToHost));
+        if (*info == 0 || *info > M)
+            SLA_HIP(hipMemcpy(c + (*ja - 1), dC.p, (size_t)N * 8, hipMemcpyDeviceToHost));
+    } catch (const std::exception &e) {
+        fatal("PDGEEQU", e);
+    }
+}
+
+// PDLAQGE (SRC/pdlaqge.f:1-2): no argument checks in the reference.
+void pdlaqge_(const int *m, const int *n, double *a, const int *ia, const int *ja, const int *desca,
+              const double *r, const double *c, const double *rowcnd, const double *colcnd,
+              const double *amax, char *equed, size_t)
+{
+    if (*m <= 0 || *n <= 0) {
+        *equed = 'N';
+        return;
+    }
+    const Grid g = gridinfo(D(desca, CTXT_));
+    if (g.nprow * g.npcol != 1) {
+        auto next = next_symbol<void (*)(const int *, const int *, double *, const int *, const int *,
+                                         const int *, const double *, const double *, const double *,
+                                         const double *, const double *, char *, size_t)>("pdlaqge_");
+        if (next) return next(m, n, a, ia, ja, desca, r, c, rowcnd, colcnd, amax, equed, 1);
+        pxerbla(g, "PDLAQGE", 6);
+        return;
+    }
+    try {
+        sla::Handle &h = default_handle();
+        const int M = *m, N = *n, lld = D(desca, LLD_), ldd = pad_ld(M);
+        DevBuf dA((size_t)ldd * N * 8), dR((size_t)M * 8), dC((size_t)N * 8);
+        double *asub = a + (size_t)(*ia - 1) + (size_t)(*ja - 1) * lld;
+        to_device(dA.as<double>(), ldd, asub, lld, M, N);
+        SLA_HIP(hipMemcpy(dR.p, r + (*ia - 1), (size_t)M * 8, hipMemcpyHostToDevice));
+        SLA_HIP(hipMemcpy(dC.p, c + (*ja - 1), (size_t)N * 8, hipMemcpyHostToDevice));
+        *equed = sla::dlaqge_device(h, M, N, dA.as<double>(), ldd, dR.as<double>(), dC.as<double>(),
+                                    *rowcnd, *colcnd, *amax);
+        if (*equed != 'N') to_host(asub, lld, dA.as<double>(), ldd, M, N);
+    } catch (const std::exception &e) {
+        fatal("PDLAQGE", e);
+    }
+}
+
+// PDGESVX (SRC/pdgesvx.f:1-5); argument checks of pdgesvx.f:452-620.
+void pdgesvx_(const char *fact, const char *trans, const int *n, const int *nrhs, double *a,
+              const int *ia, const int *ja, const int *desca, double *af, const int *iaf,
+              const int *jaf, const int *descaf, int *ipiv, char *equed, double *r, double *c,
+              double *b, const int *ib, const int *jb, const int *descb, double *x, const int *ix,
+              const int *jx, const int *descx, double *rcond, double *ferr, double *berr,
+              double *work, const int *lwork, int *iwork, const int *liwork, int *info, size_t,
+              size_t, size_t)
+{
+    const Grid g = gridinfo(D(desca, CTXT_));
+    const bool nofact = lsame(*fact, 'N'), equil = lsame(*fact, 'E'), notran = lsame(*trans, 'N');
+    const bool factored = lsame(*fact, 'F');
+    bool rowequ = false, colequ = false, lquery = false;
+    const int N = *n;
+    *info = 0;
+    int lwmin = 0, liwmin = 0;
+    if (g.nprow == -1) {
+        *info = -(800 + CTXT_);
+    } else {
+        chk1mat(N, 3, N, 3, *ia, *ja, desca, 8, *info);
+        if (factored) chk1mat(N, 3, N, 3, *iaf, *jaf, descaf, 12, *info);
+        chk1mat(N, 3, *nrhs, 4, *ib, *jb, descb, 20, *info);
+        chk1mat(N, 3, *nrhs, 4, *ix, *jx, descx, 24, *info);
+        if (!(nofact || equil)) {
+            rowequ = lsame(*equed, 'R') || lsame(*equed, 'B');
+            colequ = lsame(*equed, 'C') || lsame(*equed, 'B');
+        }
+        if (*info == 0) {
+            const int nb = D(desca, NB_);
+            const int iarow = indxg2p(*ia, D(desca, MB_), D(desca, RSRC_), g.nprow);
+            const int iafrow = indxg2p(*iaf, D(descaf, MB_), D(descaf, RSRC_), g.nprow);
+            const int ibrow = indxg2p(*ib, D(descb, MB_), D(descb, RSRC_), g.nprow);
+            const int ixrow = indxg2p(*ix, D(descx, MB_), D(descx, RSRC_), g.nprow);
+            const int iroffa = (*ia - 1) % D(desca, MB_), iroffaf = (*iaf - 1) % D(descaf, MB_);
+            const int icoffa = (*ja - 1) % nb;
+            // 1 x 1: NP = NQ = N; LCM = 1 (pdgesvx.f:498-517)
+            const int conwrk = 2 * N + 2 * N + std::max(2, std::max(nb, N + nb));
+            int rfswrk = 3 * N + (notran ? N + N + ((N + iroffa + nb - 1) / nb) * nb : N + N);
+            lwmin = std::max(conwrk, rfswrk);
+            liwmin = N;
+            if (!nofact && !equil && !factored) *info = -1;
+            else if (!notran && !lsame(*trans, 'T') && !lsame(*trans, 'C')) *info = -2;
+            else if (iroffa != 0) *info = -6;
+            else if (icoffa != 0 || iroffa != icoffa) *info = -7;
+            else if (D(desca, MB_) != D(desca, NB_)) *info = -(800 + NB_);
+            else if (iafrow != iarow) *info = -10;
+            else if (iroffaf != 0) *info = -10;
+            else if (D(desca, CTXT_) != D(descaf, CTXT_)) *info = -(1200 + CTXT_);
+            else if (factored && !(rowequ || colequ || lsame(*equed, 'N'))) *info = -13;
+            else {
+                if (rowequ) {
+                    double mn = 1.0 / DBL_MIN;
+                    for (int i = 0; i < N; i++) mn = std::min(mn, r[(*ia - 1) + i]);
+                    if (mn <= 0.0) *info = -14;
+                }
+                if (colequ && *info == 0) {
+                    double mn = 1.0 / DBL_MIN;
+                    for (int j = 0; j < N; j++) mn = std::min(mn, c[(*ja - 1) + j]);
+                    if (mn <= 0.0) *info = -15;
+                }
+            }
+            work[0] = (double)lwmin;
+            iwork[0] = liwmin;
+            lquery = (*lwork == -1 || *liwork == -1);
+            if (*info == 0) {
+                if (ibrow != iarow) *info = -18;
+                else if (ixrow != ibrow) *info = -22;
+                else if (D(descb, MB_) != D(desca, NB_)) *info = -(2000 + NB_);
+                else if (D(descb, CTXT_) != D(desca, CTXT_)) *info = -(2000 + CTXT_);
+                else if (D(descx, MB_) != D(desca, NB_)) *info = -(2400 + NB_);
+                else if (D(descx, CTXT_) != D(desca, CTXT_)) *info = -(2400 + CTXT_);
+                else if (*lwork < lwmin && !lquery) *info = -29;
+                else if (*liwork < liwmin && !lquery) *info = -31;
+            }
+        }
+    }
+    if (*info != 0) {
+        pxerbla(g, "PDGESVX", -*info);
+        return;
+    }
+    if (lquery) return;
+    if (g.nprow * g.npcol != 1) {
+        auto next = next_symbol<void (*)(const char *, const char *, const int *, const int *, double *,
+                                         const int *, const int *, const int *, double *, const int *,
+                                         const int *, const int *, int *, char *, double *, double *,
+                                         double *, const int *, const int *, const int *, double *,
+                                         const int *, const int *, const int *, double *, double *,
+                                         double *, double *, const int *, int *, const int *, int *,
+                                         size_t, size_t, size_t)>("pdgesvx_");
+        if (next) return next(fact, trans, n, nrhs, a, ia, ja, desca, af, iaf, jaf, descaf, ipiv,
+                              equed, r, c, b, ib, jb, descb, x, ix, jx, descx, rcond, ferr, berr,
+                              work, lwork, iwork, liwork, info, 1, 1, 1);
+        *info = -(800 + CTXT_);
+        pxerbla(g, "PDGESVX", -*info);
+        return;
+    }
+    if (N == 0) return;
+    try {
+        sla::Handle &h = default_handle();
+        PermKeyReset reset{h};
+        const int NR = *nrhs, ldd = pad_ld(N);
+        const int llda = D(desca, LLD_), lldaf = D(descaf, LLD_), lldb = D(descb, LLD_),
+                  lldx = D(descx, LLD_);
+        DevBuf dA((size_t)ldd * N * 8), dAF((size_t)ldd * N * 8), dB((size_t)ldd * std::max(NR, 1) * 8),
+            dX((size_t)ldd * std::max(NR, 1) * 8), dp((size_t)N * 4), dR((size_t)N * 8),
+            dC((size_t)N * 8);
+        double *asub = a + (size_t)(*ia - 1) + (size_t)(*ja - 1) * llda;
+        double *afsub = af + (size_t)(*iaf - 1) + (size_t)(*jaf - 1) * lldaf;
+        double *bsub = b + (size_t)(*ib - 1) + (size_t)(*jb - 1) * lldb;
+        double *xsub = x + (size_t)(*ix - 1) + (size_t)(*jx - 1) * lldx;
+        to_device(dA.as<double>(), ldd, asub, llda, N, N);
+        if (NR > 0) to_device(dB.as<double>(), ldd, bsub, lldb, N, NR);
+        if (factored) {
+            to_device(dAF.as<double>(), ldd, afsub, lldaf, N, N);
+            std::vector<int> piv(N);
+            for (int i = 0; i < N; i++) piv[i] = ipiv[(*iaf - 1) + i] - (*iaf - 1);
+            SLA_HIP(hipMemcpy(dp.p, piv.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+            SLA_HIP(hipMemcpy(dR.p, r + (*ia - 1), (size_t)N * 8, hipMemcpyHostToDevice));
+            SLA_HIP(hipMemcpy(dC.p, c + (*ja - 1), (size_t)N * 8, hipMemcpyHostToDevice));
+        }
+        std::vector<double> fe(std::max(NR, 1)), be(std::max(NR, 1));
+        for (int k = 0; k < NR; k++) fe[k] = ferr[(*jb - 1) + k];
+        char eq = *equed;
+        const int dinfo = sla::dgesvx_device(h, *fact, !notran, N, NR, dA.as<double>(), ldd,
+                                             dAF.as<double>(), ldd, dp.as<int>(), &eq,
+                                             dR.as<double>(), dC.as<double>(), dB.as<double>(), ldd,
+                                             dX.as<double>(), ldd, rcond, fe.data(), be.data());
+        *equed = eq;
+        // outputs as the reference leaves them: the equilibrated A, the
+        // factors and pivots, the scales, the scaled B, X and the bounds
+        if (eq != 'N') to_host(asub, llda, dA.as<double>(), ldd, N, N);
+        if (!factored) {
+            to_host(afsub, lldaf, dAF.as<double>(), ldd, N, N);
+            std::vector<int> piv(N);
+            SLA_HIP(hipMemcpy(piv.data(), dp.p, sizeof(int) * N, hipMemcpyDeviceToHost));
+            for (int i = 0; i < N; i++) ipiv[(*iaf - 1) + i] = piv[i] + (*iaf - 1);
+        }
+        if (lsame(*fact, 'E')) {
+            SLA_HIP(hipMemcpy(r + (*ia - 1), dR.p, (size_t)N * 8, hipMemcpyDeviceToHost));
+            SLA_HIP(hipMemcpy(c + (*ja - 1), dC.p, (size_t)N * 8, hipMemcpyDeviceToHost));
+        }
+        if (NR > 0) to_host(bsub, lldb, dB.as<double>(), ldd, N, NR);
+        if (dinfo == 0 && NR > 0) {
+            to_host(xsub, lldx, dX.as<double>(), ldd, N, NR);
+            for (int k = 0; k < NR; k++) {
+                ferr[(*jb - 1) + k] = fe[k];
+                berr[(*jb - 1) + k] = be[k];
+            }
+        }
+        *info = dinfo == N + 1 ? *ia + N : dinfo;
+        work[0] = (double)lwmin;
+        iwork[0] = liwmin;
+    } catch (const std::exception &e) {
+        fatal("PDGESVX", e);
+    }
+}
+
 // ============================================================================
 // Part 2: device-resident interface
 // ============================================================================
@@ -914,6 +1151,48 @@ int sla_gesv_batched_tune(sla_handle_t h, int prec, int batch, const int *n, con
         g_last_error = e.what();
         return SLA_ERR_RUNTIME;
     }
+}
+
+int sla_dgeequ(sla_handle_t h, int m, int n, const double *dA, int lda, double *dR, double *dC,
+               double *rowcnd, double *colcnd, double *amax, int *info)
+{
+    SLA_NEED_HANDLE(h);
+    if (m < 0 || n < 0 || lda < std::max(1, m) || !rowcnd || !colcnd || !amax || !info) {
+        g_last_error = "sla_dgeequ: illegal m, n, lda or output";
+        return SLA_ERR_ARG;
+    }
+    SLA_TRY(*info = sla::dgeequ_device(*reinterpret_cast<sla::Handle *>(h), m, n, dA, lda, dR, dC,
+                                       rowcnd, colcnd, amax))
+}
+
+int sla_dlaqge(sla_handle_t h, int m, int n, double *dA, int lda, const double *dR,
+               const double *dC, double rowcnd, double colcnd, double amax, char *equed)
+{
+    SLA_NEED_HANDLE(h);
+    if (m < 0 || n < 0 || lda < std::max(1, m) || !equed) {
+        g_last_error = "sla_dlaqge: illegal m, n, lda or output";
+        return SLA_ERR_ARG;
+    }
+    SLA_TRY(*equed = sla::dlaqge_device(*reinterpret_cast<sla::Handle *>(h), m, n, dA, lda, dR, dC,
+                                        rowcnd, colcnd, amax))
+}
+
+int sla_dgesvx(sla_handle_t h, char fact, char trans, int n, int nrhs, double *dA, int lda,
+               double *dAF, int ldaf, int *dipiv, char *equed, double *dR, double *dC, double *dB,
+               int ldb, double *dX, int ldx, double *rcond, double *ferr, double *berr, int *info)
+{
+    SLA_NEED_HANDLE(h);
+    const bool fok = lsame(fact, 'N') || lsame(fact, 'E') || lsame(fact, 'F');
+    const bool tok = lsame(trans, 'N') || lsame(trans, 'T') || lsame(trans, 'C');
+    const int ld1 = std::max(1, n);
+    if (!fok || !tok || n < 0 || nrhs < 0 || lda < ld1 || ldaf < ld1 || ldb < ld1 || ldx < ld1 ||
+        !equed || !rcond || !info || (nrhs > 0 && (!ferr || !berr))) {
+        g_last_error = "sla_dgesvx: illegal FACT, TRANS, sizes or output";
+        return SLA_ERR_ARG;
+    }
+    SLA_TRY(*info = sla::dgesvx_device(*reinterpret_cast<sla::Handle *>(h), fact, !lsame(trans, 'N'),
+                                       n, nrhs, dA, lda, dAF, ldaf, dipiv, equed, dR, dC, dB, ldb,
+                                       dX, ldx, rcond, ferr, berr))
 }
 
 int sla_dlange(sla_handle_t h, char norm, int m, int n, const double *dA, int lda, double *value)

@@ -22,4 +22,17 @@ void dgerfs_device(Handle &h, bool trans, int n, int nrhs, const double *dA, int
                    const double *dAF, int ldaf, const int *dipiv, const double *dB, int ldb,
                    double *dX, int ldx, double *ferr, double *berr);
 
+// PDGEEQU: row scales dR[m], column scales dC[n] (device), the ratios and
+// max |a|; returns INFO (i: row i exactly zero; m + j: column j).
+int dgeequ_device(Handle &h, int m, int n, const double *dA, int lda, double *dR, double *dC,
+                  double *rowcnd, double *colcnd, double *amax);
+// PDLAQGE: scales dA in place as the ratios ask; returns EQUED.
+char dlaqge_device(Handle &h, int m, int n, double *dA, int lda, const double *dR, const double *dC,
+                   double rowcnd, double colcnd, double amax);
+// PDGESVX with FACT 'N', 'E' or 'F' (equed in/out); returns INFO (n + 1:
+// RCOND below eps, X not computed).
+int dgesvx_device(Handle &h, char fact, bool trans, int n, int nrhs, double *dA, int lda, double *dAF,
+                  int ldaf, int *dipiv, char *equed, double *dR, double *dC, double *dB, int ldb,
+                  double *dX, int ldx, double *rcond, double *ferr, double *berr);
+
 }  // namespace sla

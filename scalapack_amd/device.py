@@ -124,6 +124,31 @@ class Handle:
                                     ctypes.byref(out)))
         return out.value
 
+    def dgeequ(self, m: int, n: int, a, lda: int, r, c) -> tuple[float, float, float, int]:
+        """PDGEEQU: scales into r, c (device); returns (rowcnd, colcnd, amax, info)."""
+        rc, cc, am, info = ctypes.c_double(), ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        _check(self._lib.sla_dgeequ(self._h, m, n, _ptr(a), lda, _ptr(r), _ptr(c), ctypes.byref(rc),
+                                    ctypes.byref(cc), ctypes.byref(am), ctypes.byref(info)))
+        return rc.value, cc.value, am.value, info.value
+
+    def dlaqge(self, m: int, n: int, a, lda: int, r, c, rowcnd, colcnd, amax) -> str:
+        e = ctypes.create_string_buffer(1)
+        _check(self._lib.sla_dlaqge(self._h, m, n, _ptr(a), lda, _ptr(r), _ptr(c), float(rowcnd),
+                                    float(colcnd), float(amax), e))
+        return e.raw.decode()
+
+    def dgesvx(self, fact: str, trans: str, n: int, nrhs: int, a, lda: int, af, ldaf: int, ipiv,
+               r, c, b, ldb: int, x, ldx: int, equed: str = "N"):
+        """PDGESVX on device data: (info, equed, rcond, ferr, berr)."""
+        e = ctypes.create_string_buffer(equed.encode()[:1], 1)
+        rc, info = ctypes.c_double(), ctypes.c_int()
+        fe = (ctypes.c_double * max(1, nrhs))()
+        be = (ctypes.c_double * max(1, nrhs))()
+        _check(self._lib.sla_dgesvx(self._h, fact.encode()[:1], trans.encode()[:1], n, nrhs, _ptr(a),
+                                    lda, _ptr(af), ldaf, _ptr(ipiv), e, _ptr(r), _ptr(c), _ptr(b),
+                                    ldb, _ptr(x), ldx, ctypes.byref(rc), fe, be, ctypes.byref(info)))
+        return info.value, e.raw.decode(), rc.value, list(fe[:nrhs]), list(be[:nrhs])
+
     def dgecon(self, norm: str, n: int, lu, lda: int, anorm: float) -> float:
         """PDGECON: reciprocal condition estimate from the factors of dgetrf."""
         out = ctypes.c_double()

@@ -69,3 +69,29 @@ def aux(N: int, NB: int, NRHS: int) -> dict:
     z = np.load(HERE / "aux_cases.npz")
     pre = f"N{N}_NB{NB}_NRHS{NRHS}_"
     return {k[len(pre):]: z[k] for k in z.files if k.startswith(pre)}
+
+
+def svx_cases() -> list[tuple[int, int, int]]:
+    z = np.load(HERE / "svx_cases.npz")
+    out = set()
+    for k in z.files:
+        p = k.split("_")
+        out.add((int(p[0][1:]), int(p[1][2:]), int(p[2][4:])))
+    return sorted(out)
+
+
+def svx(N: int, NB: int, NRHS: int) -> dict:
+    """Outputs of the reference's PDGEEQU / PDLAQGE / PDGESVX (see
+    tests/golden/gen_golden.py)."""
+    z = np.load(HERE / "svx_cases.npz")
+    pre = f"N{N}_NB{NB}_NRHS{NRHS}_"
+    return {k[len(pre):]: z[k] for k in z.files if k.startswith(pre)}
+
+
+def svx_matrix(oracle, N: int) -> np.ndarray:
+    """The scaled test matrix of the svx fixtures: PDMATGEN(N, N, 100) with
+    A(i, j) * 2^(3 (i mod 7) - 4 (j mod 5)), 0-based indices."""
+    a = oracle.matgen(N, N, 100)
+    i = np.arange(N)[:, None] % 7
+    j = np.arange(N)[None, :] % 5
+    return np.asfortranarray(np.ldexp(a, 3 * i - 4 * j))

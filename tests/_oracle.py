@@ -155,3 +155,42 @@ def pdgerfs(trans: str, a: np.ndarray, lu: np.ndarray, ipiv: np.ndarray, b: np.n
                          _p(np.ascontiguousarray(ipiv, np.int32)), _p(b), n, _p(x), n,
                          _p(ferr), _p(berr))
     return x, ferr, berr
+
+
+def pdgeequ(a: np.ndarray):
+    """(R, C, ROWCND, COLCND, AMAX, INFO) of PDGEEQU."""
+    a = np.asfortranarray(a)
+    m, n = a.shape
+    r, c = np.zeros(max(1, m)), np.zeros(max(1, n))
+    rc, cc, am = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    info = lib().oracle_pdgeequ(m, n, _p(a), max(1, m), _p(r), _p(c), ctypes.byref(rc),
+                                ctypes.byref(cc), ctypes.byref(am))
+    return r[:m], c[:n], rc.value, cc.value, am.value, info
+
+
+def pdlaqge(a: np.ndarray, r, c, rowcnd, colcnd, amax) -> str:
+    """Scales a in place; returns EQUED."""
+    lib().oracle_pdlaqge.restype = ctypes.c_char
+    m, n = a.shape
+    e = lib().oracle_pdlaqge(m, n, _p(a), max(1, m), _p(np.ascontiguousarray(r, np.float64)),
+                             _p(np.ascontiguousarray(c, np.float64)), ctypes.c_double(rowcnd),
+                             ctypes.c_double(colcnd), ctypes.c_double(amax))
+    return e.decode()
+
+
+def pdgesvx(fact: str, trans: str, a: np.ndarray, b: np.ndarray, nb: int):
+    """PDGESVX with FACT 'E' or 'N' on copies of a and b; returns a dict."""
+    a = np.asfortranarray(a.copy())
+    b = np.asfortranarray(b.copy())
+    n, nrhs = b.shape
+    af, x = np.zeros((n, n), order="F"), np.zeros((n, nrhs), order="F")
+    ipiv = np.zeros(n + nb, np.int32)
+    r, c = np.zeros(n), np.zeros(n)
+    ferr, berr = np.zeros(nrhs), np.zeros(nrhs)
+    eq = ctypes.create_string_buffer(1)
+    rcond = ctypes.c_double()
+    info = lib().oracle_pdgesvx(ctypes.c_char(fact.encode()), ctypes.c_char(trans.encode()), n,
+                                nrhs, nb, _p(a), _p(af), _p(ipiv), eq, _p(r), _p(c), _p(b),
+                                _p(x), ctypes.byref(rcond), _p(ferr), _p(berr))
+    return dict(info=info, equed=eq.raw.decode(), R=r, C=c, X=x, rcond=rcond.value, ferr=ferr,
+                berr=berr, B=b, A=a, AF=af, ipiv=ipiv[:n])

@@ -16,7 +16,10 @@ IPIV, INFO, the solution X and the tester's scaled residual.  Cases:
   * the routines beside the solve (ref_lu aux): PDLANGE 'M' '1' 'I' 'F' of A
     and of A(2:N-2, 3:N), PDGECON '1' and 'I', PDGETRS 'T', and PDGERFS 'N'
     from the PDGETRS solution, 'N' and 'T' from perturbed starts, for
-    (N, NB, NRHS) in AUX_CASES with A = PDMATGEN(100), B = PDMATGEN(200).
+    (N, NB, NRHS) in AUX_CASES with A = PDMATGEN(100), B = PDMATGEN(200);
+  * equilibration and the expert driver (ref_lu svx): PDGEEQU, PDLAQGE and
+    PDGESVX (FACT E/TRANS N, E/T, N/N) on PDMATGEN(100) with exact
+    power-of-two row and column scales, for SVX_CASES.
 
 Usage (in the build container, where /root/reference exists):
     make -C oracle ref && python tests/golden/gen_golden.py [--skip-c2]
@@ -45,6 +48,7 @@ LU_DAT_NB = [2, 3, 4]
 LU_DAT_NRHS = [1, 3, 9]
 LU_DAT_GRIDS = [(1, 1), (2, 2), (1, 4), (4, 1)]
 AUX_CASES = [(37, 4, 2), (200, 32, 3), (513, 64, 1)]
+SVX_CASES = [(48, 8, 2), (150, 32, 3)]
 
 
 def read_out(path: Path) -> dict:
@@ -115,6 +119,44 @@ def run_aux(N, NB, NRHS) -> dict:
         r[f"rfs{c}_ferr"] = vec()
         r[f"rfs{c}_berr"] = vec()
     assert off == len(b)
+    return r
+
+
+def run_svx(N, NB, NRHS) -> dict:
+    """ref_lu svx: see oracle/ref_driver.c svx() for the file layout."""
+    with tempfile.TemporaryDirectory() as td:
+        out = Path(td) / "svx.bin"
+        env = dict(os.environ, MKL_THREADING_LAYER="SEQUENTIAL", MKL_NUM_THREADS="1")
+        subprocess.run([str(REF_LU), "svx", str(N), str(NB), str(NRHS), str(out)], check=True,
+                       capture_output=True, env=env)
+        b = out.read_bytes()
+    hdr = np.frombuffer(b[:24], np.int32)
+    assert hdr[0] == 0x53565852 and tuple(hdr[1:4]) == (N, NB, NRHS)
+    cinfo = np.frombuffer(b[24:48], np.int32)
+    off = 48
+    d = np.frombuffer(b[off:], np.float64)
+    nr = N * NRHS
+    q = 0
+
+    def take(k):
+        nonlocal q
+        v = d[q:q + k].copy()
+        q += k
+        return v
+
+    r = dict(info_equ=np.array(hdr[4]), equed_laqge=np.array(hdr[5]))
+    r["rowcnd"], r["colcnd"], r["amax"] = take(1), take(1), take(1)
+    r["R"], r["C"] = take(N), take(N)
+    r["A_laqge"] = take(N * N).reshape(N, N).T.copy(order="F")
+    for c in range(3):
+        r[f"c{c}_info"] = np.array(cinfo[2 * c])
+        r[f"c{c}_equed"] = np.array(cinfo[2 * c + 1])
+        r[f"c{c}_rcond"] = take(1)
+        r[f"c{c}_R"], r[f"c{c}_C"] = take(N), take(N)
+        r[f"c{c}_X"] = take(nr).reshape(NRHS, N).T.copy(order="F")
+        r[f"c{c}_ferr"], r[f"c{c}_berr"] = take(NRHS), take(NRHS)
+        r[f"c{c}_B"] = take(nr).reshape(NRHS, N).T.copy(order="F")
+    assert q == d.size
     return r
 
 
@@ -226,6 +268,12 @@ def main() -> None:
                 aux[f"N{N}_NB{NB}_NRHS{NRHS}_{k}"] = v
         np.savez_compressed(HERE / "aux_cases.npz", **aux)
         print("aux:", AUX_CASES)
+        svx = {}
+        for (N, NB, NRHS) in SVX_CASES:
+            for k, v in run_svx(N, NB, NRHS).items():
+                svx[f"N{N}_NB{NB}_NRHS{NRHS}_{k}"] = v
+        np.savez_compressed(HERE / "svx_cases.npz", **svx)
+        print("svx:", SVX_CASES)
         return
 
     # LU.dat cases
@@ -271,6 +319,12 @@ def main() -> None:
             aux[f"N{N}_NB{NB}_NRHS{NRHS}_{k}"] = v
     np.savez_compressed(HERE / "aux_cases.npz", **aux)
     print("aux:", AUX_CASES)
+    svx = {}
+    for (N, NB, NRHS) in SVX_CASES:
+        for k, v in run_svx(N, NB, NRHS).items():
+            svx[f"N{N}_NB{NB}_NRHS{NRHS}_{k}"] = v
+    np.savez_compressed(HERE / "svx_cases.npz", **svx)
+    print("svx:", SVX_CASES)
 
     # 6 x 6 example
     a6 = read_dat(REFERENCE / "EXAMPLE" / "DSCAEXMAT.dat")

@@ -176,3 +176,35 @@ def test_device_handle_params_need_no_gpu_for_validation():
     rc = lib.sla_set_param(ctypes.c_void_p(0), b"no_such_knob", 1)
     assert rc != 0
     assert "unknown parameter" in scalapack_amd._lib.last_error()
+
+
+def test_pdgesvx_argument_checks(capfd):
+    """PDGESVX's checks and workspace query return before any GPU work
+    (pdgesvx.f:452-620): LWMIN = max(5N + NB, 5N + ceil(N/NB) NB) on 1 x 1."""
+    ctx = api.blacs_gridinit(1, 1)
+    n, nb, nrhs = 10, 4, 1
+    d, _ = api.descinit(n, n, nb, nb, 0, 0, ctx, n)
+    db, _ = api.descinit(n, nrhs, nb, nb, 0, 0, ctx, n)
+    a = np.asfortranarray(np.eye(n))
+    af = np.asfortranarray(np.zeros((n, n)))
+    b = np.asfortranarray(np.ones((n, nrhs)))
+    x = np.asfortranarray(np.zeros((n, nrhs)))
+    ipiv = np.zeros(n + nb, np.int32)
+    r, c = np.ones(n), np.ones(n)
+    fe, be = np.zeros(nrhs), np.zeros(nrhs)
+    work, iwork = np.zeros(200), np.zeros(20, np.int32)
+
+    def svx(fact="N", trans="N", lwork=200, liwork=20, equed="N", rr=r, ja=1):
+        return api.pdgesvx(fact, trans, n, nrhs, a, 1, ja, d, af, 1, 1, d, ipiv, equed, rr, c, b,
+                           1, 1, db, x, 1, 1, db, fe, be, work, lwork, iwork, liwork)[0]
+    assert svx(lwork=-1) == 0
+    lwmin = max(5 * n + nb, 5 * n + -(-n // nb) * nb)
+    assert work[0] == lwmin and iwork[0] == n
+    assert svx(fact="Q") == -1
+    assert svx(trans="Q") == -2
+    assert svx(fact="F", equed="Z") == -13
+    assert svx(fact="F", equed="R", rr=np.zeros(n)) == -14
+    assert svx(lwork=lwmin - 1) == -29
+    assert svx(liwork=n - 1) == -31
+    out = capfd.readouterr().out
+    assert "On entry to PDGESVX parameter number   29 had an illegal value" in out

@@ -305,3 +305,105 @@ def test_fortran_entries_match_reference(ctx, oracle, golden):
     bad = api.pdgerfs("N", N, NRHS, a, 1, 1, desca, af, 1, 1, desca, ipiv, b, 1, 1, descb,
                       x, 1, 1, descb, ferr, berr, np.zeros(lw), lw - 1, np.zeros(liw, np.int32), liw)
     assert bad == -24
+
+
+# -------------------------------------------- PDGEEQU / PDLAQGE / PDGESVX
+@pytest.mark.parametrize("idx", range(2))
+def test_equilibration_matches_reference_bitwise(dev, oracle, golden, idx):
+    N, NB, NRHS = golden.svx_cases()[idx]
+    _note(f"dgeequ/dlaqge golden N={N}")
+    r = golden.svx(N, NB, NRHS)
+    a = golden.svx_matrix(oracle, N)
+    dA, lda = to_dev(a, N + 2)
+    dR = torch.zeros(N, dtype=torch.float64, device="cuda")
+    dC = torch.zeros(N, dtype=torch.float64, device="cuda")
+    rc, cc, am, info = dev.dgeequ(N, N, dA, lda, dR, dC)
+    assert info == 0 and (rc, cc, am) == (r["rowcnd"][0], r["colcnd"][0], r["amax"][0])
+    assert np.array_equal(dR.cpu().numpy(), r["R"]) and np.array_equal(dC.cpu().numpy(), r["C"])
+    assert dev.dlaqge(N, N, dA, lda, dR, dC, rc, cc, am) == "B"
+    assert np.array_equal(from_dev(dA, N), r["A_laqge"])
+
+
+def test_dgeequ_zero_row_and_column(dev, oracle):
+    _note("dgeequ zero row / column")
+    a = oracle.matgen(60, 45, 3)
+    for kind, info_expect in (("row", 17 + 1), ("col", 60 + 9 + 1)):
+        z = a.copy(order="F")
+        if kind == "row":
+            z[17, :] = 0.0
+        else:
+            z[:, 9] = 0.0
+        dA, lda = to_dev(z)
+        dR = torch.zeros(60, dtype=torch.float64, device="cuda")
+        dC = torch.zeros(45, dtype=torch.float64, device="cuda")
+        assert dev.dgeequ(60, 45, dA, lda, dR, dC)[3] == info_expect == oracle.pdgeequ(z)[5]
+
+
+@pytest.mark.parametrize("idx", range(2))
+def test_dgesvx_matches_reference(dev, oracle, golden, idx):
+    N, NB, NRHS = golden.svx_cases()[idx]
+    _note(f"dgesvx golden N={N}")
+    r = golden.svx(N, NB, NRHS)
+    a = golden.svx_matrix(oracle, N)
+    b = oracle.matgen(N, NRHS, 200)
+    for c, (fact, trans) in enumerate([("E", "N"), ("E", "T"), ("N", "N")]):
+        dA, lda = to_dev(a, N + 1)
+        dAF, ldaf = to_dev(np.zeros((N, N)), N + 3)
+        dB, ldb = to_dev(b)
+        dX, ldx = to_dev(np.zeros((N, NRHS)), N + 2)
+        dp = torch.zeros(N, dtype=torch.int32, device="cuda")
+        dR = torch.zeros(N, dtype=torch.float64, device="cuda")
+        dC = torch.zeros(N, dtype=torch.float64, device="cuda")
+        info, equed, rcond, ferr, berr = dev.dgesvx(fact, trans, N, NRHS, dA, lda, dAF, ldaf, dp, dR,
+                                                   dC, dB, ldb, dX, ldx)
+        assert info == 0 and equed == chr(int(r[f"c{c}_equed"])), (fact, trans)
+        X = r[f"c{c}_X"]
+        assert np.allclose(from_dev(dX, N), X, rtol=0, atol=1e-13 * np.abs(X).max()), (fact, trans)
+        assert np.isclose(rcond, r[f"c{c}_rcond"][0], rtol=1e-10, atol=0)
+        assert np.allclose(ferr, r[f"c{c}_ferr"], rtol=8 / (N + 1), atol=0)
+        assert np.all(np.array(berr) <= 4 * EPS)
+        assert np.array_equal(from_dev(dB, N), r[f"c{c}_B"])
+        if fact == "E":
+            assert np.array_equal(dR.cpu().numpy(), r[f"c{c}_R"])
+            assert np.array_equal(dC.cpu().numpy(), r[f"c{c}_C"])
+            # FACT = 'F' with the factors, pivots, scales and EQUED of this call
+            # on the equilibrated A: the same X
+            dB2, _ = to_dev(b)
+            dX2, _ = to_dev(np.zeros((N, NRHS)), N + 2)
+            info2, eq2, rc2, fe2, be2 = dev.dgesvx("F", trans, N, NRHS, dA, lda, dAF, ldaf, dp, dR, dC,
+                                                  dB2, ldb, dX2, ldx, equed=equed)
+            assert info2 == 0 and eq2 == equed and rc2 == rcond
+            assert np.allclose(from_dev(dX2, N), X, rtol=0, atol=1e-13 * np.abs(X).max())
+
+
+def test_fortran_pdgesvx_matches_reference(ctx, oracle, golden):
+    from scalapack_amd import api
+    N, NB, NRHS = 150, 32, 3
+    _note("fortran pdgeequ/pdlaqge/pdgesvx")
+    r = golden.svx(N, NB, NRHS)
+    a0 = golden.svx_matrix(oracle, N)
+    b0 = oracle.matgen(N, NRHS, 200)
+    desca, _ = api.descinit(N, N, NB, NB, 0, 0, ctx, N)
+    descb, _ = api.descinit(N, NRHS, NB, NB, 0, 0, ctx, N)
+    R, C = np.zeros(N), np.zeros(N)
+    rc, cc, am, info = api.pdgeequ(N, N, a0, 1, 1, desca, R, C)
+    assert info == 0 and np.array_equal(R, r["R"]) and np.array_equal(C, r["C"])
+    a1 = a0.copy(order="F")
+    assert api.pdlaqge(N, N, a1, 1, 1, desca, R, C, rc, cc, am) == "B"
+    assert np.array_equal(a1, r["A_laqge"])
+    for c, (fact, trans) in enumerate([("E", "N"), ("E", "T"), ("N", "N")]):
+        a, b = a0.copy(order="F"), b0.copy(order="F")
+        af, x = np.zeros((N, N), order="F"), np.zeros((N, NRHS), order="F")
+        ipiv = np.zeros(N + NB, np.int32)
+        R, C, fe, be = np.zeros(N), np.zeros(N), np.zeros(NRHS), np.zeros(NRHS)
+        lw = 6 * N + NB
+        info, equed, rcond = api.pdgesvx(fact, trans, N, NRHS, a, 1, 1, desca, af, 1, 1, desca, ipiv,
+                                         "N", R, C, b, 1, 1, descb, x, 1, 1, descb, fe, be,
+                                         np.zeros(lw), lw, np.zeros(N, np.int32), N)
+        assert info == 0 and equed == chr(int(r[f"c{c}_equed"]))
+        X = r[f"c{c}_X"]
+        assert np.allclose(x, X, rtol=0, atol=1e-13 * np.abs(X).max())
+        assert np.isclose(rcond, r[f"c{c}_rcond"][0], rtol=1e-10, atol=0)
+        assert np.array_equal(b, r[f"c{c}_B"])
+        if fact == "E":
+            assert np.array_equal(a, r["A_laqge"])

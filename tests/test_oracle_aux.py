@@ -178,3 +178,70 @@ def test_pdgerfs_quick_return(oracle):
     x, ferr, berr = oracle.pdgerfs("N", a, a.copy(order="F"), np.array([1], np.int32),
                                    np.asfortranarray([[4.0]]), np.asfortranarray([[2.0]]))
     assert ferr[0] == 0.0 and berr[0] == 0.0
+
+
+# ------------------------------------------------ PDGEEQU / PDLAQGE / PDGESVX
+@pytest.mark.parametrize("idx", range(2))
+def test_equilibration_matches_reference_bitwise(oracle, golden, idx):
+    """PDGEEQU and PDLAQGE are maxima, reciprocals and products: bit for bit."""
+    N, NB, NRHS = golden.svx_cases()[idx]
+    r = golden.svx(N, NB, NRHS)
+    a = golden.svx_matrix(oracle, N)
+    R, C, rc, cc, am, info = oracle.pdgeequ(a)
+    assert info == int(r["info_equ"]) == 0
+    assert np.array_equal(R, r["R"]) and np.array_equal(C, r["C"])
+    assert (rc, cc, am) == (r["rowcnd"][0], r["colcnd"][0], r["amax"][0])
+    a2 = a.copy(order="F")
+    assert oracle.pdlaqge(a2, R, C, rc, cc, am) == chr(int(r["equed_laqge"])) == "B"
+    assert np.array_equal(a2, r["A_laqge"])
+
+
+def test_equilibration_closed_forms(oracle):
+    """R_i = 1 / max_j |a_ij|, C_j = 1 / max_i |a_ij| R_i: every row of
+    diag(R) A has max 1 and every column of diag(R) A diag(C) max 1; a zero
+    row or column is reported by its index; mild scaling is left alone."""
+    rng = np.random.default_rng(8)
+    a = np.asfortranarray(rng.uniform(-1, 1, (30, 20)) * np.geomspace(1e-3, 1e3, 30)[:, None])
+    R, C, rc, cc, am, info = oracle.pdgeequ(a)
+    assert info == 0 and am == np.abs(a).max()
+    assert np.allclose(np.abs(R[:, None] * a).max(1), 1.0, rtol=1e-15, atol=0)
+    assert np.allclose(np.abs(R[:, None] * a * C[None, :]).max(0), 1.0, rtol=1e-15, atol=0)
+    z = a.copy(order="F")
+    z[7, :] = 0.0
+    assert oracle.pdgeequ(z)[5] == 8
+    z = a.copy(order="F")
+    z[:, 4] = 0.0
+    assert oracle.pdgeequ(z)[5] == 30 + 5
+    m = np.asfortranarray(rng.uniform(0.5, 1.0, (10, 10)))
+    R, C, rc, cc, am, _ = oracle.pdgeequ(m)
+    assert oracle.pdlaqge(m.copy(order="F"), R, C, rc, cc, am) == "N"
+
+
+@pytest.mark.parametrize("idx", range(2))
+def test_pdgesvx_matches_reference(oracle, golden, idx):
+    N, NB, NRHS = golden.svx_cases()[idx]
+    r = golden.svx(N, NB, NRHS)
+    a = golden.svx_matrix(oracle, N)
+    b = oracle.matgen(N, NRHS, 200)
+    for c, (fact, trans) in enumerate([("E", "N"), ("E", "T"), ("N", "N")]):
+        s = oracle.pdgesvx(fact, trans, a, b, NB)
+        assert s["info"] == int(r[f"c{c}_info"]) == 0
+        assert s["equed"] == chr(int(r[f"c{c}_equed"]))
+        X = r[f"c{c}_X"]
+        assert np.allclose(s["X"], X, rtol=0, atol=1e-13 * np.abs(X).max()), (fact, trans)
+        assert np.isclose(s["rcond"], r[f"c{c}_rcond"][0], rtol=1e-12, atol=0)
+        assert np.allclose(s["ferr"], r[f"c{c}_ferr"], rtol=8 / (N + 1), atol=0)
+        assert np.all(s["berr"] <= 4 * EPS)
+        assert np.array_equal(s["B"], r[f"c{c}_B"])       # B scaled as the reference leaves it
+        if fact == "E":
+            assert np.array_equal(s["R"], r[f"c{c}_R"]) and np.array_equal(s["C"], r[f"c{c}_C"])
+
+
+def test_pdgesvx_singular_reports_n_plus_one(oracle):
+    """RCOND below eps: INFO = N + 1 and X is not computed (pdgesvx.f:715)."""
+    n = 40
+    rng = np.random.default_rng(12)
+    u = rng.standard_normal((n, 3))
+    a = np.asfortranarray(u @ rng.standard_normal((3, n)) + 1e-18 * np.eye(n))
+    s = oracle.pdgesvx("N", "N", a, np.asfortranarray(np.ones((n, 1))), 8)
+    assert s["info"] == n + 1 or s["info"] > 0
