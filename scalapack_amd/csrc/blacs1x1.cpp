@@ -70,7 +70,14 @@ bool parse_first_value(const std::string &line, double *v)
     while (j < line.size() && !std::isspace((unsigned char)line[j]) && line[j] != ',') j++;
     std::string tok = line.substr(i, j - i);
     for (char &c : tok)
-        if (c == 'D' || c == 'd') c = 'E';
+        if (c == 'D' || c == 'd' || c == 'q' || c == 'Q') c = 'E';
+    // Fortran also takes an exponent without its letter ("0.123-299", as
+    // D30.18 writes three-digit exponents): a sign after the first digit
+    if (tok.find_first_of("Ee") == std::string::npos) {
+        const size_t k = tok.find_last_of("+-");
+        if (k != std::string::npos && k > 0 && std::isdigit((unsigned char)tok[k - 1]))
+            tok.insert(k, 1, 'E');
+    }
     char *end = nullptr;
     *v = std::strtod(tok.c_str(), &end);
     return end && *end == '\0';
@@ -126,7 +133,8 @@ void blacs_abort_(const int *, const int *errornum)
     std::fprintf(stderr, "{0,0}, pnum=0, Contxt=0, killed other procs, exiting with error #%d.\n",
                  *errornum);
     std::fflush(stderr);
-    std::exit(*errornum);
+    // an abort never reports success to the operating system
+    std::exit(*errornum != 0 ? *errornum : 1);
 }
 
 void blacs_barrier_(const int *, const char *, size_t) {}

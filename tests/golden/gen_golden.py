@@ -19,7 +19,10 @@ IPIV, INFO, the solution X and the tester's scaled residual.  Cases:
     (N, NB, NRHS) in AUX_CASES with A = PDMATGEN(100), B = PDMATGEN(200);
   * equilibration and the expert driver (ref_lu svx): PDGEEQU, PDLAQGE and
     PDGESVX (FACT E/TRANS N, E/T, N/N) on PDMATGEN(100) with exact
-    power-of-two row and column scales, for SVX_CASES.
+    power-of-two row and column scales, for SVX_CASES;
+  * ASCII matrix I/O (ref_lu io): the reference's PDLAWRITE file of a 9 x 6
+    matrix with special values, byte for byte, and its PDLAREAD of
+    EXAMPLE/DSCAEXMAT.dat.
 
 Usage (in the build container, where /root/reference exists):
     make -C oracle ref && python tests/golden/gen_golden.py [--skip-c2]
@@ -160,6 +163,21 @@ def run_svx(N, NB, NRHS) -> dict:
     return r
 
 
+def run_io() -> None:
+    """ref_lu io: the reference's PDLAWRITE of a 9 x 6 matrix with special
+    values (kept byte for byte as pdlawrite_9x6.txt) and its PDLAREAD of
+    EXAMPLE/DSCAEXMAT.dat into a 12 x 8 array (pdlaread_dscaexmat.npy)."""
+    with tempfile.TemporaryDirectory() as td:
+        txt, raw = Path(td) / "w.txt", Path(td) / "r.bin"
+        env = dict(os.environ, MKL_THREADING_LAYER="SEQUENTIAL")
+        subprocess.run([str(REF_LU), "io", "9", "6", "4", str(txt),
+                        str(REFERENCE / "EXAMPLE" / "DSCAEXMAT.dat"), str(raw)], check=True,
+                       capture_output=True, env=env, cwd=td)
+        (HERE / "pdlawrite_9x6.txt").write_bytes(txt.read_bytes())
+        r = np.fromfile(raw, np.float64).reshape(8, 12).T.copy(order="F")
+        np.save(HERE / "pdlaread_dscaexmat.npy", r, allow_pickle=False)
+
+
 def read_dat(path: Path) -> np.ndarray:
     """PDLAREAD's format (TOOLS/pdlaread.f:67-116): 'M N', then values down
     the columns, one per line, Fortran D exponents."""
@@ -274,6 +292,8 @@ def main() -> None:
                 svx[f"N{N}_NB{NB}_NRHS{NRHS}_{k}"] = v
         np.savez_compressed(HERE / "svx_cases.npz", **svx)
         print("svx:", SVX_CASES)
+        run_io()
+        print("io: pdlawrite_9x6.txt, pdlaread_dscaexmat.npy")
         return
 
     # LU.dat cases
@@ -325,6 +345,7 @@ def main() -> None:
             svx[f"N{N}_NB{NB}_NRHS{NRHS}_{k}"] = v
     np.savez_compressed(HERE / "svx_cases.npz", **svx)
     print("svx:", SVX_CASES)
+    run_io()
 
     # 6 x 6 example
     a6 = read_dat(REFERENCE / "EXAMPLE" / "DSCAEXMAT.dat")
