@@ -146,7 +146,10 @@ int sla_set_stream(sla_handle_t h, void *stream);
  * others to the panel), "lacon_lapack" (0/1, default 0: the condition and
  * forward-error estimates of sla_dgecon / sla_dgerfs / pdgecon_ / pdgerfs_
  * return what the reference's PDLACON returns; 1: LAPACK DLACON's
- * iteration, never smaller and usually closer to the true norm). */
+ * iteration, never smaller and usually closer to the true norm),
+ * "update_lds_kb" (0..160, default 112: LDS reserved per workgroup of the FP32
+ * trailing update so that one of its workgroups occupies a CU and the panel
+ * leaf running beside it finds registers; 0 = none). */
 int sla_set_param(sla_handle_t h, const char *name, int value);
 int sla_get_param(sla_handle_t h, const char *name, int *value);
 const char *sla_last_error(void);

@@ -54,6 +54,16 @@ __device__ __forceinline__ void argmax_fold(T &v, int &i, T v2, int i2)
         i = i2;
     }
 }
+// The same carrying the signed value at the winning row.
+template <typename T>
+__device__ __forceinline__ void argmax_fold3(T &v, int &i, T &sv, T v2, int i2, T sv2)
+{
+    if (v2 > v || (v2 == v && i2 < i)) {
+        v = v2;
+        i = i2;
+        sv = sv2;
+    }
+}
 
 template <typename T, int W>
 __device__ void factor_system(int n, T *__restrict__ a, int lda, int *__restrict__ piv, int &info,
@@ -62,59 +72,89 @@ __device__ void factor_system(int n, T *__restrict__ a, int lda, int *__restrict
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int kb = 0; kb < n; kb += W) {
         const int w = min(W, n - kb);
-        // ---- panel (unblocked), columns kb .. kb+w-1
-        for (int j = kb; j < kb + w; j++) {
-            T best = T(-1);
-            int bi = INT_MAX;
-            const T *col = a + (size_t)j * lda;
-            for (int i = j + tid; i < n; i += BT) {
-                const T v = fabs(col[i]);
+        // ---- panel (unblocked), columns kb .. kb+w-1.  Thread t owns rows
+        // j+1+t, j+1+t+256, ... of every step, so the rank-1 update of
+        // column j and the pivot search of column j+1 run in one pass over
+        // those rows with no barrier between them.  The search carries the
+        // signed value of the winning row, and the owner of row j (thread 0)
+        // publishes A(j, j) before the barrier, so after it no thread reads
+        // the two elements that the interchange then rewrites; the
+        // interchange of the other columns and the scaling of column j touch
+        // disjoint data: two workgroup barriers per column.
+        T best = T(-1), bsv = T(0);
+        int bi = INT_MAX;
+        {
+            const T *col = a + (size_t)kb * lda;
+            for (int i = kb + tid; i < n; i += BT) {
+                const T x = col[i], v = fabs(x);
                 if (v > best) {
                     best = v;
                     bi = i;
+                    bsv = x;
                 }
             }
-            for (int o = 32; o > 0; o >>= 1) argmax_fold(best, bi, __shfl_xor(best, o), __shfl_xor(bi, o));
+            if (tid == 0) s_v[4] = col[kb];
+        }
+        for (int j = kb; j < kb + w; j++) {
+            T *cj = a + (size_t)j * lda;
+            for (int o = 32; o > 0; o >>= 1)
+                argmax_fold3(best, bi, bsv, __shfl_xor(best, o), __shfl_xor(bi, o), __shfl_xor(bsv, o));
             if (lane == 0) {
                 s_v[wave] = best;
                 s_i[wave] = bi;
+                s_v[5 + wave] = bsv;
             }
             __syncthreads();
             best = s_v[0];
             bi = s_i[0];
-            for (int q = 1; q < 4; q++) argmax_fold(best, bi, s_v[q], s_i[q]);
+            bsv = s_v[5];
+            for (int q = 1; q < 4; q++) argmax_fold3(best, bi, bsv, s_v[q], s_i[q], s_v[5 + q]);
+            const T ajj = s_v[4];
             const int p = bi < n ? bi : j;  // no comparable entry (all NaN): keep row j
-            const T pv = col[p];
-            __syncthreads();  // s_v / s_i are reused by the next column
+            const T pv = bi < n ? bsv : ajj;
             if (tid == 0) piv[j] = p + 1;
             if (pv != T(0)) {
                 if (p != j)
                     for (int c = tid; c < n; c += BT) {
+                        if (c == j) continue;
                         T *cc = a + (size_t)c * lda;
                         const T t = cc[j];
                         cc[j] = cc[p];
                         cc[p] = t;
                     }
-                __syncthreads();
                 const T r = T(1) / pv;
-                T *cj = a + (size_t)j * lda;
-                for (int i = j + 1 + tid; i < n; i += BT) cj[i] *= r;
+                // column j: rows j+1.. scaled (row p receives the old A(j, j))
+                for (int i = j + 1 + tid; i < n; i += BT) cj[i] = (i == p ? ajj : cj[i]) * r;
+                if (tid == 0) cj[j] = pv;
             } else if (info == 0) {
                 info = j + 1;
             }
-            __syncthreads();
-            // rank-1 update of the panel's columns right of j
-            const int rows = n - j - 1, cols = kb + w - j - 1;
-            if (rows > 0 && cols > 0) {
-                const T *lj = a + (size_t)j * lda;
-                for (int idx = tid; idx < rows * cols; idx += BT) {
-                    const int i = j + 1 + idx % rows, c = j + 1 + idx / rows;
+            __syncthreads();  // row j (all columns) and column j are final
+            // rank-1 update of the panel's columns right of j, fused with the
+            // pivot search of column j + 1 over the same rows
+            best = T(-1);
+            bi = INT_MAX;
+            bsv = T(0);
+            const int cend = kb + w;
+            for (int i = j + 1 + tid; i < n; i += BT) {
+                const T l = cj[i];
+                for (int c = j + 1; c < cend; c++) {
                     T *cc = a + (size_t)c * lda;
-                    cc[i] = fma(-lj[i], cc[j], cc[i]);
+                    cc[i] = fma(-l, cc[j], cc[i]);
+                }
+                if (j + 1 < cend) {
+                    const T x = a[(size_t)(j + 1) * lda + i], v = fabs(x);
+                    if (v > best) {
+                        best = v;
+                        bi = i;
+                        bsv = x;
+                    }
                 }
             }
-            __syncthreads();
+            // row j+1 is owned by thread 0 in this pass: A(j+1, j+1) is final
+            if (tid == 0 && j + 1 < cend && j + 1 < n) s_v[4] = a[(size_t)(j + 1) * (lda + 1)];
         }
+        __syncthreads();  // the panel is final before U12 reads L11
         const int c0 = kb + w;
         if (c0 >= n) break;
         // ---- U12 = L11^{-1} A12 (unit lower L11), one thread per column
@@ -242,7 +282,7 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int batch, const int *
 {
     __shared__ T Ls[UT][W + 1];
     __shared__ T Us[W][UT + 1];
-    __shared__ T s_v[4];
+    __shared__ T s_v[9];  // [0..3] wave maxima, [4] A(j, j), [5..8] signed values
     __shared__ int s_i[4];
     __shared__ int s_q;
     for (;;) {

@@ -981,7 +981,8 @@ int sla_set_param(sla_handle_t h, const char *name, int value)
     static const char *known[] = {"nb",      "leaf_w",  "panel_wgs",        "profile",
                                   "ir_max_iter", "gmres_m", "ir_gmres_on_stall", "gmres_tol_exp",
                                   "pivoting", "lookahead", "update_cus", "lacon_lapack",
-                                  "batched_w", "batched_wgs_per_cu", "batched_graph"};
+                                  "batched_w", "batched_wgs_per_cu", "batched_graph",
+                                  "update_lds_kb"};
     if (std::find_if(std::begin(known), std::end(known), [&](const char *x) { return k == x; }) ==
         std::end(known)) {
         g_last_error = "sla_set_param: unknown parameter " + k;
@@ -1002,6 +1003,7 @@ int sla_set_param(sla_handle_t h, const char *name, int value)
     else if (k == "batched_w" && (value == 8 || value == 16 || value == 32)) H->batched_w = value;
     else if (k == "batched_wgs_per_cu" && value >= 1 && value <= 8) H->batched_wgs_per_cu = value;
     else if (k == "batched_graph" && (value == 0 || value == 1)) H->batched_graph = value;
+    else if (k == "update_lds_kb" && value >= 0 && value <= 160) H->update_lds_kb = value;
     else if (k == "gmres_tol_exp" && value >= 1 && value <= 15) H->gmres_tol = std::pow(10.0, -value);
     else {
         g_last_error = "sla_set_param: value out of range for " + k;
@@ -1029,6 +1031,7 @@ int sla_get_param(sla_handle_t h, const char *name, int *value)
     else if (k == "batched_w") *value = H->batched_w;
     else if (k == "batched_wgs_per_cu") *value = H->batched_wgs_per_cu;
     else if (k == "batched_graph") *value = H->batched_graph;
+    else if (k == "update_lds_kb") *value = H->update_lds_kb;
     else if (k == "gmres_tol_exp") *value = (int)std::lround(-std::log10(H->gmres_tol));
     else {
         g_last_error = "sla_get_param: unknown parameter " + k;

@@ -401,7 +401,7 @@ __global__ void convert_f64_f32_kernel(int m, int n, const double *__restrict__ 
 static bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
 
 void sgemm_nn_sub(int m, int n, int k, const float *A, int lda, const float *B, int ldb, float *C,
-                  int ldc, hipStream_t s, bool trailing, int max_wgs)
+                  int ldc, hipStream_t s, bool trailing, int max_wgs, int reserve_lds)
 {
     if (m <= 0 || n <= 0 || k <= 0) return;
     if (!trailing && gemm_small<float>(m, n, k, A, lda, B, ldb, C, ldc, s)) return;
@@ -410,18 +410,33 @@ void sgemm_nn_sub(int m, int n, int k, const float *A, int lda, const float *B, 
         aligned16(A) && aligned16(B) && aligned16(C)) {
         const int tiles = (mf / 128) * (nf / 128);
         const int grid = (max_wgs >= 8 && tiles > max_wgs) ? (max_wgs / 8) * 8 : tiles;
-        if (trailing)
-            hipLaunchKernelGGL(sgemm_nn_sub_direct_kernel<1>, dim3(grid), dim3(256), 0, s, mf, nf,
-                               k, A, lda, B, ldb, C, ldc);
+        if (trailing) {
+            // reserve_lds: LDS the kernel does not use, reserved so that at
+            // most floor(LDS per CU / reserve_lds) workgroups share a CU and
+            // registers stay free for the panel running beside the update
+            if (reserve_lds > 0) {
+                static int attr_set = 0;
+                if (reserve_lds > attr_set) {
+                    SLA_HIP(hipFuncSetAttribute(
+                        reinterpret_cast<const void *>(&sgemm_nn_sub_direct_kernel<1>),
+                        hipFuncAttributeMaxDynamicSharedMemorySize, reserve_lds));
+                    attr_set = reserve_lds;
+                }
+            }
+            hipLaunchKernelGGL(sgemm_nn_sub_direct_kernel<1>, dim3(grid), dim3(256),
+                               (unsigned)std::max(0, reserve_lds), s, mf, nf, k, A, lda, B, ldb, C,
+                               ldc);
+        }
         else
             hipLaunchKernelGGL(sgemm_nn_sub_direct_kernel<0>, dim3((mf / 128) * (nf / 128)),
                                dim3(256), 0, s, mf, nf, k, A, lda, B, ldb, C, ldc);
         SLA_LAUNCH_CHECK();
         if (m > mf)
-            sgemm_nn_sub(m - mf, n, k, A + mf, lda, B, ldb, C + mf, ldc, s, trailing, max_wgs);
+            sgemm_nn_sub(m - mf, n, k, A + mf, lda, B, ldb, C + mf, ldc, s, trailing, max_wgs,
+                         reserve_lds);
         if (n > nf)
             sgemm_nn_sub(mf, n - nf, k, A, lda, B + (size_t)nf * ldb, ldb, C + (size_t)nf * ldc,
-                         ldc, s, trailing, max_wgs);
+                         ldc, s, trailing, max_wgs, reserve_lds);
         return;
     }
     hipLaunchKernelGGL((gemm_lp_edge_kernel<float, false>), dim3(ceil_div(m, 16), ceil_div(n, 16)),

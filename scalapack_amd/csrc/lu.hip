@@ -418,7 +418,8 @@ void Lu<float>::update(int m, int n, int k, const float *l21, const float *u12, 
     if (m <= 0 || n <= 0 || k <= 0) return;
     if (schur == SCHUR_NATIVE) {
         h.timing_begin(CAT_GEMM, s);
-        sgemm_nn_sub(m, n, k, l21, lda, u12, lda, c, lda, s, true, 2 * h.update_cus);
+        sgemm_nn_sub(m, n, k, l21, lda, u12, lda, c, lda, s, true, 2 * h.update_cus,
+                     h.update_lds_kb * 1024);
         h.timing_end(2.0 * m * n * k, s);
         return;
     }

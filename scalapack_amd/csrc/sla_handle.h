@@ -131,6 +131,9 @@ struct Handle {
     int update_cus = 0;    // > 0: the trailing update runs on at most this many CUs' worth
                            // of workgroups (the rest stay free for the panel)
     int num_cus = 256;     // CUs of the device
+    int update_lds_kb = 112;  // LDS (KiB) reserved per workgroup of the FP32 trailing update:
+                              // one of its workgroups per CU, leaving registers to the panel
+                              // leaf that runs beside it (0: no reservation)
     int lacon_lapack = 0;  // condition / error estimates: 0 the reference's PDLACON result,
                            // 1 LAPACK DLACON's iteration (estimate kept across steps)
     // batched solves

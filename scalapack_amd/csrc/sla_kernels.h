@@ -22,8 +22,11 @@ bool gemm_small(int m, int n, int k, const T *A, int lda, const T *B, int ldb, T
 
 // ---- low-precision Schur-complement updates (FP32 storage) -----------------
 // C[m x n] -= A[m x k] * B[k x n], FP32 operands on FP32 MFMA.
+// reserve_lds: bytes of LDS reserved per workgroup of the trailing launch
+// (limits the workgroups sharing a CU; the kernel itself does not use it).
 void sgemm_nn_sub(int m, int n, int k, const float *A, int lda, const float *B, int ldb, float *C,
-                  int ldc, hipStream_t s, bool trailing = false, int max_wgs = 0);
+                  int ldc, hipStream_t s, bool trailing = false, int max_wgs = 0,
+                  int reserve_lds = 0);
 // Same with BF16 operands packed k-contiguous: Ap row-major m x k (ld ldak),
 // Bp column-major k x n (ld ldbk); FP32 accumulation into C.
 void gemm_bf16_nn_sub(int m, int n, int k, const __bf16 *Ap, int ldak, const __bf16 *Bp, int ldbk,
