@@ -139,7 +139,7 @@ int sla_set_stream(sla_handle_t h, void *stream);
  * (0/1, default 1), "gmres_m" (GMRES steps per correction, default 50),
  * "gmres_tol_exp" (e: relative reduction 10^-e per correction, default 6),
  * "pivoting" (FP32-storage factorizations: 0 partial pivoting, 1 tournament
- * pivoting of the panel leaves, default 0; the FP64 path always pivots as
+ * pivoting of the panel leaves, default 1; the FP64 path always pivots as
  * PDGETF2), "lookahead" (0/1, default 1: block column p+1 is factored on a
  * second stream during p's update), "update_cus" (0 = all; else the trailing
  * update runs as a persistent grid sized for this many CUs, leaving the

@@ -25,7 +25,8 @@
 //      other row r is L(r) = A(src(r)) U11^{-1}, where src(r) is the row the
 //      interchanges bring to position r (a displaced top row comes from the
 //      published copy, any other row is the workgroup's own); the
-//      interchanges are applied to the rest of the block column in order.
+//      interchanges reach the rest of the block column as their net moves
+//      (at most 2W rows change place), one gather and one scatter a column.
 #include "sla_common.h"
 #include "sla_kernels.h"
 
@@ -178,7 +179,6 @@ __global__ __launch_bounds__(TPT) void panel_tslu_kernel(
     __shared__ T s_prow[TW];
     __shared__ int s_k2[TW], s_p2[TW];
     __shared__ T s_lu[TW][TW + 1];
-    __shared__ int s_x[TW];  // position of the k-th interchange's partner
     __shared__ int s_dst[2 * TW], s_src[2 * TW], s_np;
 
     if (!FULL)
@@ -290,10 +290,7 @@ __global__ __launch_bounds__(TPT) void panel_tslu_kernel(
             const unsigned long long bx = __ballot(pos >= 0 && at == pk);
             const int xl = bx ? __builtin_ctzll(bx) : k;  // always found: pivots are distinct rows
             const int xpos = __builtin_amdgcn_readlane(pos, xl);
-            if (lane == 0) {
-                s_x[k] = xpos;
-                if (g == 0) ipiv[row0 + k] = row0 + xpos + 1;
-            }
+            if (lane == 0 && g == 0) ipiv[row0 + k] = row0 + xpos + 1;
             const int at_k = __builtin_amdgcn_readlane(at, k);
             const int at_x = __builtin_amdgcn_readlane(at, xl);
             if (lane == k) at = at_x;
@@ -354,19 +351,18 @@ __global__ __launch_bounds__(TPT) void panel_tslu_kernel(
         }
     }
 
-    // 5c. the interchanges on the block column's other columns, in order;
-    //     column e always by thread e / G of workgroup e % G
+    // 5c. the interchanges on the block column's other columns, as their net
+    //     moves (position s_dst[i] receives the row at s_src[i], at most 2 W
+    //     of them): one wave per column, lane i moves one element; the whole
+    //     wave's load completes before its store, so no move reads a value
+    //     another one already overwrote
     const int ne = lcols + rcols;
-    for (int e = g + G * tid; e < ne; e += G * TPT) {
-        const int cc = (e < lcols) ? (e - lcols) : (wa + e - lcols);
-        T *col = A + (ptrdiff_t)cc * lda;
-        for (int k = 0; k < wa; k++) {
-            const int xk = s_x[k];
-            if (xk != k) {
-                const T a = col[k], b = col[xk];
-                col[k] = b;
-                col[xk] = a;
-            }
+    for (int e = g * TNW + wave; e < ne; e += G * TNW) {
+        if (lane < np) {
+            const int cc = (e < lcols) ? (e - lcols) : (wa + e - lcols);
+            T *col = A + (ptrdiff_t)cc * lda;
+            const T x = col[s_src[lane]];
+            col[s_dst[lane]] = x;
         }
     }
 }

@@ -126,7 +126,7 @@ struct Handle {
     int nb = 256;          // block column width of the outer factorization
     int leaf_w = 32;       // panel leaf width (columns per device-wide-barrier kernel)
     int panel_wgs = 32;    // preferred workgroups per leaf launch
-    int pivoting = 0;      // FP32-storage factorization: 0 partial, 1 tournament
+    int pivoting = 1;      // FP32-storage factorization: 0 partial, 1 tournament
     int lookahead = 1;     // factor block column p+1 on a second stream during p's update
     int update_cus = 0;    // > 0: the trailing update runs on at most this many CUs' worth
                            // of workgroups (the rest stay free for the panel)

@@ -12,18 +12,31 @@
 #include "sla_common.h"
 #include "sla_kernels.h"
 
+#include <climits>
+
 namespace sla {
 
 namespace {
 
 constexpr int PAIRS_MAX_PIV = 512;
+constexpr int PAIRS_HASH_BITS = 10, PAIRS_HASH = 1 << PAIRS_HASH_BITS;
 
 // One workgroup.  Slots 0..n-1 stand for rows k1..k2-1; slot n+i stands for
-// pivot row ipiv[k1+i] outside [k1, k2) at its first occurrence (factorization
-// pivots are >= their row, a caller's IPIV may hold any row).  The
-// interchanges are replayed on slot contents (row ids), then each slot s
-// emits the pair (row_of(s), content(s)).  A pivot row outside [0, nrows)
+// pivot row ipiv[k1+i] outside [k1, k2) at its first occurrence.  Step i
+// exchanges the contents (row ids) of slot i and slot s_i; each slot x then
+// emits the pair (row_of(x), content(x)).  A pivot row outside [0, nrows)
 // raises *bad and the block emits no pairs.
+//
+// Factorization pivots are >= their row, so s_i >= i and slot i is final
+// after step i.  Then the replay needs no sequential loop: with val(i) the
+// content of slot i just before step i,
+//   val(i)   = val(pred(i)), pred(i) = last step before i whose s is i,
+//              or orig(i) when there is none (pointer jumping, log2 n rounds);
+//   final(i) = val(i) if s_i = i, else val(l) for the last step l < i with
+//              s_l = s_i, else orig(s_i);
+//   final(x) for an outside slot = val(last step with s = x), else orig(x).
+// A caller's IPIV may point backwards (any row): that block is replayed in
+// order by one thread.
 __global__ __launch_bounds__(512) void laswp_pairs_kernel(const int *__restrict__ ipiv, int k1,
                                                           int k2, int nrows,
                                                           int *__restrict__ pairs,
@@ -31,15 +44,29 @@ __global__ __launch_bounds__(512) void laswp_pairs_kernel(const int *__restrict_
 {
     __shared__ int tgt[PAIRS_MAX_PIV];
     __shared__ int first[PAIRS_MAX_PIV];
-    __shared__ int content[2 * PAIRS_MAX_PIV];
-    __shared__ int s_bad;
+    __shared__ int slot[PAIRS_MAX_PIV];
+    __shared__ int prev[PAIRS_MAX_PIV];
+    __shared__ int jump[PAIRS_MAX_PIV];
+    __shared__ int val[PAIRS_MAX_PIV];
+    __shared__ int orig[2 * PAIRS_MAX_PIV];
+    __shared__ int last[2 * PAIRS_MAX_PIV];
+    __shared__ int cnt[2 * PAIRS_MAX_PIV];
+    __shared__ int hkey[PAIRS_HASH], hval[PAIRS_HASH];
+    __shared__ int s_bad, s_back;
     const int n = k2 - k1, tid = threadIdx.x;
-    if (tid == 0) s_bad = 0;
+    if (tid == 0) {
+        s_bad = 0;
+        s_back = 0;
+    }
     __syncthreads();
     for (int i = tid; i < n; i += blockDim.x) {
         const int t = ipiv[k1 + i] - 1;
         tgt[i] = t;
         if (t < 0 || t >= nrows) s_bad = 1;
+    }
+    for (int x = tid; x < PAIRS_HASH; x += blockDim.x) {
+        hkey[x] = -1;
+        hval[x] = INT_MAX;
     }
     __syncthreads();
     if (s_bad) {
@@ -49,40 +76,114 @@ __global__ __launch_bounds__(512) void laswp_pairs_kernel(const int *__restrict_
         }
         return;
     }
+    // first occurrence of each outside row: open-addressing table (<= 512
+    // keys in PAIRS_HASH entries), the lowest step per key by atomicMin
     for (int i = tid; i < n; i += blockDim.x) {
         const int t = tgt[i];
-        int f = -1;
         if (t < k1 || t >= k2) {
-            f = i;
-            for (int i2 = 0; i2 < i; i2++)
-                if (tgt[i2] == t) { f = i2; break; }
+            int hx = (int)(((unsigned)t * 2654435761u) >> (32 - PAIRS_HASH_BITS));
+            for (;;) {
+                const int old = atomicCAS(&hkey[hx], -1, t);
+                if (old == -1 || old == t) break;
+                hx = (hx + 1) & (PAIRS_HASH - 1);
+            }
+            atomicMin(&hval[hx], i);
+            first[i] = hx;  // the entry, resolved below
         }
-        first[i] = f;
     }
     __syncthreads();
-    for (int s = tid; s < 2 * n; s += blockDim.x) {
-        if (s < n) content[s] = k1 + s;
-        else content[s] = (first[s - n] == s - n) ? tgt[s - n] : -1;
+    for (int i = tid; i < n; i += blockDim.x) {
+        const int t = tgt[i];
+        first[i] = (t < k1 || t >= k2) ? hval[first[i]] : -1;
     }
     __syncthreads();
-    if (tid == 0) {
-        for (int i = 0; i < n; i++) {
-            const int t = tgt[i];
-            const int s = (t >= k1 && t < k2) ? (t - k1) : (n + first[i]);
-            if (s != i) {
-                const int x = content[i];
-                content[i] = content[s];
-                content[s] = x;
+    for (int x = tid; x < 2 * n; x += blockDim.x) {
+        orig[x] = (x < n) ? k1 + x : ((first[x - n] == x - n) ? tgt[x - n] : -1);
+        last[x] = -1;
+        cnt[x] = 0;
+    }
+    for (int i = tid; i < n; i += blockDim.x) {
+        const int t = tgt[i];
+        const int sl = (t >= k1 && t < k2) ? (t - k1) : (n + first[i]);
+        slot[i] = sl;
+        if (sl < i) s_back = 1;
+    }
+    __syncthreads();
+    if (s_back) {
+        // in-order replay on orig (becomes the contents)
+        if (tid == 0) {
+            for (int i = 0; i < n; i++) {
+                const int sl = slot[i];
+                if (sl != i) {
+                    const int x = orig[i];
+                    orig[i] = orig[sl];
+                    orig[sl] = x;
+                }
             }
         }
+        __syncthreads();
+        for (int x = tid; x < 2 * n; x += blockDim.x) {
+            const int row = (x < n) ? k1 + x : ((first[x - n] == x - n) ? tgt[x - n] : -1);
+            pairs[2 * x] = row;
+            pairs[2 * x + 1] = (row >= 0) ? orig[x] : -1;
+        }
+        if (tid == 0) *count = 2 * n;
+        return;
+    }
+    for (int i = tid; i < n; i += blockDim.x) {
+        const int sl = slot[i];
+        if (sl != i) {
+            atomicMax(&last[sl], i);
+            atomicAdd(&cnt[sl], 1);
+        }
     }
     __syncthreads();
-    for (int s = tid; s < 2 * n; s += blockDim.x) {
-        int row;
-        if (s < n) row = k1 + s;
-        else row = (first[s - n] == s - n) ? tgt[s - n] : -1;
-        pairs[2 * s] = row;
-        pairs[2 * s + 1] = (row >= 0) ? content[s] : -1;
+    // the previous step with the same slot: a scan only where the slot is
+    // shared (a row chosen as pivot twice)
+    for (int i = tid; i < n; i += blockDim.x) {
+        const int sl = slot[i];
+        int l = -1;
+        if (sl != i && cnt[sl] > 1)
+            for (int j = i - 1; j >= 0; j--)
+                if (slot[j] == sl) {
+                    l = j;
+                    break;
+                }
+        prev[i] = l;
+    }
+    __syncthreads();
+    // n <= 512 = blockDim: one step per thread
+    const bool mine = tid < n;
+    int p = 0;
+    if (mine) {
+        p = (last[tid] >= 0) ? last[tid] : tid;
+        jump[tid] = p;
+    }
+    __syncthreads();
+    for (int span = 1; span < n; span *= 2) {
+        const int q = mine ? jump[p] : 0;
+        __syncthreads();
+        if (mine) {
+            p = q;
+            jump[tid] = p;
+        }
+        __syncthreads();
+    }
+    if (mine) val[tid] = orig[p];
+    __syncthreads();
+    for (int x = tid; x < 2 * n; x += blockDim.x) {
+        const int row = orig[x];
+        int f;
+        if (x < n) {
+            const int sl = slot[x];
+            const int l = prev[x];
+            f = (sl == x) ? val[x] : (l >= 0 ? val[l] : orig[sl]);
+        } else {
+            const int l = last[x];
+            f = (l >= 0) ? val[l] : row;
+        }
+        pairs[2 * x] = row;
+        pairs[2 * x + 1] = (row >= 0) ? f : -1;
     }
     if (tid == 0) *count = 2 * n;
 }

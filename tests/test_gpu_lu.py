@@ -203,12 +203,18 @@ def test_pdlaswp_rows_and_columns(oracle, ctx):
     a = np.asfortranarray(rng.standard_normal((M, N)))
     desca, _ = api.descinit(M, N, 4, 4, 0, 0, ctx, M)
     ipiv = np.array([rng.integers(i + 1, M + 1) for i in range(M)], np.int32)
-    for direc in ("F", "B"):
-        g = a.copy(order="F")
-        oracle.pdlaswp(direc, g, 3, 20, 2, 17, ipiv)
-        t = a.copy(order="F")
-        api.pdlaswp(direc, "R", 20, t, 1, 3, desca, 2, 17, ipiv)
-        assert np.array_equal(t, g), direc
+    # pivots a factorization can produce (>= their row), pivots pointing
+    # anywhere (a caller's IPIV), and a few rows chosen many times
+    ipiv_any = rng.integers(1, M + 1, M).astype(np.int32)
+    ipiv_rep = np.array([max(i + 1, int(rng.choice([i + 1, 25, 33, 40]))) for i in range(M)],
+                        np.int32)
+    for piv in (ipiv, ipiv_any, ipiv_rep):
+        for direc in ("F", "B"):
+            g = a.copy(order="F")
+            oracle.pdlaswp(direc, g, 3, 20, 2, 17, piv)
+            t = a.copy(order="F")
+            api.pdlaswp(direc, "R", 20, t, 1, 3, desca, 2, 17, piv)
+            assert np.array_equal(t, g), direc
     cp = np.array([rng.integers(i + 1, N + 1) for i in range(N)], np.int32)
     t = a.copy(order="F")
     api.pdlaswp("F", "C", 25, t, 4, 1, desca, 3, 12, cp)
