@@ -78,9 +78,6 @@ Handle::Handle()
     pairs = alloc_dev<int>(2 * 2048);
     pair_counts = alloc_dev<int>(2);
 
-    trsv_max_blocks = 1 << 16;
-    trsv_flags = alloc_dev<int>(trsv_max_blocks);
-    fill_int(trsv_flags, trsv_max_blocks, 0, stream);
     fill_int(ctl, 64, 0, stream);
     SLA_HIP(hipStreamSynchronize(stream));
 }
@@ -90,7 +87,7 @@ Handle::~Handle()
     (void)hipStreamSynchronize(stream);
     batched_plan_free(batched);
     for (void *p : {(void *)ctl, (void *)slot_cand_v, (void *)slot_cand_r, (void *)slot_rows,
-                    (void *)slot_rowj, (void *)pairs, (void *)pair_counts, (void *)trsv_flags,
+                    (void *)slot_rowj, (void *)pairs, (void *)pair_counts,
                     (void *)perm, (void *)work})
         if (p) hipFree(p);
     for (auto &b : bufs)
@@ -121,13 +118,20 @@ PanelWork<T> Handle::panel_work()
     return pw;
 }
 
-TrsvWork Handle::trsv_work()
+TrsvWork Handle::trsv_work(int n)
 {
     TrsvWork tw;
+    const size_t words = (size_t)ceil_div(std::max(n, 1), 64) * 64 * 16 * 2;
+    const bool grow = words * sizeof(uint64_t) > bufs.at(BUF_TRSV_MAIL).cap;
+    tw.mail = static_cast<uint64_t *>(buffer(BUF_TRSV_MAIL, words * sizeof(uint64_t)));
+    if (grow) {
+        // epoch 0 everywhere: no word of a fresh mailbox matches a launch
+        fill_u32(reinterpret_cast<unsigned *>(tw.mail), (int)(words * 2), 0u, stream);
+        SLA_HIP(hipStreamSynchronize(stream));
+    }
+    tw.mail_words = words;
     tw.ticket = ticket;
-    tw.flags = trsv_flags;
     tw.epoch = trsv_epoch;
-    tw.max_blocks = trsv_max_blocks;
     tw.abort_flag = abort_flag;
     tw.spin_limit = spin_limit;
     return tw;
@@ -533,7 +537,7 @@ void getrs_perm_device(Handle &h, bool trans, int n, int nrhs, const double *dA,
     const int chunk = std::min(nrhs, 16);
     h.ensure_work((size_t)n * chunk * sizeof(double));
     double *W = reinterpret_cast<double *>(h.work);
-    TrsvWork tw = h.trsv_work();
+    TrsvWork tw = h.trsv_work(n);
     for (int j = 0; j < nrhs; j += 16) {
         const int nr = std::min(16, nrhs - j);
         double *Bj = dB + (size_t)j * ldb;

@@ -331,7 +331,7 @@ struct Mixed {
                            st(), n, nr, Rc, ldrc, h.perm, W, n);
         SLA_LAUNCH_CHECK();
         h.timing_end(12.0 * n * nr);
-        TrsvWork tw = h.trsv_work();
+        TrsvWork tw = h.trsv_work(n);
         h.timing_begin(CAT_TRSV);
         trsv_persistent<float, float>(TrsvOp::L, n, nr, S, lds, W, n, tw, st());
         h.timing_end(2.0 * (double)n * n);
@@ -348,7 +348,7 @@ struct Mixed {
         h.timing_begin(CAT_LASWP);
         gather_rows<double>(v, n, z, n, h.perm, n, 1, st());
         h.timing_end(16.0 * n);
-        TrsvWork tw = h.trsv_work();
+        TrsvWork tw = h.trsv_work(n);
         h.timing_begin(CAT_TRSV);
         trsv_persistent<float, double>(TrsvOp::L, n, 1, S, lds, z, n, tw, st());
         h.timing_end(2.0 * (double)n * n);
@@ -514,7 +514,7 @@ void sgetrs_device(Handle &h, int n, int nrhs, const float *dA, int lda, const i
     if (!(h.perm_key_a == dA && h.perm_key_ipiv == dipiv && h.perm_m == n))
         perm_from_ipiv(h, n, dipiv);
     float *W = static_cast<float *>(h.buffer(BUF_VEC, (size_t)n * 16 * 4 + 256));
-    TrsvWork tw = h.trsv_work();
+    TrsvWork tw = h.trsv_work(n);
     fill_int(h.abort_flag, 1, 0, h.stream);
     for (int j = 0; j < nrhs; j += 16) {
         const int nr = std::min(16, nrhs - j);

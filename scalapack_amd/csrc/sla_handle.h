@@ -59,6 +59,7 @@ enum BufId {
     BUF_MLP,         // activations and gradients of the selector network
     BUF_SELECT,      // sampled submatrices of the selector's features
     BUF_MODEL,       // the selector's parameters (sla_selector_set_model)
+    BUF_TRSV_MAIL,   // epoch-tagged hand-off words of the persistent TRSV
     BUF_COUNT
 };
 
@@ -101,8 +102,7 @@ struct Handle {
 
     int *pairs = nullptr;        // two pair lists of 2048 ints
     int *pair_counts = nullptr;  // their counts
-    int *trsv_flags = nullptr;
-    int trsv_max_blocks = 0, trsv_epoch = 0;
+    int trsv_epoch = 0;
 
     int *perm = nullptr;
     int perm_cap = 0;
@@ -169,7 +169,7 @@ struct Handle {
 
     template <typename T>
     PanelWork<T> panel_work();
-    TrsvWork trsv_work();
+    TrsvWork trsv_work(int n);  // mailbox sized for triangles of order n
     void ensure_perm(int m);
     void ensure_work(size_t bytes);
 };

@@ -131,9 +131,9 @@ void trsm_llnu(int m, int n, const T *L, int ldl, T *B, int ldb, hipStream_t s);
 enum class TrsvOp { L, U, UT, LT };
 struct TrsvWork {
     int *ticket;         // workgroup ticket counter (reset before each launch)
-    int *flags;          // per 64-row block: == epoch once that block is solved
+    uint64_t *mail;      // solved values of every block, tagged with the epoch
+    size_t mail_words;   // capacity of mail (64 * 16 * 2 words per 64-row block)
     int epoch;           // host-side counter, one value per launch
-    int max_blocks;      // capacity of flags
     int *abort_flag;
     uint64_t spin_limit;
 };

@@ -12,9 +12,10 @@
 //   (SRC/pdgecon.f:342-370) for a few right-hand sides.  Those are
 //   latency-bound, so one launch covers the whole triangle: block row b
 //   (64 rows) is owned by the workgroup that takes ticket b; it accumulates
-//   -T[b, c] X[c] for every finished block c (a device-wide flag per block,
-//   bounded wait), then solves its 64 x 64 diagonal block in registers of one
-//   wave (lane = row, values broadcast by lane reads) and publishes X[b].
+//   -T[b, c] X[c] for every finished block c (X[c] read from a mailbox whose
+//   words carry the launch's epoch, bounded wait), then solves its 64 x 64
+//   diagonal block in registers of one wave (lane = row, values broadcast by
+//   lane reads) and publishes X[b].
 //   Tickets are taken in start order, so a workgroup only ever waits on
 //   workgroups that are already running.  The transposed solves read the
 //   off-diagonal block by columns (lane = column of the stored triangle,
@@ -67,7 +68,7 @@ __global__ __launch_bounds__(256) void trsm_llnu_strip_kernel(int m, int n, cons
 }
 
 // ---------------------------------------------------------------------------
-constexpr int TV_BS = 64, TV_NR = 16;
+constexpr int TV_BS = 64, TV_NR = 16, TV_PART = TV_BS / 4;
 
 // The stored matrix holds L (unit lower, below the diagonal) and U (upper).
 //   !TRANS,  FORWARD: X = L^{-1} X        !TRANS, !FORWARD: X = U^{-1} X
@@ -77,11 +78,44 @@ constexpr int TV_BS = 64, TV_NR = 16;
 // (nrhs <= NR <= 16) with leading dimension ldx; the matrix is n x n.
 // TA: storage type of the triangle; T: type of X and of the arithmetic (a
 // FP32 factorization may be applied in FP64, as GMRES-IR's preconditioner).
+//
+// Hand-off between blocks: a solved block publishes its values in a mailbox
+// of 64-bit words, each holding 32 bits of a value and the launch's epoch in
+// the upper half (a double takes two words), written with agent-coherent
+// stores.  A waiting workgroup polls the words themselves (wave 0, lane = row)
+// until every one carries the epoch -- one memory round trip from producer to
+// consumer, no separate flag and no store drain.  Stale words of earlier
+// launches hold older epochs; the mailbox is zeroed when allocated.  The next
+// off-diagonal block of the triangle is loaded into registers before the
+// wait, so the matrix read stays off the chain.
+template <typename T>
+__device__ __forceinline__ uint32_t tv_half(T v, int w)
+{
+    if constexpr (sizeof(T) == 8) {
+        const uint64_t u = __builtin_bit_cast(uint64_t, v);
+        return w ? (uint32_t)(u >> 32) : (uint32_t)u;
+    } else {
+        return __builtin_bit_cast(uint32_t, v);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T tv_join(uint32_t lo, uint32_t hi)
+{
+    if constexpr (sizeof(T) == 8) {
+        return __builtin_bit_cast(T, ((uint64_t)hi << 32) | lo);
+    } else {
+        (void)hi;
+        return __builtin_bit_cast(T, lo);
+    }
+}
+
 template <typename TA, typename T, bool FORWARD, bool TRANS, int NR>
 __global__ __launch_bounds__(256) void trsv_persistent_kernel(
-    int n, int nrhs, const TA *__restrict__ A, int lda, T *X, int ldx, int *ticket, int *flags,
-    int epoch, int *abort_flag, uint64_t spin_limit)
+    int n, int nrhs, const TA *__restrict__ A, int lda, T *X, int ldx, int *ticket,
+    uint64_t *mail, unsigned epoch, int *abort_flag, uint64_t spin_limit)
 {
+    constexpr int WPV = sizeof(T) / 4;  // mailbox words per value
     __shared__ int s_b, s_abort;
     __shared__ T Xs[TV_BS][NR];
     __shared__ T Part[4][TV_BS][NR + 1];
@@ -89,35 +123,85 @@ __global__ __launch_bounds__(256) void trsv_persistent_kernel(
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nblk = ceil_div(n, TV_BS);
 
-    if (tid == 0) s_b = atomicAdd(ticket, 1);
+    if (tid == 0) {
+        s_b = atomicAdd(ticket, 1);
+        s_abort = 0;
+    }
     __syncthreads();
     const int t = s_b;
     if (t >= nblk) return;
     const int b = FORWARD ? t : nblk - 1 - t;
     const int rb = b * TV_BS, rows = min(TV_BS, n - rb);
+    // mailbox word (block c, column j, half w, row r)
+    auto mbox = [&](int c, int j, int w, int r) -> uint64_t * {
+        return mail + ((((size_t)c * NR + j) * WPV + w) * TV_BS + r);
+    };
 
     // Diagonal block staged in LDS before any wait (it does not depend on
-    // other blocks); wave 0 lane r keeps 1 / U(r, r).
+    // other blocks); wave 0 lane r keeps 1 / U(r, r) and its right-hand side.
     for (int idx = tid; idx < TV_BS * TV_BS; idx += 256) {
         const int r = idx % TV_BS, k = idx / TV_BS;
         Ds[r][k] = (r < rows && k < rows) ? (T)A[(size_t)(rb + r) + (size_t)(rb + k) * lda] : T(0);
     }
     constexpr bool UNIT = (FORWARD != TRANS);  // L and L^T have a unit diagonal
     T rdiag = T(1);
-    if (!UNIT && wave == 0 && lane < rows) rdiag = T(1) / (T)A[(size_t)(rb + lane) * (lda + 1)];
+    T rhs[NR];
+#pragma unroll
+    for (int j = 0; j < NR; j++) rhs[j] = T(0);
+    if (wave == 0 && lane < rows) {
+        if (!UNIT) rdiag = T(1) / (T)A[(size_t)(rb + lane) * (lda + 1)];
+#pragma unroll
+        for (int j = 0; j < NR; j++)
+            if (j < nrhs) rhs[j] = X[(size_t)(rb + lane) + (size_t)j * ldx];
+    }
 
     T part[NR];
 #pragma unroll
     for (int j = 0; j < NR; j++) part[j] = T(0);
 
+    // wave w covers k in [16w, 16w+16) of the 64 x 64 block A[rb.., cb..]
+    // (TRANS: of the block A[cb.., rb..], read down its columns); zero
+    // outside the block
+    TA a[TV_PART];
+    auto load_block = [&](int s) {
+        const int c = FORWARD ? s : nblk - 1 - s;
+        const int cb = c * TV_BS, crows = min(TV_BS, n - cb);
+        const TA *Ar = TRANS ? A + (size_t)cb + (size_t)(rb + lane) * lda
+                             : A + (size_t)(rb + lane) + (size_t)cb * lda;
+        const size_t kstride = TRANS ? 1 : (size_t)lda;
+#pragma unroll
+        for (int kk = 0; kk < TV_PART; kk++) {
+            const int k = wave * TV_PART + kk;
+            a[kk] = (lane < rows && k < crows) ? Ar[(size_t)k * kstride] : TA(0);
+        }
+    };
+    if (t > 0) load_block(0);
+
     // Off-diagonal contributions, in dependency order.
     for (int s = 0; s < t; s++) {
         const int c = FORWARD ? s : nblk - 1 - s;
-        const int cb = c * TV_BS, crows = min(TV_BS, n - cb);
-        if (tid == 0) {
+        const int crows = min(TV_BS, n - c * TV_BS);
+        if (wave == 0) {
+            uint32_t got[NR][WPV];
             int ab = 0;
             const uint64_t t0 = wall_clock64();
-            while (__hip_atomic_load(&flags[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+            for (;;) {
+                bool ok = true;
+                if (lane < crows) {
+#pragma unroll
+                    for (int j = 0; j < NR; j++) {
+                        if (j < nrhs) {
+#pragma unroll
+                            for (int w = 0; w < WPV; w++) {
+                                const uint64_t u = __hip_atomic_load(
+                                    mbox(c, j, w, lane), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                                got[j][w] = (uint32_t)u;
+                                ok = ok && (uint32_t)(u >> 32) == epoch;
+                            }
+                        }
+                    }
+                }
+                if (__all(ok)) break;
                 if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
                     ab = 1;
                     break;
@@ -127,35 +211,28 @@ __global__ __launch_bounds__(256) void trsv_persistent_kernel(
                     ab = 1;
                     break;
                 }
-                __builtin_amdgcn_s_sleep(1);
+                // workgroups far behind the chain poll less often
+                if (t - s > 4) __builtin_amdgcn_s_sleep(16);
+                else __builtin_amdgcn_s_sleep(1);
             }
-            // X of block c is read with coherent loads only: workgroup scope
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-            s_abort = ab;
+#pragma unroll
+            for (int j = 0; j < NR; j++)
+                Xs[lane][j] = (lane < crows && j < nrhs) ? tv_join<T>(got[j][0], got[j][WPV - 1])
+                                                         : T(0);
+            if (lane == 0) s_abort = ab;
         }
         __syncthreads();
         if (s_abort) return;  // the whole workgroup leaves together
-        for (int idx = tid; idx < TV_BS * NR; idx += 256) {
-            const int r = idx % TV_BS, j = idx / TV_BS;
-            Xs[r][j] = (r < crows && j < nrhs) ? coherent_load(&X[(size_t)(cb + r) + (size_t)j * ldx]) : T(0);
-        }
-        __syncthreads();
-        // wave w covers k in [16w, 16w+16) of the 64 x 64 block A[rb.., cb..]
-        // (TRANS: of the block A[cb.., rb..], read down its columns)
         if (lane < rows) {
-            const TA *Ar = TRANS ? A + (size_t)cb + (size_t)(rb + lane) * lda
-                                 : A + (size_t)(rb + lane) + (size_t)cb * lda;
-            const size_t kstride = TRANS ? 1 : (size_t)lda;
-#pragma unroll 4
-            for (int kk = 0; kk < 16; kk++) {
-                const int k = wave * 16 + kk;
-                if (k < crows) {
-                    const T a = (T)Ar[(size_t)k * kstride];
 #pragma unroll
-                    for (int j = 0; j < NR; j++) part[j] = fma(a, Xs[k][j], part[j]);
-                }
+            for (int kk = 0; kk < TV_PART; kk++) {
+                const T av = (T)a[kk];
+                const int k = wave * TV_PART + kk;
+#pragma unroll
+                for (int j = 0; j < NR; j++) part[j] = fma(av, Xs[k][j], part[j]);
             }
         }
+        if (s + 1 < t) load_block(s + 1);
         __syncthreads();  // Xs is reused by the next block
     }
 #pragma unroll
@@ -166,9 +243,8 @@ __global__ __launch_bounds__(256) void trsv_persistent_kernel(
         T x[NR];
 #pragma unroll
         for (int j = 0; j < NR; j++) {
-            const T rhs = (lane < rows && j < nrhs) ? X[(size_t)(rb + lane) + (size_t)j * ldx] : T(0);
-            const T s = ((Part[0][lane][j] + Part[1][lane][j]) + Part[2][lane][j]) + Part[3][lane][j];
-            x[j] = rhs - s;
+            const T sum = ((Part[0][lane][j] + Part[1][lane][j]) + Part[2][lane][j]) + Part[3][lane][j];
+            x[j] = rhs[j] - sum;
         }
         // Column i of the triangle applied to x: element (lane, i) of the
         // diagonal block, or (i, lane) for the transposed solves.
@@ -194,17 +270,18 @@ __global__ __launch_bounds__(256) void trsv_persistent_kernel(
             }
         }
         if (lane < rows) {
+            const uint64_t tag = (uint64_t)epoch << 32;
 #pragma unroll
-            for (int j = 0; j < NR; j++)
-                if (j < nrhs) coherent_store(&X[(size_t)(rb + lane) + (size_t)j * ldx], x[j]);
+            for (int j = 0; j < NR; j++) {
+                if (j < nrhs) {
+#pragma unroll
+                    for (int w = 0; w < WPV; w++)
+                        __hip_atomic_store(mbox(b, j, w, lane), tag | tv_half(x[j], w),
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    X[(size_t)(rb + lane) + (size_t)j * ldx] = x[j];
+                }
+            }
         }
-    }
-    drain_memory();  // X[b]'s coherent stores complete before the flag rises
-    __syncthreads();
-    if (tid == 0) {
-        // X[b] was written with coherent stores; order them before the flag
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __hip_atomic_store(&flags[b], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -229,13 +306,14 @@ void trsv_persistent(TrsvOp op, int n, int nrhs, const TA *A, int lda, T *X, int
     if (n <= 0 || nrhs <= 0) return;
     if (nrhs > TV_NR) throw std::logic_error("trsv_persistent: at most 16 right-hand sides per call");
     const int nblk = ceil_div(n, TV_BS);
-    if (nblk > tw.max_blocks) throw std::logic_error("trsv_persistent: flag array too small");
+    if ((size_t)nblk * TV_BS * TV_NR * 2 > tw.mail_words)
+        throw std::logic_error("trsv_persistent: mailbox too small for this order");
     fill_int(tw.ticket, 1, 0, s);
     const int epoch = ++tw.epoch;
 #define SLA_TRSV(FW, TR, NRV)                                                                  \
     hipLaunchKernelGGL((trsv_persistent_kernel<TA, T, FW, TR, NRV>), dim3(nblk), dim3(256), 0, \
-                       s, n, nrhs, A, lda, X, ldx, tw.ticket, tw.flags, epoch, tw.abort_flag, \
-                       tw.spin_limit)
+                       s, n, nrhs, A, lda, X, ldx, tw.ticket, tw.mail, (unsigned)epoch,        \
+                       tw.abort_flag, tw.spin_limit)
 #define SLA_TRSV_NR(FW, TR)                           \
     do {                                              \
         if (nrhs <= 1) SLA_TRSV(FW, TR, 1);           \
