@@ -32,6 +32,11 @@
 
 #include <climits>
 
+// wavefront-wide maximum of the ROCm device library (declared by the HIP
+// headers only with their optional 64-bit warp-sync types)
+extern "C" __device__ __attribute__((const)) unsigned long long __ockl_wfred_max_u64(
+    unsigned long long);
+
 namespace sla {
 
 namespace {
@@ -42,28 +47,42 @@ constexpr int TW = 32;           // leaf width
 constexpr int TMR = 2;           // candidate rows per thread in the tournament
 constexpr int TGMAX = TPT * TMR / TW;  // workgroups per launch at most (32)
 
+// Pivot key of a row: |v| as float bits (monotone for |v| >= 0; a NaN counts
+// as zero) in the upper half, INT_MAX - row in the lower, so the larger key
+// is the larger magnitude and, on a tie, the lower row; 0 = no row.
+__device__ __forceinline__ uint64_t piv_key(float v, int row)
+{
+    const float a = fabsf(v);
+    const uint32_t ab = (a == a) ? __float_as_uint(a) : 0u;
+    return row == INT_MAX ? 0ull : (((uint64_t)ab << 32) | (uint32_t)(INT_MAX - row));
+}
+
 // Partial-pivoting elimination over the rows a whole workgroup holds (NR
 // per thread; key = row identity and tie-break, INT_MAX = no row; pid =
 // payload id).  Selects up to wa pivots in order: out_key[j], out_pid[j]
-// (written by thread 0).  Per column: a wave-shuffle argmax, one workgroup
-// barrier to combine the 8 wave winners (every thread folds them in the same
-// order), the winner's row through LDS, a second barrier, and the rank-1
-// update.
+// (written by thread 0); with s_u, row j of U (columns j..) goes to s_u[j].
+// Per column: one wave max over the packed pivot keys, the winning lane of
+// every wave posts its row to LDS, one workgroup barrier, every thread folds
+// the 8 posts in the same order and applies the rank-1 update with the
+// winner's row.  The posts are double-buffered by column parity: a buffer is
+// rewritten two columns later, after a barrier every reader has passed.
 //
 // The columns rotate through the registers: at step j, register column 0
 // holds matrix column j, and the update writes column c+1's new value into
 // register column c, so every register index is a compile-time constant while
 // the step loop stays rolled.
 struct GeppLds {
-    double wv[TNW];
-    int wk[TNW];
-    int pid;
+    uint64_t wkey[2][TNW];
+    float wrow[2][TNW][TW];
+    int wpid[2][TNW];
 };
 
 template <typename T, int NR, bool FULL>
 __device__ __forceinline__ void wg_gepp(T (&v)[NR][TW], const int (&key)[NR], const int (&pid)[NR],
-                                        int wa, int *out_key, int *out_pid, GeppLds &sl, T *s_prow)
+                                        int wa, int *out_key, int *out_pid, GeppLds &sl,
+                                        T (*s_u)[TW + 1])
 {
+    static_assert(sizeof(T) == 4, "the packed pivot key holds a float magnitude");
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     bool act[NR];
 #pragma unroll
@@ -71,74 +90,62 @@ __device__ __forceinline__ void wg_gepp(T (&v)[NR][TW], const int (&key)[NR], co
     const int steps = FULL ? TW : wa;
 #pragma unroll 1
     for (int j = 0; j < steps; j++) {
-        double bv = -1.0;
-        int bk = INT_MAX, bq = 0;
+        const int pb = j & 1;
+        uint64_t bk = 0;
+        int bq = 0;
 #pragma unroll
         for (int q = 0; q < NR; q++) {
-            if (act[q]) {
-                const double x = fabs((double)v[q][0]);
-                if (pivot_better(x, key[q], bv, bk)) {
-                    bv = x;
-                    bk = key[q];
-                    bq = q;
-                }
+            const uint64_t k = act[q] ? piv_key(v[q][0], key[q]) : 0ull;
+            if (k > bk) {
+                bk = k;
+                bq = q;
             }
         }
-        double wv = bv;
-        int wk = bk;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            const double ov = __shfl_xor(wv, off);
-            const int ok = __shfl_xor(wk, off);
-            if (pivot_better(ov, ok, wv, wk)) {
-                wv = ov;
-                wk = ok;
-            }
-        }
-        if (lane == 0) {
-            sl.wv[wave] = wv;
-            sl.wk[wave] = wk;
-        }
-        __syncthreads();
-        double fv = sl.wv[0];
-        int fk = sl.wk[0];
-#pragma unroll
-        for (int w = 1; w < TNW; w++)
-            if (pivot_better(sl.wv[w], sl.wk[w], fv, fk)) {
-                fv = sl.wv[w];
-                fk = sl.wk[w];
-            }
-        if (fk == INT_MAX) {  // uniform: no rows left
-            if (tid == 0) {
-                out_key[j] = INT_MAX;
-                out_pid[j] = -1;
-            }
-            __syncthreads();
-            continue;
-        }
-        const bool mine = (bk == fk);  // one thread holds the winning row
-        if (mine) {
+        const uint64_t wmax = __ockl_wfred_max_u64(bk);
+        if (wmax != 0 && bk == wmax) {  // the one lane holding the wave's winner
 #pragma unroll
             for (int c = 0; c < TW; c++) {
                 T sel = v[0][c];
 #pragma unroll
                 for (int q = 1; q < NR; q++) sel = (bq == q) ? v[q][c] : sel;
-                s_prow[c] = sel;
+                sl.wrow[pb][wave][c] = sel;
             }
             int sp = pid[0];
 #pragma unroll
             for (int q = 1; q < NR; q++) sp = (bq == q) ? pid[q] : sp;
-            sl.pid = sp;
+            sl.wpid[pb][wave] = sp;
+        }
+        if (lane == 0) sl.wkey[pb][wave] = wmax;
+        __syncthreads();
+        uint64_t fk = sl.wkey[pb][0];
+        int fw = 0;
+#pragma unroll
+        for (int w = 1; w < TNW; w++) {
+            const uint64_t k = sl.wkey[pb][w];
+            if (k > fk) {
+                fk = k;
+                fw = w;
+            }
+        }
+        if (fk == 0) {  // uniform: no rows left
+            if (tid == 0) {
+                out_key[j] = INT_MAX;
+                out_pid[j] = -1;
+            }
+            continue;
+        }
+        if (bk == fk) {  // the owner of the pivot row retires it
 #pragma unroll
             for (int q = 0; q < NR; q++)
                 if (bq == q) act[q] = false;
         }
-        __syncthreads();
+        const float *prow = sl.wrow[pb][fw];
         if (tid == 0) {
-            out_key[j] = fk;
-            out_pid[j] = sl.pid;
+            out_key[j] = INT_MAX - (int)(uint32_t)fk;
+            out_pid[j] = sl.wpid[pb][fw];
         }
-        const T pj = s_prow[0];
+        if (s_u != nullptr && tid < TW - j) s_u[j][j + tid] = prow[tid];
+        const T pj = prow[0];
         const bool nz = pj != T(0);
         const T rcp = nz ? T(1) / pj : T(0);
 #pragma unroll
@@ -147,12 +154,12 @@ __device__ __forceinline__ void wg_gepp(T (&v)[NR][TW], const int (&key)[NR], co
                 T l = v[q][0];
                 if (nz) l = l * rcp;
 #pragma unroll
-                for (int c = 1; c < TW; c++) v[q][c - 1] = fma(-l, s_prow[c], v[q][c]);
+                for (int c = 1; c < TW; c++) v[q][c - 1] = fma(-l, prow[c], v[q][c]);
                 v[q][TW - 1] = T(0);
             }
         }
     }
-    __syncthreads();  // out_* complete before anyone reads them
+    __syncthreads();  // out_* and s_u complete before anyone reads them
 }
 
 template <bool FULL>
@@ -176,7 +183,6 @@ __global__ __launch_bounds__(TPT) void panel_tslu_kernel(
     // narrow leaf repeat its last column (never written back, never pivoted on)
 
     __shared__ GeppLds s_gl;
-    __shared__ T s_prow[TW];
     __shared__ int s_k2[TW], s_p2[TW];
     __shared__ T s_lu[TW][TW + 1];
     __shared__ int s_dst[2 * TW], s_src[2 * TW], s_np;
@@ -198,7 +204,7 @@ __global__ __launch_bounds__(TPT) void panel_tslu_kernel(
 #pragma unroll
             for (int c = 0; c < TW; c++) v[q][c] = A[rr + (size_t)colx<FULL>(c, wa) * lda];
         }
-        wg_gepp<T, RPT, FULL>(v, key, key, wa, s_k2, s_p2, s_gl, s_prow);
+        wg_gepp<T, RPT, FULL>(v, key, key, wa, s_k2, s_p2, s_gl, nullptr);
     }
     // publish the workgroup's candidates (ids, original values); workgroup 0
     // also the leaf's top rows
@@ -218,9 +224,9 @@ __global__ __launch_bounds__(TPT) void panel_tslu_kernel(
     if (!grid_sync(ctr, target, abort_flag, spin_limit)) return;
 
     // 3. the tournament, identical in every workgroup: one elimination over
-    //    all G*TW candidates (original values) fixes the pivot rows and their
-    //    order; wave 0 then factors those rows (original values, in that
-    //    order, no further pivoting) into L11\U11
+    //    all G*TW candidates (original values) fixes the pivot rows, their
+    //    order and U11 (the pivot rows as the elimination leaves them); L11
+    //    is then solved from the pivot rows' original values
     if (tid < TW) s_k2[tid] = INT_MAX;
     __syncthreads();
     {
@@ -238,36 +244,25 @@ __global__ __launch_bounds__(TPT) void panel_tslu_kernel(
 #pragma unroll
             for (int c = 0; c < TW; c++) v[q][c] = coherent_load(&slots.cand_v[(size_t)sv * TW + c]);
         }
-        wg_gepp<T, TMR, FULL>(v, key, pid, wa, s_k2, s_p2, s_gl, s_prow);
+        wg_gepp<T, TMR, FULL>(v, key, pid, wa, s_k2, s_p2, s_gl, s_lu);
     }
-    if (wave == 0) {
-        // lane i < wa holds pivot row i; the columns rotate as in wg_gepp;
-        // each lane writes its multipliers, and the pivot row of the step its
-        // U part, straight into s_lu
+    if (wave == 0 && lane < wa) {
+        // L11: lane r solves its pivot row's original values against U11
+        // (x U11^{-1}; the first r entries are row r's multipliers), the
+        // same computation every other row of the leaf gets below
         T x[TW];
-        const int sid = (lane < wa) ? s_p2[lane] : 0;
+        const int sid = s_p2[lane];
 #pragma unroll
-        for (int c = 0; c < TW; c++)
-            x[c] = (lane < wa) ? coherent_load(&slots.cand_v[(size_t)sid * TW + c]) : T(0);
-        const int steps = FULL ? TW : wa;
-#pragma unroll 1
-        for (int k = 0; k < steps; k++) {
-            T prow[TW];
+        for (int c = 0; c < TW; c++) x[c] = coherent_load(&slots.cand_v[(size_t)sid * TW + c]);
 #pragma unroll
-            for (int c = 0; c < TW; c++) prow[c] = lane_read(x[c], k);
-            if (lane == k) {
+        for (int c = 0; c < TW; c++) {
+            if (c < lane) {
+                T acc = x[c];
 #pragma unroll
-                for (int c = 0; c < TW; c++)
-                    if (c < TW - k) s_lu[k][k + c] = prow[c];
-            }
-            if (lane > k && lane < wa) {
-                const T pj = prow[0];
-                T l = x[0];
-                if (pj != T(0)) l = l * (T(1) / pj);
-                s_lu[lane][k] = l;
-#pragma unroll
-                for (int c = 1; c < TW; c++) x[c - 1] = fma(-l, prow[c], x[c]);
-                x[TW - 1] = T(0);
+                for (int k = 0; k < c; k++) acc = fma(-x[k], s_lu[k][c], acc);
+                const T u = s_lu[c][c];
+                x[c] = (u != T(0)) ? acc / u : acc;
+                s_lu[lane][c] = x[c];
             }
         }
     }
