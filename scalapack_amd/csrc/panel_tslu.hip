@@ -127,36 +127,33 @@ __device__ __forceinline__ void wg_gepp(T (&v)[NR][TW], const int (&key)[NR], co
                 fw = w;
             }
         }
-        if (fk == 0) {  // uniform: no rows left
-            if (tid == 0) {
-                out_key[j] = INT_MAX;
-                out_pid[j] = -1;
-            }
-            continue;
-        }
-        if (bk == fk) {  // the owner of the pivot row retires it
+        // fk == 0 (uniform): no rows left -- no pivot, and every multiplier
+        // below is 0 because no row is active
+        const bool any = fk != 0;
+        if (any && bk == fk) {  // the owner of the pivot row retires it
 #pragma unroll
             for (int q = 0; q < NR; q++)
                 if (bq == q) act[q] = false;
         }
         const float *prow = sl.wrow[pb][fw];
         if (tid == 0) {
-            out_key[j] = INT_MAX - (int)(uint32_t)fk;
-            out_pid[j] = sl.wpid[pb][fw];
+            out_key[j] = any ? INT_MAX - (int)(uint32_t)fk : INT_MAX;
+            out_pid[j] = any ? sl.wpid[pb][fw] : -1;
         }
-        if (s_u != nullptr && tid < TW - j) s_u[j][j + tid] = prow[tid];
+        if (any && s_u != nullptr && tid < TW - j) s_u[j][j + tid] = prow[tid];
         const T pj = prow[0];
         const bool nz = pj != T(0);
         const T rcp = nz ? T(1) / pj : T(0);
+        // every row rotates (a retired row with multiplier 0: its values are
+        // never read again), so the registers need no copies between steps
 #pragma unroll
         for (int q = 0; q < NR; q++) {
-            if (act[q]) {
-                T l = v[q][0];
-                if (nz) l = l * rcp;
+            T l = v[q][0];
+            if (nz) l = l * rcp;
+            l = act[q] ? l : T(0);
 #pragma unroll
-                for (int c = 1; c < TW; c++) v[q][c - 1] = fma(-l, prow[c], v[q][c]);
-                v[q][TW - 1] = T(0);
-            }
+            for (int c = 1; c < TW; c++) v[q][c - 1] = fma(-l, prow[c], v[q][c]);
+            v[q][TW - 1] = T(0);
         }
     }
     __syncthreads();  // out_* and s_u complete before anyone reads them
