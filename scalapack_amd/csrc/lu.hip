@@ -23,6 +23,7 @@
 #include <climits>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 namespace sla {
@@ -277,8 +278,13 @@ struct Lu {
     void trsm(int m, int n, const T *l, T *b, hipStream_t s)
     {
         if (m <= 0 || n <= 0) return;
-        // <= 32 rows, or a small block inside the panel: one strip launch
-        if (m <= 32 || (m <= 256 && n <= 256)) {
+        // <= 32 rows, or a small block inside the panel: one strip launch;
+        // for FP32 storage also up to 1024 columns (the next block column's
+        // U12 then costs 3 launches instead of 31 -- it is on the critical
+        // path; the FP64 path keeps the rounding order its pivots were
+        // checked with)
+        const int wide = std::is_same<T, float>::value ? 1024 : 256;
+        if (m <= 32 || (m <= 256 && n <= wide)) {
             h.timing_begin(CAT_TRSM, s);
             trsm_llnu<T>(m, n, l, lda, b, lda, s);
             h.timing_end((double)m * m * n, s);
