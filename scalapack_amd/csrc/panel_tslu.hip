@@ -46,6 +46,9 @@ constexpr int TNW = TPT / 64;    // waves per workgroup
 constexpr int TW = 32;           // leaf width
 constexpr int TMR = 2;           // candidate rows per thread in the tournament
 constexpr int TGMAX = TPT * TMR / TW;  // workgroups per launch at most (32)
+// candidate values are stored column by column (value c of candidate sid at
+// c * TCAND + sid), so the tournament's loads are contiguous across lanes
+constexpr int TCAND = TGMAX * TW;
 
 // Pivot key of a row: |v| as float bits (monotone for |v| >= 0; a NaN counts
 // as zero) in the upper half, INT_MAX - row in the lower, so the larger key
@@ -206,10 +209,10 @@ __global__ __launch_bounds__(TPT) void panel_tslu_kernel(
     // publish the workgroup's candidates (ids, original values); workgroup 0
     // also the leaf's top rows
     for (int idx = tid; idx < TW * TW; idx += TPT) {
-        const int j = idx / TW, c = idx % TW;
+        const int c = idx / TW, j = idx % TW;  // consecutive threads: consecutive candidates
         const int r = (j < wa) ? s_k2[j] : INT_MAX;
         const T x = r != INT_MAX ? A[r + (size_t)colx<FULL>(c, wa) * lda] : T(0);
-        coherent_store(&slots.cand_v[((size_t)g * TW + j) * TW + c], x);
+        coherent_store(&slots.cand_v[(size_t)c * TCAND + g * TW + j], x);
         if (c == 0) coherent_store(&slots.cand_r[g * TW + j], r);
         if (g == 0) {
             const T y = (j < wa) ? A[j + (size_t)colx<FULL>(c, wa) * lda] : T(0);
@@ -239,7 +242,7 @@ __global__ __launch_bounds__(TPT) void panel_tslu_kernel(
             pid[q] = ok ? sid : -1;
             const int sv = (r != INT_MAX) ? sid : 0;
 #pragma unroll
-            for (int c = 0; c < TW; c++) v[q][c] = coherent_load(&slots.cand_v[(size_t)sv * TW + c]);
+            for (int c = 0; c < TW; c++) v[q][c] = coherent_load(&slots.cand_v[(size_t)c * TCAND + sv]);
         }
         wg_gepp<T, TMR, FULL>(v, key, pid, wa, s_k2, s_p2, s_gl, s_lu);
     }
@@ -250,7 +253,7 @@ __global__ __launch_bounds__(TPT) void panel_tslu_kernel(
         T x[TW];
         const int sid = s_p2[lane];
 #pragma unroll
-        for (int c = 0; c < TW; c++) x[c] = coherent_load(&slots.cand_v[(size_t)sid * TW + c]);
+        for (int c = 0; c < TW; c++) x[c] = coherent_load(&slots.cand_v[(size_t)c * TCAND + sid]);
 #pragma unroll
         for (int c = 0; c < TW; c++) {
             if (c < lane) {

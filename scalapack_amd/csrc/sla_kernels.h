@@ -87,7 +87,7 @@ unsigned panel_leaf(T *A, int lda, int m, int wa, int lcols, int rcols, int row0
 template <typename T>
 struct TsluSlots {
     int *cand_r;  // [32 workgroups][32] candidate rows (leaf-relative)
-    T *cand_v;    // [32][32][32] their original values
+    T *cand_v;    // [32][32 * 32] their original values, column by column
     T *top_v;     // [32][32] the leaf's top rows, original values
 };
 constexpr size_t tslu_slot_bytes(size_t esz) { return 32 * 32 * 4 + (32 * 32 * 32 + 32 * 32) * esz + 256; }
