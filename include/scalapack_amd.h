@@ -133,10 +133,12 @@ int sla_destroy(sla_handle_t h);
  * own: inputs produced on other streams must be complete before a call.
  * Calls return after the stream has drained. */
 int sla_set_stream(sla_handle_t h, void *stream);
-/* Tuning knobs: "nb" (block column width), "leaf_w" (panel leaf width,
- * <= 32), "panel_wgs" (workgroups per panel leaf), "profile" (0/1); for the
+/* Tuning knobs: "nb" (block column width, 1..512), "leaf_w" (panel leaf width,
+ * <= 32), "panel_wgs" (workgroups per panel leaf, 1..128), "profile" (0/1); for the
  * mixed-precision solve "ir_max_iter" (3..1000, default 30), "ir_gmres_on_stall"
- * (0/1, default 1), "gmres_m" (GMRES steps per correction, default 50),
+ * (0/1, default 1), "ir_accept_stall" (0/1, default 1: a refinement that
+ * stalls with scaled residual < 16 returns its X; 0: only DSGESV's
+ * convergence test ends refinement, a stall falls back to FP64), "gmres_m" (GMRES steps per correction, default 50),
  * "gmres_tol_exp" (e: relative reduction 10^-e per correction, default 6),
  * "pivoting" (FP32-storage factorizations: 0 partial pivoting, 1 tournament
  * pivoting of the panel leaves, default 1; the FP64 path always pivots as
@@ -236,7 +238,14 @@ typedef struct {
                         the FP64 fallback (A then holds the factors) */
     double sresid;   /* bwd_err / (N eps), eps = 2^-53: the testers' scaled residual */
     double history[16]; /* bwd_err after the first solve and each refinement step */
+    int stop_rule;   /* what ended the refinement: SLA_STOP_CONVERGED (DSGESV's test),
+                        SLA_STOP_STALL_ACCEPTED (a stall, PDGERFS's rule, with scaled
+                        residual < 16; only while "ir_accept_stall" is 1) or
+                        SLA_STOP_FALLBACK (X from the FP64 factorization) */
 } sla_solve_info_t;
+#define SLA_STOP_CONVERGED 0
+#define SLA_STOP_STALL_ACCEPTED 1
+#define SLA_STOP_FALLBACK 2
 
 /* X = A^{-1} B (n x n FP64 dA, nrhs <= 16, device pointers) by a
  * factorization in precision `prec` and refinement `refine` to FP64 backward

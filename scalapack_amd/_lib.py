@@ -27,7 +27,7 @@ class SolveInfo(ctypes.Structure):
                 ("iter", ctypes.c_int), ("outer", ctypes.c_int), ("inner", ctypes.c_int),
                 ("fell_back", ctypes.c_int), ("anorm", ctypes.c_double),
                 ("bwd_err", ctypes.c_double), ("sresid", ctypes.c_double),
-                ("history", ctypes.c_double * 16)]
+                ("history", ctypes.c_double * 16), ("stop_rule", ctypes.c_int)]
 
     def as_dict(self) -> dict:
         d = {k: getattr(self, k) for k, _ in self._fields_}

@@ -51,21 +51,26 @@ def _run(cmd: list[str]) -> None:
         raise RuntimeError(f"build step failed: {' '.join(cmd)}")
 
 
-def _hip_object(src: str) -> Path:
+def _hip_object(src: str, incremental: bool = False) -> Path:
     obj = BUILD / (Path(src).stem + ".o")
+    if incremental and obj.exists():
+        deps = [CSRC / src, *CSRC.glob("*.h"), *INCLUDE.glob("*.h")]
+        if all(d.stat().st_mtime <= obj.stat().st_mtime for d in deps):
+            return obj
     _run([HIPCC, *HIP_FLAGS, "-c", str(CSRC / src), "-o", str(obj)])
     return obj
 
 
-def build(jobs: int | None = None) -> None:
-    """Compile every native component; raises on any compiler error."""
+def build(jobs: int | None = None, incremental: bool = False) -> None:
+    """Compile every native component; raises on any compiler error.
+    incremental: keep object files newer than their source and every header."""
     BUILD.mkdir(exist_ok=True)
     shim = PKG / "libsla_blacs1x1.so"
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", f"-I{INCLUDE}",
           str(CSRC / "blacs1x1.cpp"), "-o", str(shim)])
     jobs = jobs or min(8, os.cpu_count() or 1)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_hip_object, HIP_SOURCES))
+        objs = list(ex.map(lambda src: _hip_object(src, incremental), HIP_SOURCES))
     lib = PKG / "libscalapack_amd.so"
     tmp = lib.with_suffix(".so.tmp")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp),

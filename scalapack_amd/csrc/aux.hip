@@ -756,6 +756,7 @@ double dgecon_device(Handle &h, char norm, int n, const double *dLU, int lda, do
     AuxWork a(h, n, 0);
     const int kase1 = onenrm ? 1 : 2;
     fill_int(h.abort_flag, 1, 0, h.stream);
+    fill_int(h.bad_pivot, 1, 0, h.stream);  // no permutation here: a stale flag must not fail
     TrsvWork tw = h.trsv_work(n);
     auto apply = [&](int kase) {
         if (kase == kase1) {  // x <- U^{-1} L^{-1} x

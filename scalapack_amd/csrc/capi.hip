@@ -982,20 +982,23 @@ int sla_set_param(sla_handle_t h, const char *name, int value)
                                   "ir_max_iter", "gmres_m", "ir_gmres_on_stall", "gmres_tol_exp",
                                   "pivoting", "lookahead", "update_cus", "lacon_lapack",
                                   "batched_w", "batched_wgs_per_cu", "batched_graph",
-                                  "update_lds_kb"};
+                                  "update_lds_kb", "ir_accept_stall"};
     if (std::find_if(std::begin(known), std::end(known), [&](const char *x) { return k == x; }) ==
         std::end(known)) {
         g_last_error = "sla_set_param: unknown parameter " + k;
         return SLA_ERR_ARG;
     }
     SLA_NEED_HANDLE(h);
-    if (k == "nb" && value >= 1 && value <= 1024) H->nb = value;
+    // ranges are what the kernels accept: the factorization's block columns
+    // are at most 512 wide, a panel leaf launches at most 128 workgroups
+    if (k == "nb" && value >= 1 && value <= 512) H->nb = value;
     else if (k == "leaf_w" && value >= 1 && value <= 32) H->leaf_w = value;
-    else if (k == "panel_wgs" && value >= 1 && value <= 256) H->panel_wgs = value;
+    else if (k == "panel_wgs" && value >= 1 && value <= 128) H->panel_wgs = value;
     else if (k == "profile") H->profile = value != 0;
     else if (k == "ir_max_iter" && value >= 3 && value <= 1000) H->ir_max_iter = value;
     else if (k == "gmres_m" && value >= 1 && value <= 1000) H->gmres_m = value;
     else if (k == "ir_gmres_on_stall") H->ir_gmres_on_stall = value != 0;
+    else if (k == "ir_accept_stall") H->ir_accept_stall = value != 0;
     else if (k == "pivoting" && (value == 0 || value == 1)) H->pivoting = value;
     else if (k == "lookahead" && (value == 0 || value == 1)) H->lookahead = value;
     else if (k == "update_cus" && value >= 0 && value <= H->num_cus) H->update_cus = value;
@@ -1024,6 +1027,7 @@ int sla_get_param(sla_handle_t h, const char *name, int *value)
     else if (k == "ir_max_iter") *value = H->ir_max_iter;
     else if (k == "gmres_m") *value = H->gmres_m;
     else if (k == "ir_gmres_on_stall") *value = H->ir_gmres_on_stall;
+    else if (k == "ir_accept_stall") *value = H->ir_accept_stall;
     else if (k == "pivoting") *value = H->pivoting;
     else if (k == "lookahead") *value = H->lookahead;
     else if (k == "update_cus") *value = H->update_cus;
@@ -1435,6 +1439,7 @@ int sla_last_solve_info(sla_handle_t h, sla_solve_info_t *out)
     out->bwd_err = mi.bwd_err;
     out->sresid = mi.sresid;
     for (int i = 0; i < 16; i++) out->history[i] = mi.history[i];
+    out->stop_rule = mi.stop_rule;
     return SLA_OK;
 }
 

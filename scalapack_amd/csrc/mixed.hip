@@ -19,7 +19,8 @@
 // normwise backward error counts as a stall (the stopping rule of PDGERFS,
 // SRC/pdgerfs.f:556-561); a stalled result is accepted when its scaled
 // residual ||r|| / (||A|| ||x|| N eps) is below the testers' threshold 16
-// (TESTING/traditional/LIN/pdlaschk.f:296).
+// (TESTING/traditional/LIN/pdlaschk.f:296), unless the handle's
+// ir_accept_stall is 0.  MixedInfo::stop_rule records which rule ended it.
 #include "sla_common.h"
 #include "sla_kernels.h"
 #include "sla_handle.h"
@@ -511,11 +512,14 @@ void sgetrs_device(Handle &h, int n, int nrhs, const float *dA, int lda, const i
                    float *dB, int ldb)
 {
     if (n <= 0 || nrhs <= 0) return;
+    // flags of this call only: an earlier call's out-of-range IPIV or timeout
+    // must neither fail this one nor survive it
+    fill_int(h.bad_pivot, 1, 0, h.stream);
+    fill_int(h.abort_flag, 1, 0, h.stream);
     if (!(h.perm_key_a == dA && h.perm_key_ipiv == dipiv && h.perm_m == n))
         perm_from_ipiv(h, n, dipiv);
     float *W = static_cast<float *>(h.buffer(BUF_VEC, (size_t)n * 16 * 4 + 256));
     TrsvWork tw = h.trsv_work(n);
-    fill_int(h.abort_flag, 1, 0, h.stream);
     for (int j = 0; j < nrhs; j += 16) {
         const int nr = std::min(16, nrhs - j);
         float *Bj = dB + (size_t)j * ldb;
@@ -526,10 +530,9 @@ void sgetrs_device(Handle &h, int n, int nrhs, const float *dA, int lda, const i
                                  hipMemcpyDeviceToDevice, h.stream));
     }
     h.trsv_epoch = tw.epoch;
-    int flags[2];
-    SLA_HIP(hipMemcpyAsync(flags, h.ctl, sizeof flags, hipMemcpyDeviceToHost, h.stream));
-    SLA_HIP(hipStreamSynchronize(h.stream));
-    if (flags[1]) throw std::runtime_error("sgetrs: a dependency wait timed out");
+    // an IPIV row outside 1..N leaves the permutation incomplete: an error,
+    // never a wrong X (the same check as dgetrs)
+    check_solve_flags(h, "sgetrs");
 }
 
 int dxgesv_device(Handle &h, int prec, int refine, int n, int nrhs, double *dA, int lda,
@@ -546,6 +549,7 @@ int dxgesv_device(Handle &h, int prec, int refine, int n, int nrhs, double *dA, 
         // FP64 factorization in place and solve of X = A^{-1} B; A then holds
         // the factors, so the errors of this X are not measured (NaN)
         info.fell_back = 1;
+        info.stop_rule = STOP_FALLBACK;
         info.iter = iter;
         info.bwd_err = info.sresid = std::nan("");
         SLA_HIP(hipMemcpy2DAsync(dX, (size_t)ldx * 8, dB, (size_t)ldb * 8, (size_t)n * 8, nrhs,
@@ -621,8 +625,9 @@ int dxgesv_device(Handle &h, int prec, int refine, int n, int nrhs, double *dA, 
         info.sresid = worst / (n * EPS64);
         info.outer = it;
         if (it < 16) info.history[it] = worst;
-        if (converged || (stalled && acceptable)) {
+        if (converged || (stalled && acceptable && h.ir_accept_stall)) {
             info.iter = it;
+            info.stop_rule = converged ? STOP_CONVERGED : STOP_STALL_ACCEPTED;
             h.timing_collect();
             return 0;
         }

@@ -24,6 +24,8 @@
 #include "sla_common.h"
 #include "sla_kernels.h"
 
+#include <algorithm>
+
 #include <climits>
 
 namespace sla {
@@ -252,9 +254,11 @@ unsigned panel_leaf(T *A, int lda, int m, int wa, int lcols, int rcols, int row0
     const int wcap = wa <= 8 ? 8 : (wa <= 16 ? 16 : 32);
     int rpt_cap = max_rpt_w / wcap;
     if (rpt_cap > 8) rpt_cap = 8;
+    // the caller's preferred grid, capped by what the leaf's exchange holds
+    const int target = std::min(pw.target_wgs, std::min(LEAF_GMAX, pw.slots.gmax));
     int rpt = 1;
     int G = ceil_div(m, PT * rpt);
-    while (G > pw.target_wgs && rpt < rpt_cap) {
+    while (G > target && rpt < rpt_cap) {
         rpt *= 2;
         G = ceil_div(m, PT * rpt);
     }

@@ -42,6 +42,14 @@ struct MixedInfo {
     double bwd_err = 0.0;  // max_j ||r_j||_inf / (||A||_inf ||x_j||_inf) of the last residual
     double sresid = 0.0;   // the same divided by N eps (the testers' scaled residual)
     double history[16] = {};  // bwd_err after the first solve and each refinement step
+    int stop_rule = 0;        // StopRule: which test ended the refinement
+};
+
+// Why the refinement of a mixed-precision solve stopped.
+enum StopRule {
+    STOP_CONVERGED = 0,       // DSGESV's test: ||r|| <= ||x|| ||A|| sqrt(N) eps
+    STOP_STALL_ACCEPTED = 1,  // a stall (PDGERFS's rule) with scaled residual < 16
+    STOP_FALLBACK = 2         // X came from the FP64 factorization
 };
 
 // Reusable device buffers, grown on demand.
@@ -145,6 +153,8 @@ struct Handle {
     // mixed-precision refinement
     int ir_max_iter = 30;        // refinement steps before the FP64 fallback (DSGESV's ITERMAX)
     int ir_gmres_on_stall = 1;   // classical refinement that stalls continues as GMRES-IR
+    int ir_accept_stall = 1;     // a stalled refinement below scaled residual 16 is a result
+                                 // (0: only DSGESV's convergence test ends it)
     int gmres_m = 50;            // GMRES steps per correction
     double gmres_tol = 1e-6;     // relative residual reduction per correction
 

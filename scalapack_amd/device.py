@@ -16,6 +16,8 @@ STAT_CLASSES = ("panel", "laswp", "trsm", "gemm", "trsv", "other", "gemm_panel")
 # factorization precisions and refinements (SLA_PREC_*, SLA_REFINE_*)
 PRECISIONS = {"fp64": 0, "fp32": 1, "bf16": 2, "fp8": 3}
 REFINEMENTS = {"ir": 0, "gmres": 1}
+# what ended a mixed-precision refinement (SLA_STOP_*)
+STOP_RULES = ("converged", "stall_accepted", "fallback")
 
 
 class SlaError(RuntimeError):
@@ -247,6 +249,7 @@ class Handle:
         d = out.as_dict()
         d["precision"] = {v: k for k, v in PRECISIONS.items()}[d["precision"]]
         d["refinement"] = {v: k for k, v in REFINEMENTS.items()}[d["refinement"]]
+        d["stop_rule"] = STOP_RULES[d["stop_rule"]]
         return d
 
     def sgetrf(self, m: int, n: int, a, lda: int, ipiv, prec: str = "fp32") -> int:

@@ -65,6 +65,17 @@ def lcg_state(seed: int, k: int) -> int:
     return lib().oracle_lcg_state(seed, k)
 
 
+def pdgetf2(a: np.ndarray, nb: int, m: int, n: int, ia=1, ja=1):
+    """PDGETF2 on a(ia:ia+m-1, ja:ja+n-1) (Fortran-ordered, in place), column
+    by column as SRC/pdgetf2.f:206-237; returns (ipiv of those rows, info)."""
+    M, N = a.shape
+    d = desc(M, N, nb, M)
+    ipiv = np.zeros(M + nb, np.int32)
+    info = lib().oracle_pdgetf2(m, n, _p(a), ia, ja, _p(d), _p(ipiv), None)
+    mn = min(m, n)
+    return ipiv[ia - 1: ia - 1 + mn].copy(), info
+
+
 def pdgetrf(a: np.ndarray, nb: int, m=None, n=None, ia=1, ja=1, with_gap=False):
     """Factor a (Fortran-ordered, modified in place); returns (ipiv, info[, gap])."""
     M, N = a.shape

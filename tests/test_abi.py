@@ -208,3 +208,38 @@ def test_pdgesvx_argument_checks(capfd):
     assert svx(liwork=n - 1) == -31
     out = capfd.readouterr().out
     assert "On entry to PDGESVX parameter number   29 had an illegal value" in out
+
+
+_PDGETF2_ABORT = """
+import sys
+sys.path.insert(0, {root!r})
+import numpy as np
+from scalapack_amd import api
+ctx = api.blacs_gridinit(1, 1)
+m, n, nb, ja = {m}, {n}, {nb}, {ja}
+d, _ = api.descinit(m, ja - 1 + n + 1, nb, nb, 0, 0, ctx, m)
+a = np.asfortranarray(np.eye(m, ja - 1 + n + 1))
+ipiv = np.zeros(m + nb, np.int32)
+api.pdgetf2(m, n, a, 1, ja, d, ipiv)
+print("RETURNED")
+"""
+
+
+@pytest.mark.parametrize("n,nb,ja,pos", [(40, 32, 1, 2), (32, 32, 2, 2), (16, 32, 2, 5)])
+def test_pdgetf2_illegal_argument_aborts(n, nb, ja, pos):
+    """PDGETF2 reports an illegal argument through PXERBLA and then calls
+    BLACS_ABORT (SRC/pdgetf2.f:177-193): N + ICOFF > NB is argument 2, a JA
+    off a block boundary argument 5.  The process ends with the abort's
+    error code; the call never returns.  Runs in a child process (no GPU
+    work happens before the abort)."""
+    import subprocess
+    import sys
+    from pathlib import Path
+    root = str(Path(__file__).resolve().parents[1])
+    code = _PDGETF2_ABORT.format(root=root, m=64, n=n, nb=nb, ja=ja)
+    proc = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                          timeout=120)
+    assert proc.returncode == 1, (proc.returncode, proc.stdout, proc.stderr)
+    assert "RETURNED" not in proc.stdout
+    assert f"On entry to PDGETF2 parameter number {pos:4d} had an illegal value" in proc.stdout
+    assert "exiting with error #1" in proc.stderr

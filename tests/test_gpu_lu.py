@@ -181,6 +181,55 @@ def test_rectangular_and_offset_submatrices(oracle, ctx, M, N, NB, IA):
     assert np.array_equal(a[mask], a_before[mask])
 
 
+@pytest.mark.parametrize("M,N,NB,IA,JA", [(300, 64, 64, 1, 1), (1000, 48, 64, 65, 129),
+                                          (257, 32, 32, 33, 1), (64, 64, 64, 1, 65),
+                                          (5000, 17, 32, 1, 33)])
+def test_pdgetf2_entry(oracle, ctx, M, N, NB, IA, JA):
+    """PDGETF2 through its Fortran entry (SRC/pdgetf2.f:206-237): N <= NB,
+    block-aligned IA / JA offsets; IPIV (absolute global rows) bit-exact
+    against the oracle's column-by-column factorization, L\\U to rounding,
+    everything outside sub(A) untouched."""
+    from scalapack_amd import api
+    _note(f"pdgetf2 M={M} N={N} NB={NB} IA={IA} JA={JA}")
+    LM, LN = IA - 1 + M + 3, JA - 1 + N + 2
+    a = oracle.matgen(LM, LN, 23)
+    ao = a.copy(order="F")
+    piv_o, info_o = oracle.pdgetf2(ao, NB, M, N, IA, JA)
+    desca, dinfo = api.descinit(LM, LN, NB, NB, 0, 0, ctx, LM)
+    assert dinfo == 0
+    ipiv = np.zeros(LM + NB, np.int32)
+    before = a.copy(order="F")
+    info = api.pdgetf2(M, N, a, IA, JA, desca, ipiv)
+    mn = min(M, N)
+    assert info == info_o == 0
+    assert np.array_equal(ipiv[IA - 1: IA - 1 + mn], piv_o)
+    sub = a[IA - 1: IA - 1 + M, JA - 1: JA - 1 + N]
+    sub_o = ao[IA - 1: IA - 1 + M, JA - 1: JA - 1 + N]
+    assert np.allclose(sub, sub_o, rtol=1e-12, atol=1e-12)
+    mask = np.ones_like(a, dtype=bool)
+    mask[IA - 1: IA - 1 + M, JA - 1: JA - 1 + N] = False
+    assert np.array_equal(a[mask], before[mask])
+
+
+def test_pdgetf2_zero_column_info(oracle, ctx):
+    """A column that is exactly zero below the diagonal after elimination:
+    INFO = its index (the first such), IPIV(j) = j for it, and the
+    factorization completes (pdgetf2.f:216-230)."""
+    from scalapack_amd import api
+    M, N, NB = 200, 24, 32
+    a = oracle.matgen(M, N, 31)
+    a[:, 9] = 0.0          # column 10 is zero: U(10, 10) = 0
+    a[:, 17] = a[:, 3]     # column 18 repeats column 4: dependent, also zero pivot
+    ao = a.copy(order="F")
+    piv_o, info_o = oracle.pdgetf2(ao, NB, M, N)
+    desca, _ = api.descinit(M, N, NB, NB, 0, 0, ctx, M)
+    ipiv = np.zeros(M + NB, np.int32)
+    info = api.pdgetf2(M, N, a, 1, 1, desca, ipiv)
+    assert info_o == 10 and info == 10
+    assert ipiv[9] == 10
+    assert np.array_equal(ipiv[:9], piv_o[:9])
+
+
 @pytest.mark.parametrize("M", [33, 257, 5000, 70000])
 def test_panel_leaf_heights(oracle, dev, M):
     """A 32-column panel: one leaf kernel over 1 .. 137 workgroups."""

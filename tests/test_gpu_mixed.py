@@ -213,6 +213,52 @@ def test_dxgesv_ill_conditioned(dev, oracle, prec, refine, kappa):
     assert fe <= 100 * fe64 + 1e-14
 
 
+@pytest.mark.parametrize("prec,refine,kappa", [("bf16", "gmres", 1e2), ("bf16", "gmres", 1e3),
+                                               ("bf16", "ir", 1e1), ("fp8", "gmres", 1e1),
+                                               ("fp8", "gmres", 1e2)])
+def test_dxgesv_low_precision_converges(dev, oracle, prec, refine, kappa):
+    """Where kappa times the factors' relative error stays well below 1
+    (BF16 updates: 8 significant bits; FP8-E4M3 with per-block scales: 4)
+    the refinement itself must reach FP64 backward error: no FP64 fallback
+    (ITER >= 0), DSGESV's convergence test or an accepted stall, and a
+    forward error within 100x that of the oracle's FP64 LU."""
+    n = 1024
+    _note(f"dxgesv {prec} {refine} kappa={kappa:g} n={n} (must converge)")
+    A = svd_matrix(n, kappa, 13)
+    rng = np.random.default_rng(14)
+    xt = np.asfortranarray(rng.standard_normal((n, 1)))
+    B = np.asfortranarray(A @ xt)
+    X, it, info, st, _, _ = _solve(dev, A, B, prec, refine)
+    assert info == 0
+    s = scaled_residual(A, X, B)
+    x64 = _oracle_x(oracle, A, B)
+    fe = np.abs(X - xt).max() / np.abs(xt).max()
+    fe64 = np.abs(x64 - xt).max() / np.abs(xt).max()
+    _note(f"  iter={it} {st} sresid={s:.3e} fwd={fe:.3e} fwd(FP64 LU)={fe64:.3e}")
+    assert it >= 0 and st["fell_back"] == 0, "refinement fell back to FP64"
+    assert st["stop_rule"] in ("converged", "stall_accepted")
+    assert s < 16.0
+    assert fe <= 100 * fe64 + 1e-14
+
+
+def test_dxgesv_stall_acceptance_is_opt_out(dev):
+    """ir_accept_stall = 0: a stalled refinement is never returned as a
+    result (the FP64 fallback answers instead); with the default it may be,
+    and stop_rule says which rule ended the refinement."""
+    n = 1024
+    A = svd_matrix(n, 1e4, 21)
+    B = np.asfortranarray(A @ np.ones((n, 1)))
+    dev.set("ir_accept_stall", 0)
+    try:
+        X, it, info, st, _, _ = _solve(dev, A, B, "bf16", "ir")
+    finally:
+        dev.set("ir_accept_stall", 1)
+    _note(f"  strict: iter={it} stop={st['stop_rule']}")
+    assert st["stop_rule"] in ("converged", "fallback")
+    assert (it >= 0) == (st["stop_rule"] == "converged")
+    assert scaled_residual(A, X, B) < 16.0
+
+
 @pytest.mark.parametrize("prec,kappa", [("fp8", 1e6), ("bf16", 1e4)])
 def test_dxgesv_low_precision_falls_back_or_converges(dev, oracle, prec, kappa):
     """kappa times the unit roundoff of the factors above 1 (FP8: 2^-4,
