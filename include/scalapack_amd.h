@@ -215,12 +215,21 @@ int sla_dgemm_sub(sla_handle_t h, int m, int n, int k, const double *dA, int lda
 /* that of LAPACK's DSGESV, the mixed-precision form of PDGESV's solve).      */
 /* ------------------------------------------------------------------------ */
 
-/* Factorization precisions: FP64 (plain), FP32 storage with FP32, BF16 or
- * FP8-E4M3 (per-block-column scales) trailing updates, FP32 accumulation. */
+/* Factorization precisions: FP64 (plain), or FP32 storage whose trailing
+ * updates run with FP32 accumulation on
+ *   FP32 MFMA (SLA_PREC_FP32), BF16 operands (_BF16), FP8-E4M3 operands with
+ *   per-block-column scales (_FP8), or split 16-bit products -- each FP32
+ *   operand as a sum of 16-bit slices, the slice products of low order kept:
+ *   _BF16X3 (2 BF16 slices, 3 products, ~2^-16 |a||b|), _BF16X6 (3 slices,
+ *   6 products, ~2^-24: FP32's own level), _FP16X3 (2 FP16 slices with a
+ *   power-of-two scale per operand block, 3 products, ~2^-21). */
 #define SLA_PREC_FP64 0
 #define SLA_PREC_FP32 1
 #define SLA_PREC_BF16 2
 #define SLA_PREC_FP8 3
+#define SLA_PREC_BF16X3 4
+#define SLA_PREC_BF16X6 5
+#define SLA_PREC_FP16X3 6
 /* Refinement: classical iterative refinement, or GMRES-IR (flexible GMRES on
  * A d = r preconditioned by the low-precision factors). */
 #define SLA_REFINE_IR 0
@@ -261,7 +270,7 @@ int sla_dxgesv(sla_handle_t h, int prec, int refine, int n, int nrhs, double *dA
 int sla_last_solve_info(sla_handle_t h, sla_solve_info_t *out);
 
 /* FP32-storage LU (the low-precision factorization alone), trailing updates
- * in `prec` (SLA_PREC_FP32, _BF16 or _FP8).  Pivots and *info as sla_dgetrf. */
+ * in `prec` (SLA_PREC_FP32 .. SLA_PREC_FP16X3).  Pivots and *info as sla_dgetrf. */
 int sla_sgetrf(sla_handle_t h, int prec, int m, int n, float *dA, int lda, int *dipiv,
                int *info);
 /* Solve A X = B with the FP32 factors of sla_sgetrf (FP32 right-hand sides). */

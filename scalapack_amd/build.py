@@ -24,8 +24,13 @@ BUILD = PKG / "_build"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SLA_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["gemm_f64.hip", "gemm_lowp.hip", "gemm_small.hip", "panel.hip", "panel_tslu.hip", "swap.hip",
-               "trsm.hip", "lu.hip", "mixed.hip", "aux.hip", "batched.hip", "mlp.hip", "capi.hip"]
+# (source, object stem, extra flags): the panel leaf's two storage types are
+# separate objects of one source so that they compile in parallel
+HIP_SOURCES = [(f, f[:-4], []) for f in
+               ["gemm_f64.hip", "gemm_lowp.hip", "gemm_small.hip", "panel_tslu.hip", "swap.hip",
+                "trsm.hip", "lu.hip", "mixed.hip", "aux.hip", "batched.hip", "mlp.hip", "capi.hip"]]
+HIP_SOURCES += [("panel.hip", "panel_f64", ["-DSLA_PANEL_TYPE=64"]),
+                ("panel.hip", "panel_f32", ["-DSLA_PANEL_TYPE=32"])]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}",
@@ -51,13 +56,14 @@ def _run(cmd: list[str]) -> None:
         raise RuntimeError(f"build step failed: {' '.join(cmd)}")
 
 
-def _hip_object(src: str, incremental: bool = False) -> Path:
-    obj = BUILD / (Path(src).stem + ".o")
+def _hip_object(entry, incremental: bool = False) -> Path:
+    src, stem, extra = entry
+    obj = BUILD / (stem + ".o")
     if incremental and obj.exists():
         deps = [CSRC / src, *CSRC.glob("*.h"), *INCLUDE.glob("*.h")]
         if all(d.stat().st_mtime <= obj.stat().st_mtime for d in deps):
             return obj
-    _run([HIPCC, *HIP_FLAGS, "-c", str(CSRC / src), "-o", str(obj)])
+    _run([HIPCC, *HIP_FLAGS, *extra, "-c", str(CSRC / src), "-o", str(obj)])
     return obj
 
 
@@ -68,7 +74,10 @@ def build(jobs: int | None = None, incremental: bool = False) -> None:
     shim = PKG / "libsla_blacs1x1.so"
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", f"-I{INCLUDE}",
           str(CSRC / "blacs1x1.cpp"), "-o", str(shim)])
-    jobs = jobs or min(8, os.cpu_count() or 1)
+    jobs = jobs or min(len(HIP_SOURCES), os.cpu_count() or 1)
+    for stale in BUILD.glob("*.o"):  # objects of sources that no longer exist
+        if stale.stem not in {e[1] for e in HIP_SOURCES}:
+            stale.unlink()
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(lambda src: _hip_object(src, incremental), HIP_SOURCES))
     lib = PKG / "libscalapack_amd.so"

@@ -145,7 +145,7 @@ inline bool lsame(char a, char b) { return std::toupper((unsigned char)a) == b; 
 struct DevBuf {
     void *p = nullptr;
     explicit DevBuf(size_t bytes) { SLA_HIP(hipMalloc(&p, bytes ? bytes : 8)); }
-    ~DevBuf() { if (p) hipFree(p); }
+    ~DevBuf() { if (p) (void)hipFree(p); }
     template <typename T> T *as() { return static_cast<T *>(p); }
 };
 
@@ -1291,7 +1291,7 @@ int sla_dxgesv(sla_handle_t h, int prec, int refine, int n, int nrhs, double *dA
                int *dipiv, const double *dB, int ldb, double *dX, int ldx, int *iter, int *info)
 {
     SLA_NEED_HANDLE(h);
-    if (prec < SLA_PREC_FP64 || prec > SLA_PREC_FP8 || refine < SLA_REFINE_IR ||
+    if (prec < SLA_PREC_FP64 || prec > SLA_PREC_FP16X3 || refine < SLA_REFINE_IR ||
         refine > SLA_REFINE_GMRES) {
         g_last_error = "sla_dxgesv: unknown precision or refinement";
         return SLA_ERR_ARG;
@@ -1443,17 +1443,15 @@ int sla_last_solve_info(sla_handle_t h, sla_solve_info_t *out)
     return SLA_OK;
 }
 
-static int schur_of(int prec)
-{
-    return prec == SLA_PREC_BF16 ? sla::SCHUR_BF16 : prec == SLA_PREC_FP8 ? sla::SCHUR_FP8
-                                                                          : sla::SCHUR_NATIVE;
-}
+static_assert(SLA_PREC_FP16X3 + 1 == sla::PREC_COUNT, "precision codes of the C-ABI");
+
+static int schur_of(int prec) { return sla::schur_of_prec(prec); }
 
 int sla_sgetrf(sla_handle_t h, int prec, int m, int n, float *dA, int lda, int *dipiv, int *info)
 {
     SLA_NEED_HANDLE(h);
-    if (prec < SLA_PREC_FP32 || prec > SLA_PREC_FP8) {
-        g_last_error = "sla_sgetrf: precision must be FP32, BF16 or FP8";
+    if (prec < SLA_PREC_FP32 || prec > SLA_PREC_FP16X3) {
+        g_last_error = "sla_sgetrf: precision must be a low-precision code (FP32 .. FP16X3)";
         return SLA_ERR_ARG;
     }
     if (m < 0 || n < 0 || lda < std::max(1, m)) {
@@ -1493,8 +1491,8 @@ int sla_gemm_lowp_sub(sla_handle_t h, int prec, int m, int n, int k, const float
                       const float *dB, int ldb, float *dC, int ldc)
 {
     SLA_NEED_HANDLE(h);
-    if (prec < SLA_PREC_FP32 || prec > SLA_PREC_FP8) {
-        g_last_error = "sla_gemm_lowp_sub: precision must be FP32, BF16 or FP8";
+    if (prec < SLA_PREC_FP32 || prec > SLA_PREC_FP16X3) {
+        g_last_error = "sla_gemm_lowp_sub: precision must be a low-precision code (FP32 .. FP16X3)";
         return SLA_ERR_ARG;
     }
     if (m < 0 || n < 0 || k < 0 || lda < std::max(1, m) || ldb < std::max(1, k) ||
@@ -1509,6 +1507,25 @@ int sla_gemm_lowp_sub(sla_handle_t h, int prec, int m, int n, int k, const float
             H.timing_begin(sla::CAT_GEMM);
             if (prec == SLA_PREC_FP32) {
                 sla::sgemm_nn_sub(m, n, k, dA, lda, dB, ldb, dC, ldc, s);
+            } else if (const int split = sla::split_of_schur(sla::schur_of_prec(prec))) {
+                const int S = sla::split_slices(split);
+                const int kpad = (k + 31) / 32 * 32;
+                const size_t lplane = (size_t)m * kpad;
+                const size_t uplane = (size_t)n * kpad;
+                void *lp = H.buffer(sla::BUF_PACK_L, lplane * 2 * S + 256);
+                void *up = H.buffer(sla::BUF_PACK_U, uplane * 2 * S + 256);
+                float *sc = static_cast<float *>(H.buffer(sla::BUF_SCALES, 64 * sizeof(float)));
+                sla::fill_u32(reinterpret_cast<unsigned *>(sc), 2, 0u, s);
+                const bool scaled = split == sla::SPLIT_FP16X3;
+                if (scaled) {
+                    sla::absmax_scale(m, k, dA, lda, sc, s);
+                    sla::absmax_scale(k, n, dB, ldb, sc + 1, s);
+                }
+                sla::pack_split(split, true, m, k, kpad, dA, lda, scaled ? sc : nullptr, lp, lplane, s);
+                sla::pack_split(split, false, n, k, kpad, dB, ldb, scaled ? sc + 1 : nullptr, up,
+                                uplane, s);
+                sla::gemm_split_nn_sub(split, m, n, kpad, lp, kpad, lplane, up, kpad, uplane, sc, dC,
+                                       ldc, s);
             } else {
                 const int kq = prec == SLA_PREC_BF16 ? 32 : 128;
                 const int kpad = (k + kq - 1) / kq * kq;

@@ -8,6 +8,10 @@
 //   * FP8-E4M3 MFMA (v_mfma_scale_f32_16x16x128_f8f6f4, unit block scales)
 //     with FP32 accumulation, each operand block divided by its own scale
 //     (max|block| / 448) before rounding and the product scaled back.
+//   * split 16-bit products (BF16 or FP16 MFMA, v_mfma_f32_16x16x32_{bf16,f16}):
+//     every FP32 operand is a sum of S 16-bit slices and the product keeps
+//     the slice products of order below S (3 MFMAs for S = 2, 6 for S = 3),
+//     which restores most of the FP32 product's accuracy at the 16-bit rate.
 // It is the same dgemm_('N','N', ..., -1, L21, U12, +1, A22) of PDGETRF
 // (SRC/pdgetrf.f:288-290), in lower precision.
 //
@@ -24,6 +28,7 @@
 #include <hip/hip_bf16.h>
 
 #include <algorithm>
+#include <type_traits>
 
 namespace sla {
 
@@ -31,6 +36,7 @@ namespace {
 
 typedef float f4_t __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
+typedef _Float16 h8_t __attribute__((ext_vector_type(8)));
 typedef short s8_t __attribute__((ext_vector_type(8)));
 typedef int i8_t __attribute__((ext_vector_type(8)));
 
@@ -315,6 +321,208 @@ __global__ __launch_bounds__(256, 2) void gemm_fp8_nn_sub_direct_kernel(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Split 16-bit products.  Each FP32 operand x (FP16: first scaled by 2^e, e
+// chosen per operand block so that max|x| 2^e lies in [2^14, 2^15)) is written
+// as x = x0 + x1 (+ x2) with x0 = r16(x), x1 = r16(x - x0), x2 = r16(x - x0 -
+// x1) (each difference is exact in FP32), r16 = round to nearest even in the
+// 16-bit format.  The product sums the slice products a_i b_j with i + j < S:
+//   S = 2: a0 b0 + a0 b1 + a1 b0; the dropped a1 b1 and the representation
+//          error are ~2^-16 (BF16) or ~2^-21 (FP16) of |a| |b|;
+//   S = 3: adds a0 b2 + a1 b1 + a2 b0; BF16 error ~2^-24 of |a| |b|, the FP32
+//          product's own level.
+// Packed operands: S planes, plane s of A row-major (k contiguous, ld ldak),
+// plane s of B column-major (k contiguous, ld ldbk), planes aplane / bplane
+// elements apart.  The smaller slice products go first into the FP32
+// accumulator.
+template <typename T16>
+__device__ __forceinline__ f4_t mfma16(s8_t a, s8_t b, f4_t c);
+template <>
+__device__ __forceinline__ f4_t mfma16<__bf16>(s8_t a, s8_t b, f4_t c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8_t, a),
+                                                   __builtin_bit_cast(bf8_t, b), c, 0, 0, 0);
+}
+template <>
+__device__ __forceinline__ f4_t mfma16<_Float16>(s8_t a, s8_t b, f4_t c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h8_t, a),
+                                                  __builtin_bit_cast(h8_t, b), c, 0, 0, 0);
+}
+
+// Power-of-two exponent of an FP16 operand block with max|x| = absmax:
+// absmax 2^e lies in [2^14, 2^15) (0 for a zero block).  BF16 has FP32's
+// range and is not scaled.
+template <typename T16>
+__device__ __forceinline__ int split_exp(float absmax)
+{
+    if (std::is_same<T16, __bf16>::value || !(absmax > 0.f)) return 0;
+    int e = 0;
+    (void)frexpf(absmax, &e);  // absmax = f 2^e, f in [0.5, 1)
+    return 15 - e;
+}
+
+template <typename T16, int S>
+__device__ __forceinline__ void split16(float x, T16 (&out)[S])
+{
+#pragma unroll
+    for (int i = 0; i < S; i++) {
+        out[i] = (T16)x;
+        x -= (float)out[i];
+    }
+}
+
+constexpr int SPDEPTH = 2;
+
+template <typename T16, int S>
+__global__ __launch_bounds__(256, S == 2 ? 2 : 1) void gemm_split_nn_sub_direct_kernel(
+    int m, int n, int k, const T16 *__restrict__ Ap, int ldak, size_t aplane,
+    const T16 *__restrict__ Bp, int ldbk, size_t bplane, const float *__restrict__ absmax,
+    float *__restrict__ C, int ldc)
+{
+    int tmi, tni;
+    tile_coords_lp(blockIdx.x, m / 128, n / 128, tmi, tni);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = tmi * 128 + (wave & 1) * 64, wn = tni * 128 + (wave >> 1) * 64;
+    const int lr = lane & 15, lk = lane >> 4;
+    const T16 *Ar = Ap + (size_t)(wm + lr) * ldak + 8 * lk;
+    const T16 *Bc = Bp + (size_t)(wn + lr) * ldbk + 8 * lk;
+    const size_t astep = (size_t)16 * ldak, bstep = (size_t)16 * ldbk;
+    s8_t a[SPDEPTH][S][4], b[SPDEPTH][S][4];
+    auto load = [&](int t, int q) {
+        const size_t k0 = (size_t)32 * t;
+#pragma unroll
+        for (int sl = 0; sl < S; sl++) {
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                a[q][sl][i] = *(const s8_t *)(Ar + sl * aplane + i * astep + k0);
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                b[q][sl][j] = *(const s8_t *)(Bc + sl * bplane + j * bstep + k0);
+        }
+    };
+    f4_t acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
+    const int nk = k / 32;
+#pragma unroll
+    for (int q = 0; q < SPDEPTH - 1; q++)
+        if (q < nk) load(q, q);
+    for (int t = 0; t < nk; t += SPDEPTH) {
+#pragma unroll
+        for (int q = 0; q < SPDEPTH; q++) {
+            if (t + q < nk) {
+                const int nt = t + q + SPDEPTH - 1;
+                if (nt < nk) load(nt, (q + SPDEPTH - 1) % SPDEPTH);
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        f4_t c = acc[i][j];
+                        if constexpr (S == 3) {
+                            c = mfma16<T16>(a[q][2][i], b[q][0][j], c);
+                            c = mfma16<T16>(a[q][1][i], b[q][1][j], c);
+                            c = mfma16<T16>(a[q][0][i], b[q][2][j], c);
+                        }
+                        c = mfma16<T16>(a[q][1][i], b[q][0][j], c);
+                        c = mfma16<T16>(a[q][0][i], b[q][1][j], c);
+                        acc[i][j] = mfma16<T16>(a[q][0][i], b[q][0][j], c);
+                    }
+            }
+        }
+    }
+    const float back = ldexpf(1.0f, -(split_exp<T16>(absmax[0]) + split_exp<T16>(absmax[1])));
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        float *Cc = C + (size_t)(wn + 16 * j + lr) * ldc;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            f4_t *cp = (f4_t *)(Cc + wm + 16 * i + 4 * lk);
+            f4_t c = *cp;
+            c[0] -= back * acc[i][j][0];
+            c[1] -= back * acc[i][j][1];
+            c[2] -= back * acc[i][j][2];
+            c[3] -= back * acc[i][j][3];
+            *cp = c;
+        }
+    }
+}
+
+// Ragged edges of the split form: one thread per element of C, the same
+// slice products in the same order per k.
+template <typename T16, int S>
+__global__ __launch_bounds__(256) void gemm_split_edge_kernel(
+    int m, int n, int k, const T16 *A, int lda, size_t aplane, const T16 *B, int ldb,
+    size_t bplane, const float *absmax, float *C, int ldc)
+{
+    const int i = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int j = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (i >= m || j >= n) return;
+    float acc = 0.f;
+    for (int kk = 0; kk < k; kk++) {
+        float av[S], bv[S];
+#pragma unroll
+        for (int sl = 0; sl < S; sl++) {
+            av[sl] = (float)A[sl * aplane + (size_t)i * lda + kk];
+            bv[sl] = (float)B[sl * bplane + kk + (size_t)j * ldb];
+        }
+        if constexpr (S == 3) {
+            acc = fmaf(av[2], bv[0], acc);
+            acc = fmaf(av[1], bv[1], acc);
+            acc = fmaf(av[0], bv[2], acc);
+        }
+        acc = fmaf(av[1], bv[0], acc);
+        acc = fmaf(av[0], bv[1], acc);
+        acc = fmaf(av[0], bv[0], acc);
+    }
+    const float back = ldexpf(1.0f, -(split_exp<T16>(absmax[0]) + split_exp<T16>(absmax[1])));
+    C[i + (size_t)j * ldc] -= back * acc;
+}
+
+// L21 (m x k, column-major) -> S row-major planes (ld kpad), zero padded.
+template <typename T16, int S>
+__global__ void pack_rows_split_kernel(int m, int k, int kpad, const float *__restrict__ A, int lda,
+                                       const float *__restrict__ absmax, T16 *__restrict__ P,
+                                       size_t plane)
+{
+    __shared__ float t[32][33];
+    const int r0 = blockIdx.x * 32, k0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    const int e = split_exp<T16>(absmax ? *absmax : 0.f);
+    for (int y = ty; y < 32; y += 8) {
+        const int r = r0 + tx, kk = k0 + y;
+        t[y][tx] = (r < m && kk < k) ? ldexpf(A[r + (size_t)kk * lda], e) : 0.f;
+    }
+    __syncthreads();
+    for (int y = ty; y < 32; y += 8) {
+        const int r = r0 + y, kk = k0 + tx;
+        if (r < m && kk < kpad) {
+            T16 sl[S];
+            split16<T16, S>(t[tx][y], sl);
+#pragma unroll
+            for (int q = 0; q < S; q++) P[q * plane + (size_t)r * kpad + kk] = sl[q];
+        }
+    }
+}
+
+// U12 (k x n, column-major) -> S column-major planes (ld kpad), zero padded.
+template <typename T16, int S>
+__global__ void pack_cols_split_kernel(int k, int kpad, int n, const float *__restrict__ B, int ldb,
+                                       const float *__restrict__ absmax, T16 *__restrict__ P,
+                                       size_t plane)
+{
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)kpad * n) return;
+    const int e = split_exp<T16>(absmax ? *absmax : 0.f);
+    const int kk = (int)(idx % kpad), j = (int)(idx / kpad);
+    T16 sl[S];
+    split16<T16, S>(kk < k ? ldexpf(B[kk + (size_t)j * ldb], e) : 0.f, sl);
+#pragma unroll
+    for (int q = 0; q < S; q++) P[q * plane + idx] = sl[q];
+}
+
 __device__ __forceinline__ float fp8_value(uint8_t b)
 {
     const int e = (b >> 3) & 15, mnt = b & 7;
@@ -540,6 +748,86 @@ void gemm_fp8_nn_sub(int m, int n, int k, const uint8_t *Ap, int ldak, const uin
     hipLaunchKernelGGL(gemm_fp8_edge_kernel, dim3(ceil_div(m, 16), ceil_div(n, 16)), dim3(256), 0,
                        s, m, n, k, Ap, ldak, Bp, ldbk, scales, C, ldc);
     SLA_LAUNCH_CHECK();
+}
+
+template <typename T16, int S>
+static void gemm_split_launch(int m, int n, int k, const T16 *Ap, int ldak, size_t aplane,
+                              const T16 *Bp, int ldbk, size_t bplane, const float *absmax,
+                              float *C, int ldc, hipStream_t s)
+{
+    if (m <= 0 || n <= 0 || k <= 0) return;
+    const int mf = (m / 128) * 128, nf = (n / 128) * 128;
+    if (mf > 0 && nf > 0 && k % 32 == 0 && ldak % 8 == 0 && ldbk % 8 == 0 && ldc % 4 == 0 &&
+        aplane % 8 == 0 && bplane % 8 == 0 && aligned16(Ap) && aligned16(Bp) && aligned16(C)) {
+        hipLaunchKernelGGL((gemm_split_nn_sub_direct_kernel<T16, S>), dim3((mf / 128) * (nf / 128)),
+                           dim3(256), 0, s, mf, nf, k, Ap, ldak, aplane, Bp, ldbk, bplane, absmax,
+                           C, ldc);
+        SLA_LAUNCH_CHECK();
+        if (m > mf)
+            gemm_split_launch<T16, S>(m - mf, n, k, Ap + (size_t)mf * ldak, ldak, aplane, Bp, ldbk,
+                                      bplane, absmax, C + mf, ldc, s);
+        if (n > nf)
+            gemm_split_launch<T16, S>(mf, n - nf, k, Ap, ldak, aplane, Bp + (size_t)nf * ldbk, ldbk,
+                                      bplane, absmax, C + (size_t)nf * ldc, ldc, s);
+        return;
+    }
+    hipLaunchKernelGGL((gemm_split_edge_kernel<T16, S>), dim3(ceil_div(m, 16), ceil_div(n, 16)),
+                       dim3(256), 0, s, m, n, k, Ap, ldak, aplane, Bp, ldbk, bplane, absmax, C,
+                       ldc);
+    SLA_LAUNCH_CHECK();
+}
+
+void gemm_split_nn_sub(int split, int m, int n, int k, const void *Ap, int ldak, size_t aplane,
+                       const void *Bp, int ldbk, size_t bplane, const float *absmax, float *C,
+                       int ldc, hipStream_t s)
+{
+    switch (split) {
+    case SPLIT_BF16X3:
+        gemm_split_launch<__bf16, 2>(m, n, k, static_cast<const __bf16 *>(Ap), ldak, aplane,
+                                     static_cast<const __bf16 *>(Bp), ldbk, bplane, absmax, C,
+                                     ldc, s);
+        break;
+    case SPLIT_BF16X6:
+        gemm_split_launch<__bf16, 3>(m, n, k, static_cast<const __bf16 *>(Ap), ldak, aplane,
+                                     static_cast<const __bf16 *>(Bp), ldbk, bplane, absmax, C,
+                                     ldc, s);
+        break;
+    case SPLIT_FP16X3:
+        gemm_split_launch<_Float16, 2>(m, n, k, static_cast<const _Float16 *>(Ap), ldak, aplane,
+                                       static_cast<const _Float16 *>(Bp), ldbk, bplane, absmax, C,
+                                       ldc, s);
+        break;
+    default: throw std::invalid_argument("split GEMM: unknown split");
+    }
+}
+
+template <typename T16, int S>
+static void pack_split_launch(bool rows, int d0, int k, int kpad, const float *A, int lda,
+                              const float *absmax, void *P, size_t plane, hipStream_t s)
+{
+    T16 *p = static_cast<T16 *>(P);
+    if (rows) {  // d0 = m
+        if (d0 <= 0 || kpad <= 0) return;
+        hipLaunchKernelGGL((pack_rows_split_kernel<T16, S>), dim3(ceil_div(d0, 32), ceil_div(kpad, 32)),
+                           dim3(256), 0, s, d0, k, kpad, A, lda, absmax, p, plane);
+    } else {     // d0 = n
+        if (d0 <= 0 || kpad <= 0) return;
+        const size_t total = (size_t)kpad * d0;
+        hipLaunchKernelGGL((pack_cols_split_kernel<T16, S>), dim3((unsigned)((total + 255) / 256)),
+                           dim3(256), 0, s, k, kpad, d0, A, lda, absmax, p, plane);
+    }
+    SLA_LAUNCH_CHECK();
+}
+
+void pack_split(int split, bool rows, int d0, int k, int kpad, const float *A, int lda,
+                const float *absmax, void *P, size_t plane, hipStream_t s)
+{
+    switch (split) {
+    case SPLIT_BF16X3: pack_split_launch<__bf16, 2>(rows, d0, k, kpad, A, lda, absmax, P, plane, s); break;
+    case SPLIT_BF16X6: pack_split_launch<__bf16, 3>(rows, d0, k, kpad, A, lda, absmax, P, plane, s); break;
+    case SPLIT_FP16X3: pack_split_launch<_Float16, 2>(rows, d0, k, kpad, A, lda, absmax, P, plane, s); break;
+    default: throw std::invalid_argument("split packing: unknown split");
+    }
 }
 
 void convert_f64_f32(int m, int n, const double *A, int lda, float *B, int ldb, int *overflow,

@@ -433,6 +433,37 @@ void Lu<float>::update(int m, int n, int k, const float *l21, const float *u12, 
         h.timing_end(2.0 * m * n * k, s);
         return;
     }
+    if (const int split = split_of_schur(schur)) {
+        // split 16-bit products: S planes of each operand, k padded to 32
+        const int S = split_slices(split);
+        const int kpad = ceil_div(k, 32) * 32;
+        const size_t lplane = (size_t)m * kpad, uplane = (size_t)n * kpad;
+        char *lp = static_cast<char *>(h.buffer(BUF_PACK_L, (size_t)M * kpad * 2 * S + 256));
+        char *up = static_cast<char *>(h.buffer(BUF_PACK_U, (size_t)N * kpad * 2 * S + 256));
+        float *absmax = static_cast<float *>(h.buffer(BUF_SCALES, 64 * sizeof(float)));
+        const bool scaled = split == SPLIT_FP16X3;
+        h.timing_begin(CAT_OTHER, s);
+        if (!l_packed) {
+            if (scaled) {
+                fill_u32(reinterpret_cast<unsigned *>(absmax), 1, 0u, s);
+                absmax_scale(m, k, l21, lda, absmax, s);
+            }
+            pack_split(split, true, m, k, kpad, l21, lda, scaled ? absmax : nullptr, lp, lplane, s);
+        }
+        if (scaled) {
+            fill_u32(reinterpret_cast<unsigned *>(absmax) + 1, 1, 0u, s);
+            absmax_scale(k, n, u12, lda, absmax + 1, s);
+        }
+        pack_split(split, false, n, k, kpad, u12, lda, scaled ? absmax + 1 : nullptr, up, uplane,
+                   s);
+        if (!scaled) fill_u32(reinterpret_cast<unsigned *>(absmax), 2, 0u, s);
+        h.timing_end(0.0, s);
+        l_packed = true;
+        h.timing_begin(CAT_GEMM, s);
+        gemm_split_nn_sub(split, m, n, kpad, lp, kpad, lplane, up, kpad, uplane, absmax, c, lda, s);
+        h.timing_end(2.0 * m * n * k, s);
+        return;
+    }
     // operands k-contiguous, k padded with zeros to the MFMA's K step
     const int kq = schur == SCHUR_BF16 ? 32 : 128;
     const int kpad = ceil_div(k, kq) * kq;
@@ -515,7 +546,7 @@ int dgetrf_device(Handle &h, int m, int n, double *dA, int lda, int *dipiv)
 
 int sgetrf_device(Handle &h, int m, int n, float *dA, int lda, int *dipiv, int schur)
 {
-    if (schur != SCHUR_NATIVE && schur != SCHUR_BF16 && schur != SCHUR_FP8)
+    if (schur < SCHUR_NATIVE || schur > SCHUR_FP16X3)
         throw std::invalid_argument("sgetrf: unknown Schur-update precision");
     return getrf_device<float>(h, m, n, dA, lda, dipiv, schur);
 }

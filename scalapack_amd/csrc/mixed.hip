@@ -589,7 +589,7 @@ int dxgesv_device(Handle &h, int prec, int refine, int n, int nrhs, double *dA, 
         info.anorm = mx.anorm;
         if (ovf) return fallback(-2);
     }
-    const int schur = prec == PREC_BF16 ? SCHUR_BF16 : prec == PREC_FP8 ? SCHUR_FP8 : SCHUR_NATIVE;
+    const int schur = schur_of_prec(prec);
     if (sgetrf_device(h, n, n, mx.S, mx.lds, mx.ipiv_s, schur) > 0) return fallback(-3);
 
     // x0 = solve(b): X = 0, R = B

@@ -275,9 +275,16 @@ unsigned panel_leaf(T *A, int lda, int m, int wa, int lcols, int rcols, int row0
     return (unsigned)wa * (unsigned)G;
 }
 
+// The build compiles this file twice, SLA_PANEL_TYPE = 64 and 32, so that
+// the fully unrolled leaf instances of the two storage types build in
+// parallel; without the macro both are instantiated here.
+#if !defined(SLA_PANEL_TYPE) || SLA_PANEL_TYPE == 64
 template unsigned panel_leaf<double>(double *, int, int, int, int, int, int, int, int *,
                                      const PanelWork<double> &, unsigned, hipStream_t);
+#endif
+#if !defined(SLA_PANEL_TYPE) || SLA_PANEL_TYPE == 32
 template unsigned panel_leaf<float>(float *, int, int, int, int, int, int, int, int *,
                                     const PanelWork<float> &, unsigned, hipStream_t);
+#endif
 
 }  // namespace sla

@@ -25,10 +25,46 @@ enum Category {
     CAT_COUNT
 };
 
-// Precision of the outer trailing update of an FP32-storage factorization.
-enum Schur { SCHUR_NATIVE = 0, SCHUR_BF16 = 1, SCHUR_FP8 = 2 };
+// Precision of the outer trailing update of an FP32-storage factorization:
+// FP32 MFMA, BF16 or FP8 operands, or split 16-bit products (Split).
+enum Schur {
+    SCHUR_NATIVE = 0,
+    SCHUR_BF16 = 1,
+    SCHUR_FP8 = 2,
+    SCHUR_BF16X3 = 3,
+    SCHUR_BF16X6 = 4,
+    SCHUR_FP16X3 = 5
+};
 
-enum Precision { PREC_FP64 = 0, PREC_FP32 = 1, PREC_BF16 = 2, PREC_FP8 = 3 };
+enum Precision {
+    PREC_FP64 = 0,
+    PREC_FP32 = 1,
+    PREC_BF16 = 2,
+    PREC_FP8 = 3,
+    PREC_BF16X3 = 4,
+    PREC_BF16X6 = 5,
+    PREC_FP16X3 = 6,
+    PREC_COUNT
+};
+
+// The trailing-update precision of a low-precision factorization.
+inline int schur_of_prec(int prec)
+{
+    switch (prec) {
+    case PREC_BF16: return SCHUR_BF16;
+    case PREC_FP8: return SCHUR_FP8;
+    case PREC_BF16X3: return SCHUR_BF16X3;
+    case PREC_BF16X6: return SCHUR_BF16X6;
+    case PREC_FP16X3: return SCHUR_FP16X3;
+    default: return SCHUR_NATIVE;
+    }
+}
+inline int split_of_schur(int schur)
+{
+    return schur == SCHUR_BF16X3 ? SPLIT_BF16X3
+         : schur == SCHUR_BF16X6 ? SPLIT_BF16X6
+         : schur == SCHUR_FP16X3 ? SPLIT_FP16X3 : 0;
+}
 enum Refinement { REFINE_IR = 0, REFINE_GMRES = 1 };
 
 struct MixedInfo {

@@ -44,9 +44,28 @@ void pack_rows_fp8(int m, int k, int kpad, const float *A, int lda, const float 
                    uint8_t *P, hipStream_t s);
 void pack_cols_fp8(int k, int kpad, int n, const float *B, int ldb, const float *scale,
                    uint8_t *P, hipStream_t s);
-// *scale = max|A| / 448 (the E4M3 maximum) over an m x n block, or 1 if the
-// block is zero; *scale must hold 0 on entry (bit-pattern max).
+// *scale = max|A| over an m x n block (the FP8 and FP16 packings derive
+// their block scales from it); *scale must hold 0 on entry (bit-pattern max).
 void absmax_scale(int m, int n, const float *A, int lda, float *scale, hipStream_t s);
+
+// Split 16-bit products (gemm_lowp.hip): an FP32 operand as S 16-bit slices,
+// the slice products of order below S on 16-bit MFMA, FP32 accumulation.
+enum Split {
+    SPLIT_BF16X3 = 1,  // 2 BF16 slices, 3 products, ~2^-16 |a||b|
+    SPLIT_BF16X6 = 2,  // 3 BF16 slices, 6 products, ~2^-24 |a||b|
+    SPLIT_FP16X3 = 3   // 2 FP16 slices (power-of-two block scale), 3 products, ~2^-21 |a||b|
+};
+inline int split_slices(int split) { return split == SPLIT_BF16X6 ? 3 : 2; }
+// Packs A (rows: m x k -> S row-major planes) or B (!rows: k x n -> S
+// column-major planes), k padded with zeros to kpad, planes `plane` elements
+// apart; absmax -> max|block| (absmax_scale), read by the FP16 split only.
+void pack_split(int split, bool rows, int d0, int k, int kpad, const float *A, int lda,
+                const float *absmax, void *P, size_t plane, hipStream_t s);
+// C[m x n] -= A B from packed planes; absmax[0], absmax[1] the maxima the
+// A and B packings were scaled by (FP16 split).
+void gemm_split_nn_sub(int split, int m, int n, int k, const void *Ap, int ldak, size_t aplane,
+                       const void *Bp, int ldbk, size_t bplane, const float *absmax, float *C,
+                       int ldc, hipStream_t s);
 
 // ---- precision conversions -------------------------------------------------
 // B = (float)A; sets *overflow when some |A(i,j)| exceeds the FP32 range.

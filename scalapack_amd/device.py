@@ -14,7 +14,7 @@ from ._lib import SolveInfo, last_error, load
 STAT_CLASSES = ("panel", "laswp", "trsm", "gemm", "trsv", "other", "gemm_panel")
 
 # factorization precisions and refinements (SLA_PREC_*, SLA_REFINE_*)
-PRECISIONS = {"fp64": 0, "fp32": 1, "bf16": 2, "fp8": 3}
+PRECISIONS = {"fp64": 0, "fp32": 1, "bf16": 2, "fp8": 3, "bf16x3": 4, "bf16x6": 5, "fp16x3": 6}
 REFINEMENTS = {"ir": 0, "gmres": 1}
 # what ended a mixed-precision refinement (SLA_STOP_*)
 STOP_RULES = ("converged", "stall_accepted", "fallback")
