@@ -300,6 +300,24 @@ int sla_dresidual(sla_handle_t h, int n, int nrhs, const double *dA, int lda, co
 int sla_gesv_batched(sla_handle_t h, int prec, int batch, const int *n, const int *nrhs,
                      void *const *dA, const int *lda, int *const *dipiv, void *const *dB,
                      const int *ldb, int *dinfo);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): sla_gesv_batched with the sizes in DEVICE arrays dn,
+ * dnrhs, dlda, dldb and no allocation of any kind -- the queue, the order and
+ * the size descriptors live in a fixed device array of the library (about
+ * 1.3 MB per GPU, held until the process ends), so at most
+ * SLA_BATCHED_NOALLOC_MAX systems per call.  A system whose sizes are out of
+ * range (n > 1024, nrhs > 16, lda or ldb < max(1, n)) gets INFO = -1 and is
+ * left untouched; every other system's A, IPIV, B and INFO are exactly those
+ * of sla_gesv_batched.  Uses the handle's stream, panel width and grid
+ * ("batched_w", "batched_wgs_per_cu"; "batched_graph" is ignored) and only
+ * enqueues the work: no synchronisation, so the call can be captured.  Calls
+ * that overlap on one GPU share the array and race: order them on one
+ * stream.  The handle's device must be current. */
+#define SLA_BATCHED_NOALLOC_MAX 65536
+int sla_gesv_batched_noalloc(sla_handle_t h, int prec, int batch, const int *dn,
+                             const int *dnrhs, void *const *dA, const int *dlda,
+                             int *const *dipiv, void *const *dB, const int *dldb, int *dinfo);
 /* Autotuner: times every panel width and workgroups-per-CU on synthetic
  * systems of these sizes (its own device memory), keeps the fastest in the
  * handle's knobs and returns it (*ms: the batch's time in milliseconds). */

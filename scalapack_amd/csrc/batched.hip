@@ -30,6 +30,7 @@
 #include <algorithm>
 #include <climits>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -295,7 +296,8 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int batch, const int *
         const SysDesc d = desc[s];
         T *a = A[s];
         int *piv = ipiv[s];
-        int inf = 0;
+        // n < 0 marks a system whose sizes the allocation-free entry rejected
+        int inf = d.n < 0 ? -1 : 0;
         if (d.n > 0) factor_system<T, W>(d.n, a, d.lda, piv, inf, Ls, Us, s_v, s_i);
         if (threadIdx.x == 0) info[s] = inf;
         if (inf == 0 && d.n > 0)
@@ -307,6 +309,46 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int batch, const int *
 __global__ void zero_queue_kernel(int *q)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) *q = 0;
+}
+
+// Set-up of the allocation-free entry, one workgroup: validates the device
+// size arrays (a rejected system gets n = -1, i.e. INFO = -1, and no work),
+// orders the systems largest n first by a counting sort (the order only
+// balances the queue: every system's result is independent of it) and resets
+// the queue.  Buckets: NMAX - n for a valid n, NMAX + 1 for a rejected one.
+constexpr int PREP_BUCKETS = BATCHED_NMAX + 2;
+
+__global__ __launch_bounds__(1024) void noalloc_prep_kernel(int batch, const int *__restrict__ n,
+                                                            const int *__restrict__ nrhs,
+                                                            const int *__restrict__ lda,
+                                                            const int *__restrict__ ldb,
+                                                            int *queue, int *order, SysDesc *desc)
+{
+    __shared__ int cnt[PREP_BUCKETS];
+    for (int b = threadIdx.x; b < PREP_BUCKETS; b += blockDim.x) cnt[b] = 0;
+    if (threadIdx.x == 0) *queue = 0;
+    __syncthreads();
+    for (int s = threadIdx.x; s < batch; s += blockDim.x) {
+        const int ns = n[s], rs = nrhs[s], la = lda[s], lb = ldb[s];
+        const bool ok = ns >= 0 && ns <= BATCHED_NMAX && rs >= 0 && rs <= BATCHED_NRHS_MAX &&
+                        la >= max(1, ns) && lb >= max(1, ns);
+        desc[s] = ok ? SysDesc{ns, rs, la, lb} : SysDesc{-1, 0, 1, 1};
+        atomicAdd(&cnt[ok ? BATCHED_NMAX - ns : BATCHED_NMAX + 1], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {  // exclusive prefix sum of the bucket counts
+        int run = 0;
+        for (int b = 0; b < PREP_BUCKETS; b++) {
+            const int c = cnt[b];
+            cnt[b] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    for (int s = threadIdx.x; s < batch; s += blockDim.x) {
+        const int ns = desc[s].n;  // written by this thread above
+        order[atomicAdd(&cnt[ns >= 0 ? BATCHED_NMAX - ns : BATCHED_NMAX + 1], 1)] = s;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -438,6 +480,82 @@ BatchedPlan &batched_plan(Handle &h, int batch, const int *n, const int *nrhs, c
 }
 
 void batched_plan_free(BatchedPlan *p) { delete p; }
+
+}  // namespace sla
+
+// ---------------------------------------------------------------------------
+// Allocation-free entry: the queue counter, the order and the size
+// descriptors live in this fixed device array (one copy per GPU, kept until
+// the process ends) instead of memory the library allocates.  Layout: queue
+// at 0, order at 256, descriptors after BATCHED_NOALLOC_MAX ints.
+static __device__ __align__(256) unsigned char
+    scratch[256 + (size_t)sla::BATCHED_NOALLOC_MAX * (sizeof(int) + 4 * sizeof(int))];
+
+namespace sla {
+
+namespace {
+
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[64] = {};  // device address of `scratch` on each GPU
+
+// Device address of this GPU's `scratch` (looked up once per device, with
+// that device current; host code never uses the array's name as an address).
+void *noalloc_scratch(int device)
+{
+    if (device < 0 || device >= 64) throw std::runtime_error("batched: device id out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        SLA_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw std::runtime_error("batched: no device address for the scratch array");
+        g_scratch_ptr[device] = p;
+    }
+    return g_scratch_ptr[device];
+}
+
+}  // namespace
+
+template <typename T>
+void gesv_batched_noalloc_device(Handle &h, int batch, const int *dn, const int *dnrhs,
+                                 T *const *dA, const int *dlda, int *const *dipiv, T *const *dB,
+                                 const int *dldb, int *dinfo)
+{
+    if (batch < 0 || batch > BATCHED_NOALLOC_MAX)
+        throw std::invalid_argument("batched (no allocation): batch above " +
+                                    std::to_string(BATCHED_NOALLOC_MAX) + " systems");
+    int cur = -1;
+    SLA_HIP(hipGetDevice(&cur));
+    if (cur != h.device)
+        throw std::runtime_error("batched (no allocation): the handle's device must be current");
+    if (batch == 0) return;
+    static_assert(sizeof(SysDesc) == 4 * sizeof(int), "descriptor layout of the scratch array");
+    char *base = static_cast<char *>(noalloc_scratch(cur));
+    int *queue = reinterpret_cast<int *>(base);
+    int *order = reinterpret_cast<int *>(base + 256);
+    SysDesc *desc = reinterpret_cast<SysDesc *>(base + 256 + sizeof(int) * BATCHED_NOALLOC_MAX);
+    hipLaunchKernelGGL(noalloc_prep_kernel, dim3(1), dim3(1024), 0, h.stream, batch, dn, dnrhs,
+                       dlda, dldb, queue, order, desc);
+    SLA_LAUNCH_CHECK();
+    const int wgs = std::max(1, h.batched_wgs_per_cu) * h.num_cus;
+#define SLA_BATCHED(WV)                                                                          \
+    hipLaunchKernelGGL((gesv_batched_kernel<T, WV>), dim3(wgs), dim3(BT), 0, h.stream, batch,     \
+                       order, desc, dA, dipiv, dB, dinfo, queue)
+    switch (h.batched_w) {
+    case 8: SLA_BATCHED(8); break;
+    case 16: SLA_BATCHED(16); break;
+    case 32: SLA_BATCHED(32); break;
+    default: throw std::invalid_argument("batched: panel width must be 8, 16 or 32");
+    }
+#undef SLA_BATCHED
+    SLA_LAUNCH_CHECK();
+}
+
+template void gesv_batched_noalloc_device<double>(Handle &, int, const int *, const int *,
+                                                  double *const *, const int *, int *const *,
+                                                  double *const *, const int *, int *);
+template void gesv_batched_noalloc_device<float>(Handle &, int, const int *, const int *,
+                                                 float *const *, const int *, int *const *,
+                                                 float *const *, const int *, int *);
 
 template <typename T>
 void gesv_batched_device(Handle &h, int batch, const int *n, const int *nrhs, T *const *dA,

@@ -1132,6 +1132,34 @@ int sla_gesv_batched(sla_handle_t h, int prec, int batch, const int *n, const in
                                             dinfo))
 }
 
+static_assert(SLA_BATCHED_NOALLOC_MAX == sla::BATCHED_NOALLOC_MAX,
+              "systems per call of the allocation-free batched entry");
+
+int sla_gesv_batched_noalloc(sla_handle_t h, int prec, int batch, const int *dn,
+                             const int *dnrhs, void *const *dA, const int *dlda,
+                             int *const *dipiv, void *const *dB, const int *dldb, int *dinfo)
+{
+    SLA_NEED_HANDLE(h);
+    if (prec != SLA_PREC_FP64 && prec != SLA_PREC_FP32) {
+        g_last_error = "sla_gesv_batched_noalloc: prec must be SLA_PREC_FP64 or SLA_PREC_FP32";
+        return SLA_ERR_ARG;
+    }
+    if (batch < 0 || batch > SLA_BATCHED_NOALLOC_MAX ||
+        (batch > 0 && (!dn || !dnrhs || !dlda || !dldb || !dA || !dipiv || !dB || !dinfo))) {
+        g_last_error = "sla_gesv_batched_noalloc: batch outside 0.." +
+                       std::to_string(SLA_BATCHED_NOALLOC_MAX) + " or null array";
+        return SLA_ERR_ARG;
+    }
+    auto &H = *reinterpret_cast<sla::Handle *>(h);
+    if (prec == SLA_PREC_FP64)
+        SLA_TRY(sla::gesv_batched_noalloc_device<double>(
+            H, batch, dn, dnrhs, reinterpret_cast<double *const *>(dA), dlda, dipiv,
+            reinterpret_cast<double *const *>(dB), dldb, dinfo))
+    SLA_TRY(sla::gesv_batched_noalloc_device<float>(
+        H, batch, dn, dnrhs, reinterpret_cast<float *const *>(dA), dlda, dipiv,
+        reinterpret_cast<float *const *>(dB), dldb, dinfo))
+}
+
 int sla_gesv_batched_tune(sla_handle_t h, int prec, int batch, const int *n, const int *nrhs,
                           int *w, int *wgs_per_cu, double *ms)
 {

@@ -9,6 +9,8 @@
 namespace sla {
 
 constexpr int BATCHED_NMAX = 1024, BATCHED_NRHS_MAX = 16;
+// Systems per call of the allocation-free entry (its fixed scratch array).
+constexpr int BATCHED_NOALLOC_MAX = 65536;
 
 struct BatchedTune {
     struct Entry {
@@ -29,6 +31,16 @@ template <typename T>
 void gesv_batched_device(Handle &h, int batch, const int *n, const int *nrhs, T *const *dA,
                          const int *lda, int *const *dipiv, T *const *dB, const int *ldb,
                          int *dinfo);
+// The same with the sizes in device arrays and no device memory allocated:
+// the queue, the order and the descriptors live in a fixed device array of
+// this library (at most BATCHED_NOALLOC_MAX systems).  A system whose sizes
+// are out of range gets INFO = -1 and no work.  Only enqueues work on the
+// handle's stream (no synchronisation, so a caller may capture it in a
+// graph); calls that overlap on one GPU share the array and race.
+template <typename T>
+void gesv_batched_noalloc_device(Handle &h, int batch, const int *dn, const int *dnrhs,
+                                 T *const *dA, const int *dlda, int *const *dipiv, T *const *dB,
+                                 const int *dldb, int *dinfo);
 // Times every (panel width, workgroups per CU) on synthetic systems of these
 // sizes, stores the fastest in the handle and returns the table.
 template <typename T>

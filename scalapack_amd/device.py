@@ -179,6 +179,14 @@ class Handle:
                                           _ptr(a_ptrs), arr(lda), _ptr(ipiv_ptrs), _ptr(b_ptrs),
                                           arr(ldb), _ptr(info)))
 
+    def gesv_batched_noalloc(self, prec: str, batch: int, dn, dnrhs, a_ptrs, dlda, ipiv_ptrs,
+                             b_ptrs, dldb, info) -> None:
+        """sla_gesv_batched_noalloc: sizes as device int32 tensors, nothing
+        allocated, work only enqueued on the handle's stream."""
+        _check(self._lib.sla_gesv_batched_noalloc(
+            self._h, PRECISIONS[prec], batch, _ptr(dn), _ptr(dnrhs), _ptr(a_ptrs), _ptr(dlda),
+            _ptr(ipiv_ptrs), _ptr(b_ptrs), _ptr(dldb), _ptr(info)))
+
     def gesv_batched_tune(self, prec: str, n, nrhs) -> tuple[int, int, float]:
         """Autotune (panel width, workgroups per CU) for these sizes; the
         handle keeps the result.  Returns (w, wgs_per_cu, ms)."""
