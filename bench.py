@@ -48,7 +48,12 @@ import time
 from pathlib import Path
 
 ROOT = Path(__file__).resolve().parent
-PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0, "fp8": 5000.0}  # dense matrix TFLOP/s
+# dense matrix peaks, TFLOP/s; a split product (bf16x3: three BF16 products per
+# FP32 product, bf16x6: six, fp16x3: three FP16) is priced at its 16-bit MFMA
+# peak divided by the products it issues, so `achieved` stays the algorithmic
+# 2 m n k rate
+PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0, "fp8": 5000.0,
+        "bf16x3": 2500.0 / 3, "bf16x6": 2500.0 / 6, "fp16x3": 2500.0 / 3}
 HBM_PEAK_GBS = 8000.0
 EPS = 2.0 ** -53
 METRIC = "2N^3/3 TFLOPS-equiv at FP64 backward error: mixed-prec LU+IR vs FP64 LU, N=32k"
@@ -64,7 +69,14 @@ def parse():
     ap.add_argument("--nb", type=int, default=512)
     ap.add_argument("--mode", choices=["mixed", "fp64", "batched"], default="mixed")
     ap.add_argument("--batch", type=int, default=10000, help="batched mode: number of systems")
-    ap.add_argument("--prec", choices=["fp32", "bf16", "fp8", "fp64"], default="fp32",
+    ap.add_argument("--policy", choices=["auto", "fixed"], default="fixed",
+                    help="mixed mode: auto = the learned selector (sla_dxgesv_auto) picks "
+                         "precision and refinement per matrix inside the timed region; fixed = "
+                         "--prec with classical IR (GMRES-IR from --gmres-from)")
+    ap.add_argument("--model", default=str(ROOT / "scalapack_amd" / "data" / "selector_model.npz"),
+                    help="selector parameters for --policy auto")
+    ap.add_argument("--prec", choices=["fp32", "bf16", "fp8", "bf16x3", "bf16x6", "fp16x3", "fp64"],
+                    default="fp32",
                     help="factorization precision of the mixed solve")
     ap.add_argument("--kappas", default="1e2,1e4,1e6,1e8")
     ap.add_argument("--gmres-from", type=float, default=1e6,
@@ -74,6 +86,11 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true", help="mixed mode: skip the FP64 comparison")
     ap.add_argument("--param", action="append", default=[],
                     help="handle knob name=value (sla_set_param), may repeat")
+    ap.add_argument("--save-matrices", default="", metavar="DIR",
+                    help="mixed mode: also write each kappa's A and x_true (raw FP64) into DIR")
+    ap.add_argument("--load-matrices", default="", metavar="DIR",
+                    help="mixed mode: read A and x_true written by --save-matrices instead of "
+                         "generating them (no QR in this process: counter runs)")
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="GIB",
                     help="only copy GIB GiB device-to-device 4 times (the known-size kernel that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -115,6 +132,22 @@ def scaled_residual(torch, AT, B0, X) -> float:
     return (R.abs().amax(dim=0) / (X.T.abs().amax(dim=0) * anorm * EPS * n)).max().item()
 
 
+def forward_error(torch, X, XT) -> float:
+    """max over columns of ||x - x_true||_inf / ||x_true||_inf; X, XT: (nrhs, n)."""
+    return ((X - XT).abs().amax(dim=1) / XT.abs().amax(dim=1)).max().item()
+
+
+def stored_matrices(torch, n: int, kappas, folder: str):
+    """The matrices svd_matrices wrote with --save-matrices (raw FP64 storage
+    of A^T, one file per kappa)."""
+    import numpy as np
+    for kappa in kappas:
+        a = np.fromfile(Path(folder) / f"A_{n}_{kappa:g}.f64", dtype=np.float64)
+        if a.size != n * n:
+            raise SystemExit(f"{folder}: no {n} x {n} matrix for kappa {kappa:g}")
+        yield kappa, torch.from_numpy(a.reshape(n, n)).to("cuda:0")
+
+
 def svd_matrices(torch, n: int, kappas, seed: int = 100):
     """Storage of A = U diag(sigma) V^T (column-major: the (n, n) tensor
     A^T = V diag(sigma) U^T) for each kappa; U, V from QR of seeded
@@ -131,11 +164,18 @@ def svd_matrices(torch, n: int, kappas, seed: int = 100):
         torch.cuda.synchronize()
 
 
+# the selector's classes (scalapack_amd.selector.CHOICES) as "precision+refinement"
+CHOICE_NAMES = ("fp8+ir", "fp8+gmres", "bf16x3+ir", "bf16x3+gmres", "fp32+ir", "fp32+gmres",
+                "fp64")
+
 KERNEL_OF = {  # kernel symbol behind each timing class (for matching kernel traces)
     ("gemm", "fp64"): "dgemm_nn_sub_direct_kernel<1> (trailing update A22 -= L21 U12)",
     ("gemm", "fp32"): "sgemm_nn_sub_direct_kernel<1> (trailing update, FP32 MFMA)",
     ("gemm", "bf16"): "gemm_bf16_nn_sub_direct_kernel (trailing update, BF16 MFMA)",
     ("gemm", "fp8"): "gemm_fp8_nn_sub_direct_kernel (trailing update, FP8 MFMA)",
+    ("gemm", "bf16x3"): "gemm_split_nn_sub_direct_kernel<__bf16, 2> (trailing update, BF16x3)",
+    ("gemm", "bf16x6"): "gemm_split_nn_sub_direct_kernel<__bf16, 3> (trailing update, BF16x6)",
+    ("gemm", "fp16x3"): "gemm_split_nn_sub_direct_kernel<_Float16, 2> (trailing update, FP16x3)",
     ("panel", "fp64"): "panel_leaf_kernel<double, 32, RPT>",
 }
 
@@ -155,14 +195,46 @@ def measured_traffic(kernel: str, workload: dict):
     return None
 
 
+def kernel_stats(h) -> dict:
+    """Per-class kernel statistics of the handle; the trailing-update class
+    also carries the launches' own execution intervals (exec_*)."""
+    stats = h.stats()
+    ex = h.update_exec()
+    stats["gemm"].update({"exec_ms": ex["ms"], "exec_work": ex["work"],
+                          "exec_launches": ex["launches"]})
+    return stats
+
+
+def add_stats(total: dict, stats: dict) -> None:
+    for k, v in stats.items():
+        a = total.setdefault(k, {f: 0.0 for f in v})
+        for f in v:
+            a[f] = a.get(f, 0.0) + v[f]
+
+
 def roofline_from(stats: dict, steps: int, gemm_prec: str, workload: dict | None = None) -> dict:
-    """Roofline of the kernel class with the most summed time."""
+    """Roofline of the kernel class with the most summed time.  For the
+    trailing update the duration is the launches' execution interval as the
+    kernels measure it (first workgroup start to last workgroup end, the
+    quantity rocprofv3's kernel trace reports); the events around each launch
+    on its stream also count its wait for CUs held by the panel stream and
+    are given beside it."""
     cls, st = max(stats.items(), key=lambda kv: kv[1]["ms"])
     secs = st["ms"] / 1e3
+    work, launches = st["work"], st["launches"]
     base = {"kernel_class": cls, "kernel": KERNEL_OF.get((cls, gemm_prec), cls),
-            "launches_per_step": st["launches"] // max(1, steps),
+            "launches_per_step": int(st["launches"] // max(1, steps)),
             "avg_launch_ms": round(st["ms"] / max(1, st["launches"]), 4),
-            "avg_launch_work": st["work"] / max(1, st["launches"])}
+            "avg_launch_work": st["work"] / max(1, st["launches"]),
+            "timing": "HIP events on the launch stream"}
+    if cls == "gemm" and st.get("exec_launches", 0) > 0:
+        secs, work, launches = st["exec_ms"] / 1e3, st["exec_work"], st["exec_launches"]
+        base.update({"avg_launch_ms": round(st["exec_ms"] / launches, 4),
+                     "avg_launch_work": work / launches,
+                     "events_avg_launch_ms": round(st["ms"] / max(1, st["launches"]), 4),
+                     "timing": "in-kernel execution interval (first workgroup start to last "
+                               "workgroup end, device wall clock)"})
+    st = dict(st, work=work)
     if cls in ("gemm", "gemm_panel", "trsm", "panel"):
         # flops in the GEMM's product precision; panel and trsm compute in the
         # storage precision (FP64, or FP32 for every mixed factorization)
@@ -183,7 +255,9 @@ def roofline_from(stats: dict, steps: int, gemm_prec: str, workload: dict | None
     return out
 
 
-def run_fp64(torch, h, AT0, B0, steps, warmup):
+def run_fp64(torch, h, AT0, B0, steps, warmup, XT=None):
+    """Timed sla_dgesv solves; returns (times, scaled residuals, kernel
+    statistics, forward errors against XT, or [] without XT)."""
     n, nrhs = AT0.shape[0], B0.shape[0]
     AT = torch.empty_like(AT0)
     B = torch.empty_like(B0)
@@ -196,7 +270,7 @@ def run_fp64(torch, h, AT0, B0, steps, warmup):
     torch.cuda.synchronize()
     h.set("profile", 1)
     h.reset_stats()
-    times, sres = [], []
+    times, sres, fwd = [], [], []
     for _ in range(steps):
         AT.copy_(AT0)
         B.copy_(B0)
@@ -207,51 +281,68 @@ def run_fp64(torch, h, AT0, B0, steps, warmup):
         times.append(time.perf_counter() - t0)
         assert info == 0, f"zero pivot at {info}"
         sres.append(scaled_residual(torch, AT0, B0, B))
-    stats = h.stats()
+        if XT is not None:
+            fwd.append(forward_error(torch, B, XT))
+    stats = kernel_stats(h)
     h.set("profile", 0)
-    return times, sres, stats
+    return times, sres, stats, fwd
 
 
-def run_mixed(torch, h, AT, B0, steps, warmup, prec, refine):
-    """Timed sla_dxgesv solves of A x = b with the (prec, refine) policy.  If
-    the untimed warmup shows that refinement would not converge (the solve
-    falls back to FP64, ITER < 0), the policy for this matrix becomes plain
-    FP64 LU (prec "fp64"), as a precision selector would choose; the line
-    records the policy used.  A solve that factors A in FP64 overwrites it, so
-    such steps restore A from AT outside the timed region."""
+def run_mixed(torch, h, AT, B0, steps, warmup, prec, refine, XT, auto=False):
+    """Timed mixed-precision solves of A x = b, ending at FP64 backward error.
+
+    auto: sla_dxgesv_auto -- the learned selector picks precision and
+    refinement from the matrix's features on every step (features, network
+    and solve all inside the timed region), with sla_dxgesv's FP64 fallback
+    as the judge.  Otherwise sla_dxgesv with the fixed (prec, refine) policy;
+    if its untimed warmup shows that refinement would not converge (ITER <
+    0), the policy for this matrix becomes plain FP64 LU, and the line
+    records that.  A solve that factors A in FP64 overwrites it, so every
+    step starts from a copy of AT made outside the timed region."""
     n, nrhs = AT.shape[0], B0.shape[0]
     X = torch.zeros_like(B0)
     ipiv = torch.zeros(n, dtype=torch.int32, device=AT.device)
     A = AT.clone()
     policy = (prec, refine)
     fell_back_in_warmup = None
+    choices = []
+
+    def solve():
+        if auto:
+            c, it, info = h.dxgesv_auto(n, nrhs, A, n, ipiv, B0, n, X, n)
+            choices.append(c)
+            return it, info
+        return h.dxgesv(n, nrhs, A, n, ipiv, B0, n, X, n, prec=policy[0], refine=policy[1])
+
     for _ in range(max(1, warmup)):
-        it, _ = h.dxgesv(n, nrhs, A, n, ipiv, B0, n, X, n, prec=policy[0], refine=policy[1])
-        if it < 0 and policy[0] != "fp64":
+        A.copy_(AT)
+        it, _ = solve()
+        if not auto and it < 0 and policy[0] != "fp64":
             fell_back_in_warmup = h.last_solve_info()
             policy = ("fp64", "ir")
-            A.copy_(AT)
-            h.dxgesv(n, nrhs, A, n, ipiv, B0, n, X, n, prec="fp64", refine="ir")
     torch.cuda.synchronize()
+    choices.clear()
     h.set("profile", 1)
     h.reset_stats()
-    times, sres, infos = [], [], []
+    times, sres, infos, fwd = [], [], [], []
     for _ in range(steps):
-        if policy[0] == "fp64":
-            A.copy_(AT)
+        A.copy_(AT)
         X.zero_()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        it, info = h.dxgesv(n, nrhs, A, n, ipiv, B0, n, X, n, prec=policy[0], refine=policy[1])
+        it, info = solve()
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t0)
         assert info == 0, f"zero pivot at {info}"
         infos.append(h.last_solve_info())
         sres.append(scaled_residual(torch, AT, B0, X))
-    stats = h.stats()
+        fwd.append(forward_error(torch, X, XT))
+    stats = kernel_stats(h)
     h.set("profile", 0)
     del A
-    return times, sres, stats, infos, policy, fell_back_in_warmup
+    if auto:
+        policy = ("auto", [CHOICE_NAMES[c] for c in choices])
+    return times, sres, stats, infos, policy, fell_back_in_warmup, fwd
 
 
 def run_batched(torch, h, args) -> dict:
@@ -386,7 +477,8 @@ def main() -> int:
         g.manual_seed(100)
         AT0 = torch.rand((n, n), generator=g, device=dev, dtype=torch.float64).mul_(2.0).sub_(1.0)
         B0 = AT0.sum(dim=0, keepdim=True).expand(nrhs, n).contiguous()  # b = A * ones
-        times, sres, stats = run_fp64(torch, h, AT0, B0, args.steps, args.warmup)
+        XT = torch.ones((nrhs, n), device=dev, dtype=torch.float64)
+        times, sres, stats, fwd = run_fp64(torch, h, AT0, B0, args.steps, args.warmup, XT)
         ms = 1e3 * sum(times) / len(times)
         line.update({
             "value": round(flops / (ms / 1e3) / 1e12, 3), "unit": "TFLOPS",
@@ -395,7 +487,7 @@ def main() -> int:
             "config": {"workload": f"plain FP64 LU solve PDGESV (1x1 grid), N={n}, NRHS={nrhs}, "
                                    f"NB={nb}", "n": n, "nrhs": nrhs, "nb": nb,
                        "precision": "fp64", "params": args.param},
-            "scaled_residual": max(sres),
+            "scaled_residual": max(sres), "forward_error": max(fwd),
             "kernel_ms_per_step": {k: round(v["ms"] / args.steps, 2) for k, v in stats.items()},
             "roofline": roofline_from(stats, args.steps, "fp64",
                                       {"mode": "fp64", "n": n, "nb": nb}),
@@ -403,71 +495,103 @@ def main() -> int:
         ok = max(sres) < 16.0
     else:
         kappas = [float(k) for k in args.kappas.split(",")]
-        t_gen = time.perf_counter()
         per_kappa, all_stats, total_t, total_steps, ok = [], {}, 0.0, 0, True
+        steps_by_prec = {}
+        if args.policy == "auto":
+            import numpy as np
+            h.selector_set_model(np.load(args.model)["params"])
         fp64_line = None
         rng = torch.Generator(device=dev)
         rng.manual_seed(200)
         XT = torch.randn((nrhs, n), generator=rng, device=dev, dtype=torch.float64)
-        for kappa, AT in svd_matrices(torch, n, kappas):
-            gen_s = time.perf_counter() - t_gen
+        source = (stored_matrices(torch, n, kappas, args.load_matrices) if args.load_matrices
+                  else svd_matrices(torch, n, kappas))
+        for kappa, AT in source:
+            if args.save_matrices:
+                Path(args.save_matrices).mkdir(parents=True, exist_ok=True)
+                AT.cpu().numpy().tofile(Path(args.save_matrices) / f"A_{n}_{kappa:g}.f64")
             B0 = (AT.T @ XT.T).T.contiguous()  # b = A x_true
             if os.environ.get("SLA_BENCH_DEBUG"):
                 print(f"kappa={kappa:g} sum(A)={AT.sum().item():.17e} sum(b)={B0.sum().item():.17e} "
                       f"ptrs A={AT.data_ptr():#x} b={B0.data_ptr():#x}", file=sys.stderr, flush=True)
             refine = "gmres" if kappa >= args.gmres_from else "ir"
-            times, sres, stats, infos, policy, fb = run_mixed(torch, h, AT, B0, args.steps,
-                                                              args.warmup, args.prec, refine)
-            for k, v in stats.items():
-                a = all_stats.setdefault(k, {"ms": 0.0, "work": 0.0, "launches": 0})
-                for f in a:
-                    a[f] += v[f]
+            times, sres, stats, infos, policy, fb, fwd = run_mixed(
+                torch, h, AT, B0, args.steps, args.warmup, args.prec, refine, XT,
+                auto=args.policy == "auto")
+            # precision of the trailing update these solves ran (the last one's)
+            used = infos[-1]
+            gemm_prec = "fp64" if used["fell_back"] or used["precision"] == "fp64" \
+                else used["precision"]
+            # the FP64 LU's errors on the same system (one untimed solve)
+            if not args.no_fp64:
+                _, fs1, _, ff1 = run_fp64(torch, h, AT, B0, 1, 0, XT)
+                fp64_errors = {"scaled_residual": fs1[0], "forward_error": ff1[0]}
+            else:
+                fp64_errors = None
+            add_stats(all_stats.setdefault(gemm_prec, {}), stats)
+            steps_by_prec[gemm_prec] = steps_by_prec.get(gemm_prec, 0) + len(times)
             total_t += sum(times)
             total_steps += len(times)
             ok &= max(sres) < 16.0
-            per_kappa.append({"kappa": kappa, "precision": policy[0], "refine": policy[1],
+            per_kappa.append({"kappa": kappa, "policy": policy[0],
+                              "choices": policy[1] if policy[0] == "auto" else None,
+                              "precision": used["precision"], "refine": used["refinement"],
+                              "fell_back": bool(used["fell_back"]),
+                              "stop_rule": used["stop_rule"],
                               "mixed_attempt_fell_back": (None if fb is None else
                                                           [float(f"{v:.3e}") for v in fb["history"]]),
                               "ms_per_solve": round(1e3 * sum(times) / len(times), 2),
                               "tflops_equiv": round(flops / (sum(times) / len(times)) / 1e12, 3),
-                              "scaled_residual": max(sres), "outer": infos[-1]["outer"],
+                              "scaled_residual": max(sres), "forward_error": max(fwd),
+                              "fp64_lu_same_matrix": fp64_errors,
+                              "outer": infos[-1]["outer"],
                               "gmres_steps": infos[-1]["inner"],
-                              "refinement_used": infos[-1]["refinement"],
                               "bwd_history": [float(f"{v:.3e}") for v in infos[-1]["history"]],
                               "kernel_ms_per_step": {k: round(v["ms"] / args.steps, 2)
                                                      for k, v in stats.items()}})
             if fp64_line is None and not args.no_fp64:
-                ft, fs, fstats = run_fp64(torch, h, AT, B0, args.steps, args.warmup)
+                ft, fs, fstats, ff = run_fp64(torch, h, AT, B0, args.steps, args.warmup, XT)
                 fms = 1e3 * sum(ft) / len(ft)
                 fp64_line = {"kappa": kappa, "ms_per_solve": round(fms, 2),
                              "tflops_equiv": round(flops / (fms / 1e3) / 1e12, 3),
-                             "scaled_residual": max(fs),
+                             "scaled_residual": max(fs), "forward_error": max(ff),
                              "kernel_ms_per_step": {k: round(v["ms"] / args.steps, 2)
                                                     for k, v in fstats.items()},
                              "gemm_roofline": roofline_from({"gemm": fstats["gemm"]},
                                                             args.steps, "fp64",
                                                             {"mode": "fp64", "n": n, "nb": nb})}
             del AT, B0
-            t_gen = time.perf_counter()
         ms = 1e3 * total_t / total_steps
         value = flops / (ms / 1e3) / 1e12
+        # the dominant kernel over every precision the solves ran
+        best = None
+        for gp, st in all_stats.items():
+            cls, v = max(st.items(), key=lambda kv: kv[1]["ms"])
+            if best is None or v["ms"] > best[2]:
+                best = (gp, cls, v["ms"])
+        dom_prec = best[0]
+        policy_desc = ("the learned selector (sla_dxgesv_auto: features + network + solve, "
+                       "FP64 fallback as the judge)" if args.policy == "auto" else
+                       f"fixed: factorization {args.prec}, IR (GMRES-IR from kappa "
+                       f"{args.gmres_from:g})")
         line.update({
             "value": round(value, 3), "unit": "TFLOPS", "ms_per_step": round(ms, 2),
-            "dtype": args.prec if args.prec != "fp32" else "f32",
+            "dtype": {"fp32": "f32", "fp64": "f64"}.get(dom_prec, dom_prec),
             "data": "synthetic: A = U diag(sigma) V^T, U,V = Q of seeded Gaussian N x N "
                     "(torch seed 100), sigma geometric 1..1/kappa; b = A x_true (seed 200); "
                     "in HBM",
             "config": {"workload": f"mixed-precision LU + refinement to FP64 backward error, "
-                                   f"N={n}, NRHS={nrhs}, NB={nb}, factorization {args.prec} "
-                                   f"(FP32 accumulation), kappa in {args.kappas}",
-                       "n": n, "nrhs": nrhs, "nb": nb, "precision": args.prec,
+                                   f"N={n}, NRHS={nrhs}, NB={nb}, kappa in {args.kappas}, "
+                                   f"policy: {policy_desc}",
+                       "n": n, "nrhs": nrhs, "nb": nb, "policy": args.policy,
+                       "precision": None if args.policy == "auto" else args.prec,
                        "kappas": kappas},
             "per_kappa": per_kappa,
             "fp64_lu": fp64_line,
             "speedup_vs_fp64_lu": (round(value / fp64_line["tflops_equiv"], 3)
                                    if fp64_line else None),
-            "roofline": roofline_from(all_stats, total_steps, args.prec,
-                                      {"mode": "mixed", "n": n, "nb": nb, "prec": args.prec}),
+            "roofline": roofline_from(all_stats[dom_prec], steps_by_prec[dom_prec], dom_prec,
+                                      {"mode": "mixed", "n": n, "nb": nb, "prec": dom_prec}),
         })
     line["cpu_baseline"] = None if args.no_cpu else cpu_baseline(args.cpu_n)
     print(json.dumps(line), flush=True)

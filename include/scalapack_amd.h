@@ -138,7 +138,13 @@ int sla_set_stream(sla_handle_t h, void *stream);
  * mixed-precision solve "ir_max_iter" (3..1000, default 30), "ir_gmres_on_stall"
  * (0/1, default 1), "ir_accept_stall" (0/1, default 1: a refinement that
  * stalls with scaled residual < 16 returns its X; 0: only DSGESV's
- * convergence test ends refinement, a stall falls back to FP64), "gmres_m" (GMRES steps per correction, default 50),
+ * convergence test ends refinement, a stall falls back to FP64), "ir_fallback"
+ * (0/1, default 1: a failed refinement is redone by the FP64 factorization;
+ * 0: the failure is reported with ITER < 0 and SLA_STOP_FAILED, A untouched --
+ * for callers timing the low-precision attempt alone), "trsv_blockinv" (0/1,
+ * default 1: solves with low-precision factors multiply by the inverses of
+ * their 128 x 128 diagonal blocks, formed once per factorization; 0:
+ * substitution), "gmres_m" (GMRES steps per correction, default 50),
  * "gmres_tol_exp" (e: relative reduction 10^-e per correction, default 6),
  * "pivoting" (FP32-storage factorizations: 0 partial pivoting, 1 tournament
  * pivoting of the panel leaves, default 1; the FP64 path always pivots as
@@ -162,6 +168,13 @@ const char *sla_last_error(void);
  * panel and TRSM.  ms = summed kernel time,
  * work = summed algorithmic flops (bytes for laswp), launches = count. */
 int sla_get_stats(sla_handle_t h, int cls, double *ms, double *work, long long *launches);
+/* The outer trailing-update launches of class 3 as the kernels themselves
+ * measure them while "profile" is 1 (no reference counterpart: a measurement
+ * aid): ms = summed execution intervals from the first workgroup's start to
+ * the last workgroup's end on the device wall clock (the stream's event pair
+ * also counts the wait for CUs held by the look-ahead panel stream),
+ * work = summed 2 m n k flops of the 128 x 128-tiled interiors. */
+int sla_get_update_exec(sla_handle_t h, double *ms, double *work, long long *launches);
 int sla_reset_stats(sla_handle_t h);
 
 /* FP64 LU of the m x n column-major matrix dA (leading dimension lda) in
@@ -250,11 +263,14 @@ typedef struct {
     int stop_rule;   /* what ended the refinement: SLA_STOP_CONVERGED (DSGESV's test),
                         SLA_STOP_STALL_ACCEPTED (a stall, PDGERFS's rule, with scaled
                         residual < 16; only while "ir_accept_stall" is 1) or
-                        SLA_STOP_FALLBACK (X from the FP64 factorization) */
+                        SLA_STOP_FALLBACK (X from the FP64 factorization),
+                        SLA_STOP_FAILED (refinement failed and the handle's
+                        "ir_fallback" is 0: X is the last iterate, A untouched) */
 } sla_solve_info_t;
 #define SLA_STOP_CONVERGED 0
 #define SLA_STOP_STALL_ACCEPTED 1
 #define SLA_STOP_FALLBACK 2
+#define SLA_STOP_FAILED 3
 
 /* X = A^{-1} B (n x n FP64 dA, nrhs <= 16, device pointers) by a
  * factorization in precision `prec` and refinement `refine` to FP64 backward
@@ -327,19 +343,23 @@ int sla_gesv_batched_tune(sla_handle_t h, int prec, int batch, const int *n, con
 /* ------------------------------------------------------------------------ */
 /* Part 5: learned precision selector (no counterpart in the reference).      */
 /* ------------------------------------------------------------------------ */
-/* A network 12 -> 64 -> 64 -> 4 (tanh, softmax; FP32 weights) picks the
- * factorization precision of sla_dxgesv -- class 0 FP8, 1 BF16, 2 FP32, 3 plain
- * FP64 -- from 12 features of A computed on the GPU (norms, row / column
- * scaling spread, sampled pivot growth and a PDGECON-style condition
- * estimate of sampled principal submatrices).  Parameters: W1[64][12], b1,
- * W2[64][64], b2, W3[4][64], b3, then the input normalisation mu[12],
- * sigma[12]: SLA_MLP_NPARAM floats. */
-#define SLA_MLP_NPARAM 5276
+/* A network 12 -> 64 -> 64 -> 7 (tanh, softmax; FP32 weights) picks the
+ * factorization precision and the refinement of sla_dxgesv -- class 0 FP8 +
+ * classical IR, 1 FP8 + GMRES-IR, 2 split BF16 (SLA_PREC_BF16X3) + IR, 3 split
+ * BF16 + GMRES-IR, 4 FP32 + IR, 5 FP32 + GMRES-IR, 6 plain FP64 LU -- from 12
+ * features of A computed on the GPU (norms, row / column scaling spread,
+ * sampled pivot growth and a PDGECON-style condition estimate of sampled
+ * principal submatrices, and the stable-rank fraction ||A||_F^2 / (n
+ * ||A||_2^2) from two power steps).  Parameters: W1[64][12], b1, W2[64][64],
+ * b2, W3[7][64], b3, then the input normalisation mu[12], sigma[12]:
+ * SLA_MLP_NPARAM floats. */
+#define SLA_SELECT_CLASSES 7
+#define SLA_MLP_NPARAM 5471
 /* The 12 features of an n x n FP64 device matrix into feat (host float[12]). */
 int sla_selector_features(sla_handle_t h, int n, const double *dA, int lda, float *feat);
 /* Mean cost-weighted cross-entropy over `batch` samples (dX device
- * float[batch][12] raw features, dy device int labels 0..3, dcost device
- * float[4] class weights or NULL) and its gradient with respect to the
+ * float[batch][12] raw features, dy device int labels 0..6, dcost device
+ * float[7] class weights or NULL) and its gradient with respect to the
  * trained parameters (the first SLA_MLP_NPARAM - 24 floats of dgrad). */
 int sla_mlp_grad(sla_handle_t h, int batch, const float *dX, const int *dy, const float *dcost,
                  const float *dparams, float *dgrad, float *loss);
@@ -348,13 +368,15 @@ int sla_mlp_grad(sla_handle_t h, int batch, const float *dX, const int *dy, cons
 int sla_mlp_train(sla_handle_t h, int batch, const float *dX, const int *dy, const float *dcost,
                   float *dparams, int epochs, float lr, float *loss_history);
 /* Classes (device int[batch], or NULL) and probabilities (device
- * float[batch][4], or NULL). */
+ * float[batch][7], or NULL). */
 int sla_mlp_predict(sla_handle_t h, int batch, const float *dX, const float *dparams, int *dpred,
                     float *dprob);
 /* Install the selector's parameters (host float[SLA_MLP_NPARAM]). */
 int sla_selector_set_model(sla_handle_t h, const float *params);
-/* sla_dxgesv with the precision the selector picks (*choice, 0..3); the
- * refinement and its FP64 fallback are sla_dxgesv's (SLA_REFINE_IR). */
+/* sla_dxgesv with the precision and refinement the selector picks (*choice,
+ * 0..6) from the features of dA; sla_dxgesv's FP64 fallback stays the judge
+ * of the result (a refinement that does not reach FP64 backward error is
+ * redone by the FP64 factorization). */
 int sla_dxgesv_auto(sla_handle_t h, int n, int nrhs, double *dA, int lda, int *dipiv,
                     const double *dB, int ldb, double *dX, int ldx, int *choice, int *iter,
                     int *info);

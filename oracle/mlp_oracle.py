@@ -1,11 +1,11 @@
 """CPU restatement of the precision selector's network (TEST INFRASTRUCTURE:
 only tests use it, as the checker of the HIP forward / backward kernels).
 
-12 -> 64 -> 64 -> 4, tanh hidden layers, softmax output, cost-weighted
+12 -> 64 -> 64 -> 7, tanh hidden layers, softmax output, cost-weighted
 cross-entropy averaged over the batch:
     loss = (1/B) sum_s cost[y_s] * -log softmax(z_s)[y_s]
-Parameter vector layout: W1[64][12], b1[64], W2[64][64], b2[64], W3[4][64],
-b3[4], mu[12], sigma[12] (inputs are (f - mu) / sigma).  Evaluated in
+Parameter vector layout: W1[64][12], b1[64], W2[64][64], b2[64], W3[7][64],
+b3[7], mu[12], sigma[12] (inputs are (f - mu) / sigma).  Evaluated in
 float64 so that it can check the float32 kernels and itself be pinned by
 finite differences.
 """
@@ -13,7 +13,7 @@ from __future__ import annotations
 
 import numpy as np
 
-IN, H, OUT = 12, 64, 4
+IN, H, OUT = 12, 64, 7
 SIZES = [("W1", (H, IN)), ("b1", (H,)), ("W2", (H, H)), ("b2", (H,)), ("W3", (OUT, H)),
          ("b3", (OUT,)), ("mu", (IN,)), ("sd", (IN,))]
 NPARAM = sum(int(np.prod(s)) for _, s in SIZES)

@@ -59,6 +59,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "sla_set_param": [vp, ctypes.c_char_p, i],
         "sla_get_param": [vp, ctypes.c_char_p, ip],
         "sla_get_stats": [vp, i, vp, vp, vp],
+        "sla_get_update_exec": [vp, vp, vp, vp],
         "sla_reset_stats": [vp],
         "sla_dgetrf": [vp, i, i, vp, i, vp, vp],
         "sla_dgetrs": [vp, i, i, vp, i, vp, vp, i],

@@ -364,6 +364,53 @@ __global__ __launch_bounds__(256) void rfs_reduce_kernel(int n, int slices, cons
 }
 
 // w = |r| + nz eps w (+ safe1 where w <= safe2), pdgerfs.f:561-572.
+// y[i] = sum over slices of part[slice][0][i] (fixed order): the product of
+// rfs_partial_n_kernel / rfs_partial_t_kernel without a right-hand side.
+__global__ __launch_bounds__(256) void slice_sum_kernel(int n, int slices,
+                                                        const double *__restrict__ part,
+                                                        double *__restrict__ y)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int c = 0; c < slices; c++) s += part[(size_t)c * 2 * n + i];
+    y[i] = s;
+}
+
+// x[i] in [-1, 1): an integer hash of i (the power steps' fixed start).
+__global__ void hash_vector_kernel(int n, double *x)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    uint32_t v = (uint32_t)i * 2654435761u + 0x9E3779B9u;
+    v ^= v >> 16;
+    v *= 0x7feb352du;
+    v ^= v >> 15;
+    v *= 0x846ca68bu;
+    v ^= v >> 16;
+    x[i] = (double)v / 2147483648.0 - 1.0;
+}
+
+// Stage 1 of x . y (fixed order, then reduce2_kernel<RED_SUM>).
+__global__ __launch_bounds__(256) void dot1_kernel(int n, const double *__restrict__ x,
+                                                   const double *__restrict__ y,
+                                                   double *__restrict__ part)
+{
+    __shared__ double ws[4];
+    double v = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) v = fma(x[i], y[i], v);
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = ((ws[0] + ws[1]) + ws[2]) + ws[3];
+}
+
+__global__ void scal_kernel(int n, double alpha, double *x)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) x[i] *= alpha;
+}
+
 __global__ void ferr_weights_kernel(int n, const double *__restrict__ r, double *__restrict__ w,
                                     double nzeps, double safe1, double safe2)
 {
@@ -1046,6 +1093,52 @@ namespace {
 double lg(double x) { return std::log10(std::min(std::max(x, 1e-300), 1e300)); }
 }  // namespace
 
+// Rayleigh estimate of ||A||_2^2 from two power steps on A^T A started at the
+// hash vector v: z1 = A^T A v scaled to max |z1| = 1, z2 = A^T A z1,
+// lambda = (z1 . z2) / (z1 . z1).  Four passes over A.
+static double power_lambda(Handle &h, int n, const double *dA, int lda)
+{
+    const int sn = ceil_div(n, RF_KC), stn = ceil_div(n, RF_TROWS);
+    AuxWork a(h, n, (size_t)std::max(sn, stn) * 2 * n);
+    hipStream_t s = h.stream;
+    auto mv = [&](bool trans, const double *in, double *out) {
+        if (!trans) {
+            hipLaunchKernelGGL(rfs_partial_n_kernel, dim3(ceil_div(n, RF_ROWS), sn), dim3(256), 0, s, n,
+                               dA, lda, in, a.part);
+            hipLaunchKernelGGL(slice_sum_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, sn, a.part,
+                               out);
+        } else {
+            hipLaunchKernelGGL(rfs_partial_t_kernel, dim3(ceil_div(n, RF_TCOLS), stn), dim3(256), 0, s,
+                               n, dA, lda, in, a.part);
+            hipLaunchKernelGGL(slice_sum_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, stn,
+                               a.part, out);
+        }
+        SLA_LAUNCH_CHECK();
+    };
+    auto dot = [&](const double *x, const double *y, int slot) {
+        const int nb = a.red_blocks(n);
+        hipLaunchKernelGGL(dot1_kernel, dim3(nb), dim3(256), 0, s, n, x, y, a.red_v);
+        hipLaunchKernelGGL(reduce2_kernel<RED_SUM>, dim3(1), dim3(256), 0, s, nb, a.red_v, 1.0,
+                           a.res + slot);
+        SLA_LAUNCH_CHECK();
+    };
+    hipLaunchKernelGGL(hash_vector_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, a.x);
+    mv(false, a.x, a.v);
+    mv(true, a.v, a.r);  // z1
+    a.maxabs(a.r, n, 0);
+    double zmax = 0.0;
+    a.fetch(&zmax, a.res, 8);
+    if (!(zmax > 0.0) || !std::isfinite(zmax)) return 0.0;
+    hipLaunchKernelGGL(scal_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, 1.0 / zmax, a.r);
+    mv(false, a.r, a.w);
+    mv(true, a.w, a.x);  // z2
+    dot(a.r, a.x, 0);
+    dot(a.r, a.r, 1);
+    double d[2];
+    a.fetch(d, a.res, sizeof d);
+    return d[1] > 0.0 ? d[0] / d[1] : 0.0;
+}
+
 void selector_features_device(Handle &h, int n, const double *dA, int lda, float *f)
 {
     if (n < 1) throw std::invalid_argument("selector features: n >= 1");
@@ -1147,7 +1240,11 @@ void selector_features_device(Handle &h, int n, const double *dA, int lda, float
     f[8] = (float)d_max;
     f[9] = (float)c_mean;
     f[10] = (float)c_max;
-    f[11] = (float)std::log10((double)n / (double)k);
+    // the stable-rank fraction ||A||_F^2 / (n ||A||_2^2): a global view of
+    // how the singular values spread, which the sampled submatrices of a
+    // large matrix cannot see
+    const double lam = power_lambda(h, n, dA, lda);
+    f[11] = (float)(lam > 0.0 ? std::min(20.0, std::max(-20.0, lg(nf * nf / ((double)n * lam)))) : -20.0);
 }
 
 }  // namespace sla

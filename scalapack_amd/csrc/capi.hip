@@ -982,7 +982,8 @@ int sla_set_param(sla_handle_t h, const char *name, int value)
                                   "ir_max_iter", "gmres_m", "ir_gmres_on_stall", "gmres_tol_exp",
                                   "pivoting", "lookahead", "update_cus", "lacon_lapack",
                                   "batched_w", "batched_wgs_per_cu", "batched_graph",
-                                  "update_lds_kb", "ir_accept_stall"};
+                                  "update_lds_kb", "ir_accept_stall", "trsv_blockinv",
+                                  "ir_fallback"};
     if (std::find_if(std::begin(known), std::end(known), [&](const char *x) { return k == x; }) ==
         std::end(known)) {
         g_last_error = "sla_set_param: unknown parameter " + k;
@@ -999,6 +1000,8 @@ int sla_set_param(sla_handle_t h, const char *name, int value)
     else if (k == "gmres_m" && value >= 1 && value <= 1000) H->gmres_m = value;
     else if (k == "ir_gmres_on_stall") H->ir_gmres_on_stall = value != 0;
     else if (k == "ir_accept_stall") H->ir_accept_stall = value != 0;
+    else if (k == "trsv_blockinv" && (value == 0 || value == 1)) H->trsv_blockinv = value;
+    else if (k == "ir_fallback" && (value == 0 || value == 1)) H->ir_fallback = value;
     else if (k == "pivoting" && (value == 0 || value == 1)) H->pivoting = value;
     else if (k == "lookahead" && (value == 0 || value == 1)) H->lookahead = value;
     else if (k == "update_cus" && value >= 0 && value <= H->num_cus) H->update_cus = value;
@@ -1028,6 +1031,8 @@ int sla_get_param(sla_handle_t h, const char *name, int *value)
     else if (k == "gmres_m") *value = H->gmres_m;
     else if (k == "ir_gmres_on_stall") *value = H->ir_gmres_on_stall;
     else if (k == "ir_accept_stall") *value = H->ir_accept_stall;
+    else if (k == "trsv_blockinv") *value = H->trsv_blockinv;
+    else if (k == "ir_fallback") *value = H->ir_fallback;
     else if (k == "pivoting") *value = H->pivoting;
     else if (k == "lookahead") *value = H->lookahead;
     else if (k == "update_cus") *value = H->update_cus;
@@ -1055,6 +1060,20 @@ int sla_get_stats(sla_handle_t h, int cls, double *ms, double *work, long long *
     *ms = H->stats[cls].ms;
     *work = H->stats[cls].work;
     *launches = H->stats[cls].launches;
+    return SLA_OK;
+}
+
+int sla_get_update_exec(sla_handle_t h, double *ms, double *work, long long *launches)
+{
+    SLA_NEED_HANDLE(h);
+    if (!ms || !work || !launches) {
+        g_last_error = "sla_get_update_exec: null output";
+        return SLA_ERR_ARG;
+    }
+    auto *H = reinterpret_cast<sla::Handle *>(h);
+    *ms = H->exec.ms;
+    *work = H->exec.work;
+    *launches = H->exec.launches;
     return SLA_OK;
 }
 
@@ -1341,15 +1360,20 @@ int sla_dxgesv(sla_handle_t h, int prec, int refine, int n, int nrhs, double *dA
 // ---------------------------------------------------------------- selector
 static_assert(sla::MLP_NPARAM == SLA_MLP_NPARAM, "parameter count of the selector network");
 
-// The selector's class -> factorization precision of sla_dxgesv.
+// The selector's class -> factorization precision and refinement of
+// sla_dxgesv: {FP8, split BF16, FP32} x {classical IR, GMRES-IR}, then FP64.
+static_assert(sla::MLP_OUT == SLA_SELECT_CLASSES, "classes of the selector network");
 static int selector_precision(int choice)
 {
-    switch (choice) {
-    case 0: return SLA_PREC_FP8;
-    case 1: return SLA_PREC_BF16;
-    case 2: return SLA_PREC_FP32;
-    default: return SLA_PREC_FP64;
-    }
+    static const int prec[SLA_SELECT_CLASSES] = {SLA_PREC_FP8,   SLA_PREC_FP8,  SLA_PREC_BF16X3,
+                                                 SLA_PREC_BF16X3, SLA_PREC_FP32, SLA_PREC_FP32,
+                                                 SLA_PREC_FP64};
+    return (choice >= 0 && choice < SLA_SELECT_CLASSES) ? prec[choice] : SLA_PREC_FP64;
+}
+static int selector_refinement(int choice)
+{
+    return (choice >= 0 && choice < SLA_SELECT_CLASSES - 1 && choice % 2 == 1) ? SLA_REFINE_GMRES
+                                                                               : SLA_REFINE_IR;
 }
 
 int sla_selector_features(sla_handle_t h, int n, const double *dA, int lda, float *feat)
@@ -1442,8 +1466,8 @@ int sla_dxgesv_auto(sla_handle_t h, int n, int nrhs, double *dA, int lda, int *d
         sla::mlp_predict_device(H, 1, df, d, dc, nullptr);
         SLA_HIP(hipMemcpy(choice, dc, sizeof(int), hipMemcpyDeviceToHost));
         sla::MixedInfo mi;
-        *info = sla::dxgesv_device(H, selector_precision(*choice), SLA_REFINE_IR, n, nrhs, dA, lda,
-                                   dipiv, dB, ldb, dX, ldx, mi);
+        *info = sla::dxgesv_device(H, selector_precision(*choice), selector_refinement(*choice), n,
+                                   nrhs, dA, lda, dipiv, dB, ldb, dX, ldx, mi);
         *iter = mi.iter;
         H.last_mixed = mi;
     })

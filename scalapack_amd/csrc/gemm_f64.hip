@@ -181,8 +181,10 @@ constexpr int DEPTH = 3;
 template <int TRAILING>
 __global__ __launch_bounds__(256, 1) void dgemm_nn_sub_direct_kernel(
     int m, int n, int k, const double *__restrict__ A, int lda,
-    const double *__restrict__ B, int ldb, double *__restrict__ C, int ldc)
+    const double *__restrict__ B, int ldb, double *__restrict__ C, int ldc,
+    unsigned long long *stamp)
 {
+    launch_stamp_begin(stamp);
     const int tiles_m = m / 128, tiles_n = n / 128;
     for (int tile = blockIdx.x; tile < tiles_m * tiles_n; tile += gridDim.x) {
     int tmi, tni;
@@ -244,6 +246,7 @@ __global__ __launch_bounds__(256, 1) void dgemm_nn_sub_direct_kernel(
         }
     }
     }  // tile loop
+    launch_stamp_end(stamp);
 }
 
 // C[m x n] -= A[m x k] * B[k x n] for n <= 16 (the right-hand sides of the
@@ -279,7 +282,8 @@ __global__ __launch_bounds__(256) void dgemm_nn_sub_skinny_kernel(
 }  // namespace
 
 void dgemm_nn_sub(int m, int n, int k, const double *A, int lda, const double *B, int ldb,
-                  double *C, int ldc, hipStream_t s, bool trailing, int max_wgs)
+                  double *C, int ldc, hipStream_t s, bool trailing, int max_wgs,
+                  unsigned long long *stamp)
 {
     if (m <= 0 || n <= 0 || k <= 0) return;
     if (!trailing && gemm_small<double>(m, n, k, A, lda, B, ldb, C, ldc, s)) return;
@@ -304,10 +308,10 @@ void dgemm_nn_sub(int m, int n, int k, const double *A, int lda, const double *B
         const int grid = (max_wgs >= 8 && tiles > max_wgs) ? (max_wgs / 8) * 8 : tiles;
         if (trailing)
             hipLaunchKernelGGL(dgemm_nn_sub_direct_kernel<1>, dim3(grid), dim3(256), 0, s, mf, nf,
-                               k, A, lda, B, ldb, C, ldc);
+                               k, A, lda, B, ldb, C, ldc, stamp);
         else
             hipLaunchKernelGGL(dgemm_nn_sub_direct_kernel<0>, dim3(tiles), dim3(256), 0, s, mf, nf,
-                               k, A, lda, B, ldb, C, ldc);
+                               k, A, lda, B, ldb, C, ldc, (unsigned long long *)nullptr);
         SLA_LAUNCH_CHECK();
         // ragged edges: the rows below the interior (all columns) and the
         // columns right of it

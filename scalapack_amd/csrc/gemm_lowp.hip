@@ -28,6 +28,7 @@
 #include <hip/hip_bf16.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <type_traits>
 
 namespace sla {
@@ -99,8 +100,9 @@ constexpr int SDEPTH = 3;
 template <int TRAILING>
 __global__ __launch_bounds__(256, 2) void sgemm_nn_sub_direct_kernel(
     int m, int n, int k, const float *__restrict__ A, int lda, const float *__restrict__ B,
-    int ldb, float *__restrict__ C, int ldc)
+    int ldb, float *__restrict__ C, int ldc, unsigned long long *stamp)
 {
+    launch_stamp_begin(stamp);
     // persistent when the grid is smaller than the tile count (see
     // dgemm_nn_sub_direct_kernel)
     for (int tile = blockIdx.x; tile < (m / 128) * (n / 128); tile += gridDim.x) {
@@ -161,6 +163,7 @@ __global__ __launch_bounds__(256, 2) void sgemm_nn_sub_direct_kernel(
         }
     }
     }  // tile loop
+    launch_stamp_end(stamp);
 }
 
 // Ragged edges (and small shapes) of either precision: one thread per element
@@ -194,8 +197,9 @@ constexpr int BDEPTH = 3;
 
 __global__ __launch_bounds__(256, 2) void gemm_bf16_nn_sub_direct_kernel(
     int m, int n, int k, const __bf16 *__restrict__ Ap, int ldak, const __bf16 *__restrict__ Bp,
-    int ldbk, float *__restrict__ C, int ldc)
+    int ldbk, float *__restrict__ C, int ldc, unsigned long long *stamp)
 {
+    launch_stamp_begin(stamp);
     int tmi, tni;
     tile_coords_lp(blockIdx.x, m / 128, n / 128, tmi, tni);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -251,6 +255,7 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16_nn_sub_direct_kernel(
             *cp = c;
         }
     }
+    launch_stamp_end(stamp);
 }
 
 // ---------------------------------------------------------------------------
@@ -262,8 +267,10 @@ constexpr int FDEPTH = 2;
 
 __global__ __launch_bounds__(256, 2) void gemm_fp8_nn_sub_direct_kernel(
     int m, int n, int k, const uint8_t *__restrict__ Ap, int ldak, const uint8_t *__restrict__ Bp,
-    int ldbk, const float *__restrict__ scales, float *__restrict__ C, int ldc)
+    int ldbk, const float *__restrict__ scales, float *__restrict__ C, int ldc,
+    unsigned long long *stamp)
 {
+    launch_stamp_begin(stamp);
     int tmi, tni;
     tile_coords_lp(blockIdx.x, m / 128, n / 128, tmi, tni);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -319,6 +326,7 @@ __global__ __launch_bounds__(256, 2) void gemm_fp8_nn_sub_direct_kernel(
             *cp = c;
         }
     }
+    launch_stamp_end(stamp);
 }
 
 // ---------------------------------------------------------------------------
@@ -378,8 +386,9 @@ template <typename T16, int S>
 __global__ __launch_bounds__(256, S == 2 ? 2 : 1) void gemm_split_nn_sub_direct_kernel(
     int m, int n, int k, const T16 *__restrict__ Ap, int ldak, size_t aplane,
     const T16 *__restrict__ Bp, int ldbk, size_t bplane, const float *__restrict__ absmax,
-    float *__restrict__ C, int ldc)
+    float *__restrict__ C, int ldc, unsigned long long *stamp)
 {
+    launch_stamp_begin(stamp);
     int tmi, tni;
     tile_coords_lp(blockIdx.x, m / 128, n / 128, tmi, tni);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -448,6 +457,124 @@ __global__ __launch_bounds__(256, S == 2 ? 2 : 1) void gemm_split_nn_sub_direct_
             *cp = c;
         }
     }
+    launch_stamp_end(stamp);
+}
+
+// The split product for large tiles: a 256 x 128 tile of C per 256-thread
+// workgroup, the operand slices of each 32-deep K step staged once in LDS
+// and shared by the four waves (wave w: rows 128 (w & 1) .., columns
+// 64 (w >> 1) ..: 8 x 4 accumulators of 16 x 16), the next K step's slices
+// loaded into registers while the MFMAs of this one run, LDS double
+// buffered (one workgroup barrier per K step).  LDS rows hold 32 k values
+// plus 8 of padding (80 bytes), so the 16 lanes that read one 16-byte
+// fragment each cover distinct banks.  Two slices (S = 2) only: the
+// double-buffered tiles take 120 KB of LDS.
+constexpr int SL_M = 256, SL_N = 128, SL_K = 32, SL_LD = SL_K + 8;
+constexpr size_t split_lds_bytes(int S)
+{
+    return (size_t)2 * S * (SL_M + SL_N) * SL_LD * 2;
+}
+
+template <typename T16, int S>
+__global__ __launch_bounds__(256, 1) void gemm_split_lds_kernel(
+    int m, int n, int k, const T16 *__restrict__ Ap, int ldak, size_t aplane,
+    const T16 *__restrict__ Bp, int ldbk, size_t bplane, const float *__restrict__ absmax,
+    float *__restrict__ C, int ldc, unsigned long long *stamp)
+{
+    static_assert(S == 2, "the LDS tiles are sized for two slices");
+    extern __shared__ __align__(16) unsigned char sl_lds[];
+    // [buf][slice][row][SL_LD] of A, then [buf][slice][col][SL_LD] of B
+    T16 *As = reinterpret_cast<T16 *>(sl_lds);
+    T16 *Bs = As + (size_t)2 * S * SL_M * SL_LD;
+    auto a_at = [&](int buf, int sl, int row) { return As + (((size_t)buf * S + sl) * SL_M + row) * SL_LD; };
+    auto b_at = [&](int buf, int sl, int col) { return Bs + (((size_t)buf * S + sl) * SL_N + col) * SL_LD; };
+
+    launch_stamp_begin(stamp);
+    const int tiles_m = m / SL_M, tiles_n = n / SL_N;
+    int tmi, tni;
+    tile_coords_lp(blockIdx.x, tiles_m, tiles_n, tmi, tni);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int row0 = tmi * SL_M, col0 = tni * SL_N;
+    const int wm = (wave & 1) * 128, wn = (wave >> 1) * 64;
+    const int lr = lane & 15, lk = lane >> 4;
+
+    // global -> register staging: A rows tid / 4 + 64 i (i < 4), B columns
+    // tid / 4 + 64 i (i < 2), 16-byte chunk tid % 4 of the 32 k values
+    const int srow = tid >> 2, sch = (tid & 3) * 8;
+    s8_t ra[S][4], rb[S][2];
+    auto gload = [&](int t) {
+        const size_t k0 = (size_t)SL_K * t + sch;
+#pragma unroll
+        for (int sl = 0; sl < S; sl++) {
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                ra[sl][i] = *(const s8_t *)(Ap + sl * aplane + (size_t)(row0 + srow + 64 * i) * ldak + k0);
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+                rb[sl][i] = *(const s8_t *)(Bp + sl * bplane + (size_t)(col0 + srow + 64 * i) * ldbk + k0);
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int sl = 0; sl < S; sl++) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) *(s8_t *)(a_at(buf, sl, srow + 64 * i) + sch) = ra[sl][i];
+#pragma unroll
+            for (int i = 0; i < 2; i++) *(s8_t *)(b_at(buf, sl, srow + 64 * i) + sch) = rb[sl][i];
+        }
+    };
+
+    f4_t acc[8][4];
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
+
+    const int nk = k / SL_K;
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    for (int t = 0; t < nk; t++) {
+        const int buf = t & 1;
+        if (t + 1 < nk) gload(t + 1);
+        s8_t fb[S][4];
+#pragma unroll
+        for (int sl = 0; sl < S; sl++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) fb[sl][j] = *(const s8_t *)(b_at(buf, sl, wn + 16 * j + lr) + 8 * lk);
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            s8_t fa[S];
+#pragma unroll
+            for (int sl = 0; sl < S; sl++) fa[sl] = *(const s8_t *)(a_at(buf, sl, wm + 16 * i + lr) + 8 * lk);
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                f4_t c = acc[i][j];
+                c = mfma16<T16>(fa[1], fb[0][j], c);
+                c = mfma16<T16>(fa[0], fb[1][j], c);
+                acc[i][j] = mfma16<T16>(fa[0], fb[0][j], c);
+            }
+        }
+        if (t + 1 < nk) lstore(buf ^ 1);
+        __syncthreads();
+    }
+
+    const float back = ldexpf(1.0f, -(split_exp<T16>(absmax[0]) + split_exp<T16>(absmax[1])));
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        float *Cc = C + (size_t)(col0 + wn + 16 * j + lr) * ldc + row0 + wm;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            f4_t *cp = (f4_t *)(Cc + 16 * i + 4 * lk);
+            f4_t c = *cp;
+            c[0] -= back * acc[i][j][0];
+            c[1] -= back * acc[i][j][1];
+            c[2] -= back * acc[i][j][2];
+            c[3] -= back * acc[i][j][3];
+            *cp = c;
+        }
+    }
+    launch_stamp_end(stamp);
 }
 
 // Ragged edges of the split form: one thread per element of C, the same
@@ -609,7 +736,8 @@ __global__ void convert_f64_f32_kernel(int m, int n, const double *__restrict__ 
 static bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
 
 void sgemm_nn_sub(int m, int n, int k, const float *A, int lda, const float *B, int ldb, float *C,
-                  int ldc, hipStream_t s, bool trailing, int max_wgs, int reserve_lds)
+                  int ldc, hipStream_t s, bool trailing, int max_wgs, int reserve_lds,
+                  unsigned long long *stamp)
 {
     if (m <= 0 || n <= 0 || k <= 0) return;
     if (!trailing && gemm_small<float>(m, n, k, A, lda, B, ldb, C, ldc, s)) return;
@@ -633,11 +761,12 @@ void sgemm_nn_sub(int m, int n, int k, const float *A, int lda, const float *B, 
             }
             hipLaunchKernelGGL(sgemm_nn_sub_direct_kernel<1>, dim3(grid), dim3(256),
                                (unsigned)std::max(0, reserve_lds), s, mf, nf, k, A, lda, B, ldb, C,
-                               ldc);
+                               ldc, stamp);
         }
         else
             hipLaunchKernelGGL(sgemm_nn_sub_direct_kernel<0>, dim3((mf / 128) * (nf / 128)),
-                               dim3(256), 0, s, mf, nf, k, A, lda, B, ldb, C, ldc);
+                               dim3(256), 0, s, mf, nf, k, A, lda, B, ldb, C, ldc,
+                               (unsigned long long *)nullptr);
         SLA_LAUNCH_CHECK();
         if (m > mf)
             sgemm_nn_sub(m - mf, n, k, A + mf, lda, B, ldb, C + mf, ldc, s, trailing, max_wgs,
@@ -653,14 +782,14 @@ void sgemm_nn_sub(int m, int n, int k, const float *A, int lda, const float *B, 
 }
 
 void gemm_bf16_nn_sub(int m, int n, int k, const __bf16 *Ap, int ldak, const __bf16 *Bp, int ldbk,
-                      float *C, int ldc, hipStream_t s)
+                      float *C, int ldc, hipStream_t s, unsigned long long *stamp)
 {
     if (m <= 0 || n <= 0 || k <= 0) return;
     const int mf = (m / 128) * 128, nf = (n / 128) * 128;
     if (mf > 0 && nf > 0 && k % 32 == 0 && ldak % 8 == 0 && ldbk % 8 == 0 && ldc % 4 == 0 &&
         aligned16(Ap) && aligned16(Bp) && aligned16(C)) {
         hipLaunchKernelGGL(gemm_bf16_nn_sub_direct_kernel, dim3((mf / 128) * (nf / 128)), dim3(256),
-                           0, s, mf, nf, k, Ap, ldak, Bp, ldbk, C, ldc);
+                           0, s, mf, nf, k, Ap, ldak, Bp, ldbk, C, ldc, stamp);
         SLA_LAUNCH_CHECK();
         if (m > mf)
             gemm_bf16_nn_sub(m - mf, n, k, Ap + (size_t)mf * ldak, ldak, Bp, ldbk, C + mf, ldc, s);
@@ -728,14 +857,15 @@ void absmax_scale(int m, int n, const float *A, int lda, float *scale, hipStream
 }
 
 void gemm_fp8_nn_sub(int m, int n, int k, const uint8_t *Ap, int ldak, const uint8_t *Bp,
-                     int ldbk, const float *scales, float *C, int ldc, hipStream_t s)
+                     int ldbk, const float *scales, float *C, int ldc, hipStream_t s,
+                     unsigned long long *stamp)
 {
     if (m <= 0 || n <= 0 || k <= 0) return;
     const int mf = (m / 128) * 128, nf = (n / 128) * 128;
     if (mf > 0 && nf > 0 && k % 128 == 0 && ldak % 32 == 0 && ldbk % 32 == 0 && ldc % 4 == 0 &&
         aligned16(Ap) && aligned16(Bp) && aligned16(C)) {
         hipLaunchKernelGGL(gemm_fp8_nn_sub_direct_kernel, dim3((mf / 128) * (nf / 128)), dim3(256),
-                           0, s, mf, nf, k, Ap, ldak, Bp, ldbk, scales, C, ldc);
+                           0, s, mf, nf, k, Ap, ldak, Bp, ldbk, scales, C, ldc, stamp);
         SLA_LAUNCH_CHECK();
         if (m > mf)
             gemm_fp8_nn_sub(m - mf, n, k, Ap + (size_t)mf * ldak, ldak, Bp, ldbk, scales, C + mf,
@@ -750,18 +880,54 @@ void gemm_fp8_nn_sub(int m, int n, int k, const uint8_t *Ap, int ldak, const uin
     SLA_LAUNCH_CHECK();
 }
 
+// SLA_SPLIT_LDS=0 in the environment keeps the split product on the
+// LDS-free 128 x 128 kernel (for comparisons).
+static bool split_lds_on()
+{
+    static const bool on = [] {
+        const char *e = getenv("SLA_SPLIT_LDS");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
 template <typename T16, int S>
 static void gemm_split_launch(int m, int n, int k, const T16 *Ap, int ldak, size_t aplane,
                               const T16 *Bp, int ldbk, size_t bplane, const float *absmax,
-                              float *C, int ldc, hipStream_t s)
+                              float *C, int ldc, hipStream_t s,
+                              unsigned long long *stamp = nullptr)
 {
     if (m <= 0 || n <= 0 || k <= 0) return;
+    const bool vec = k % 32 == 0 && ldak % 8 == 0 && ldbk % 8 == 0 && ldc % 4 == 0 &&
+                     aplane % 8 == 0 && bplane % 8 == 0 && aligned16(Ap) && aligned16(Bp) &&
+                     aligned16(C);
+    const int ml = (m / SL_M) * SL_M, nl = (n / SL_N) * SL_N;
+    if (S == 2 && split_lds_on() && vec && ml > 0 && nl > 0) {
+        // 256 x 128 tiles through LDS; the borders by the kernels below
+        static bool attr = false;
+        if (!attr) {
+            SLA_HIP(hipFuncSetAttribute(
+                reinterpret_cast<const void *>(&gemm_split_lds_kernel<T16, 2>),
+                hipFuncAttributeMaxDynamicSharedMemorySize, (int)split_lds_bytes(2)));
+            attr = true;
+        }
+        hipLaunchKernelGGL((gemm_split_lds_kernel<T16, 2>), dim3((ml / SL_M) * (nl / SL_N)),
+                           dim3(256), split_lds_bytes(2), s, ml, nl, k, Ap, ldak, aplane, Bp, ldbk,
+                           bplane, absmax, C, ldc, stamp);
+        SLA_LAUNCH_CHECK();
+        if (m > ml)
+            gemm_split_launch<T16, S>(m - ml, n, k, Ap + (size_t)ml * ldak, ldak, aplane, Bp, ldbk,
+                                      bplane, absmax, C + ml, ldc, s);
+        if (n > nl)
+            gemm_split_launch<T16, S>(ml, n - nl, k, Ap, ldak, aplane, Bp + (size_t)nl * ldbk, ldbk,
+                                      bplane, absmax, C + (size_t)nl * ldc, ldc, s);
+        return;
+    }
     const int mf = (m / 128) * 128, nf = (n / 128) * 128;
-    if (mf > 0 && nf > 0 && k % 32 == 0 && ldak % 8 == 0 && ldbk % 8 == 0 && ldc % 4 == 0 &&
-        aplane % 8 == 0 && bplane % 8 == 0 && aligned16(Ap) && aligned16(Bp) && aligned16(C)) {
+    if (mf > 0 && nf > 0 && vec) {
         hipLaunchKernelGGL((gemm_split_nn_sub_direct_kernel<T16, S>), dim3((mf / 128) * (nf / 128)),
                            dim3(256), 0, s, mf, nf, k, Ap, ldak, aplane, Bp, ldbk, bplane, absmax,
-                           C, ldc);
+                           C, ldc, stamp);
         SLA_LAUNCH_CHECK();
         if (m > mf)
             gemm_split_launch<T16, S>(m - mf, n, k, Ap + (size_t)mf * ldak, ldak, aplane, Bp, ldbk,
@@ -779,23 +945,23 @@ static void gemm_split_launch(int m, int n, int k, const T16 *Ap, int ldak, size
 
 void gemm_split_nn_sub(int split, int m, int n, int k, const void *Ap, int ldak, size_t aplane,
                        const void *Bp, int ldbk, size_t bplane, const float *absmax, float *C,
-                       int ldc, hipStream_t s)
+                       int ldc, hipStream_t s, unsigned long long *stamp)
 {
     switch (split) {
     case SPLIT_BF16X3:
         gemm_split_launch<__bf16, 2>(m, n, k, static_cast<const __bf16 *>(Ap), ldak, aplane,
                                      static_cast<const __bf16 *>(Bp), ldbk, bplane, absmax, C,
-                                     ldc, s);
+                                     ldc, s, stamp);
         break;
     case SPLIT_BF16X6:
         gemm_split_launch<__bf16, 3>(m, n, k, static_cast<const __bf16 *>(Ap), ldak, aplane,
                                      static_cast<const __bf16 *>(Bp), ldbk, bplane, absmax, C,
-                                     ldc, s);
+                                     ldc, s, stamp);
         break;
     case SPLIT_FP16X3:
         gemm_split_launch<_Float16, 2>(m, n, k, static_cast<const _Float16 *>(Ap), ldak, aplane,
                                        static_cast<const _Float16 *>(Bp), ldbk, bplane, absmax, C,
-                                       ldc, s);
+                                       ldc, s, stamp);
         break;
     default: throw std::invalid_argument("split GEMM: unknown split");
     }

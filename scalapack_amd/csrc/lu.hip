@@ -58,6 +58,7 @@ Handle::Handle()
     if (hipDeviceGetAttribute(&clk_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess ||
         clk_khz <= 0)
         clk_khz = 100000;
+    wall_khz = clk_khz;
     // A device-wide barrier or dependency wait gives up after ~4 s.
     spin_limit = (uint64_t)clk_khz * 1000ull * 4ull;
 
@@ -80,6 +81,8 @@ Handle::Handle()
     pair_counts = alloc_dev<int>(2);
 
     fill_int(ctl, 64, 0, stream);
+    kstamps = alloc_dev<unsigned long long>(2 * KSTAMP_CAP);
+    reset_kstamps(kstamps, KSTAMP_CAP, stream);
     SLA_HIP(hipStreamSynchronize(stream));
 }
 
@@ -89,7 +92,7 @@ Handle::~Handle()
     batched_plan_free(batched);
     for (void *p : {(void *)ctl, (void *)slot_cand_v, (void *)slot_cand_r, (void *)slot_rows,
                     (void *)slot_rowj, (void *)pairs, (void *)pair_counts,
-                    (void *)perm, (void *)work})
+                    (void *)perm, (void *)work, (void *)kstamps})
         if (p) hipFree(p);
     for (auto &b : bufs)
         if (b.p) hipFree(b.p);
@@ -122,7 +125,9 @@ PanelWork<T> Handle::panel_work()
 TrsvWork Handle::trsv_work(int n)
 {
     TrsvWork tw;
-    const size_t words = (size_t)ceil_div(std::max(n, 1), 64) * 64 * 16 * 2;
+    // 16 right-hand sides x 2 words per row, rows rounded up to the larger
+    // block height of the two solvers (64 / 128)
+    const size_t words = (size_t)ceil_div(std::max(n, 1), 128) * 128 * 16 * 2;
     const bool grow = words * sizeof(uint64_t) > bufs.at(BUF_TRSV_MAIL).cap;
     tw.mail = static_cast<uint64_t *>(buffer(BUF_TRSV_MAIL, words * sizeof(uint64_t)));
     if (grow) {
@@ -204,11 +209,33 @@ void Handle::timing_end(double work_units, hipStream_t s)
     }
 }
 
+unsigned long long *Handle::next_kstamp(double w)
+{
+    if (!profile || kstamp_used >= KSTAMP_CAP) return nullptr;
+    kstamp_work.push_back(w);
+    return kstamps + 2 * kstamp_used++;
+}
+
 void Handle::timing_collect()
 {
     if (!profile) return;
     SLA_HIP(hipStreamSynchronize(stream));
     SLA_HIP(hipStreamSynchronize(stream2));
+    if (kstamp_used > 0) {
+        std::vector<unsigned long long> t(2 * (size_t)kstamp_used);
+        SLA_HIP(hipMemcpy(t.data(), kstamps, t.size() * sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost));
+        for (int i = 0; i < kstamp_used; i++) {
+            if (t[2 * i + 1] < t[2 * i]) continue;  // the stamped kernel did not run
+            exec.ms += (double)(t[2 * i + 1] - t[2 * i]) / wall_khz;
+            exec.work += kstamp_work[i];
+            exec.launches += 1;
+        }
+        reset_kstamps(kstamps, kstamp_used, stream);
+        SLA_HIP(hipStreamSynchronize(stream));
+        kstamp_used = 0;
+        kstamp_work.clear();
+    }
     for (auto &p : pending) {
         float ms = 0.f;
         SLA_HIP(hipEventElapsedTime(&ms, p.e0, p.e1));
@@ -224,6 +251,7 @@ void Handle::timing_collect()
 void Handle::reset_stats()
 {
     for (auto &s : stats) s = Stat{};
+    exec = Stat{};
 }
 
 // ---------------------------------------------------------------------------
@@ -385,7 +413,8 @@ void Lu<double>::update(int m, int n, int k, const double *l21, const double *u1
 {
     if (m <= 0 || n <= 0 || k <= 0) return;
     h.timing_begin(CAT_GEMM, s);
-    dgemm_nn_sub(m, n, k, l21, lda, u12, lda, c, lda, s, true, h.update_cus);
+    unsigned long long *st = h.next_kstamp(2.0 * (m / 128 * 128) * (n / 128 * 128) * k);
+    dgemm_nn_sub(m, n, k, l21, lda, u12, lda, c, lda, s, true, h.update_cus, st);
     h.timing_end(2.0 * m * n * k, s);
 }
 
@@ -428,8 +457,9 @@ void Lu<float>::update(int m, int n, int k, const float *l21, const float *u12, 
     if (m <= 0 || n <= 0 || k <= 0) return;
     if (schur == SCHUR_NATIVE) {
         h.timing_begin(CAT_GEMM, s);
+        unsigned long long *st = h.next_kstamp(2.0 * (m / 128 * 128) * (n / 128 * 128) * k);
         sgemm_nn_sub(m, n, k, l21, lda, u12, lda, c, lda, s, true, 2 * h.update_cus,
-                     h.update_lds_kb * 1024);
+                     h.update_lds_kb * 1024, st);
         h.timing_end(2.0 * m * n * k, s);
         return;
     }
@@ -460,7 +490,9 @@ void Lu<float>::update(int m, int n, int k, const float *l21, const float *u12, 
         h.timing_end(0.0, s);
         l_packed = true;
         h.timing_begin(CAT_GEMM, s);
-        gemm_split_nn_sub(split, m, n, kpad, lp, kpad, lplane, up, kpad, uplane, absmax, c, lda, s);
+        unsigned long long *st = h.next_kstamp(2.0 * (m / 128 * 128) * (n / 128 * 128) * k);
+        gemm_split_nn_sub(split, m, n, kpad, lp, kpad, lplane, up, kpad, uplane, absmax, c, lda, s,
+                          st);
         h.timing_end(2.0 * m * n * k, s);
         return;
     }
@@ -488,12 +520,13 @@ void Lu<float>::update(int m, int n, int k, const float *l21, const float *u12, 
     h.timing_end(0.0, s);
     l_packed = true;
     h.timing_begin(CAT_GEMM, s);
+    unsigned long long *st = h.next_kstamp(2.0 * (m / 128 * 128) * (n / 128 * 128) * k);
     if (schur == SCHUR_BF16)
         gemm_bf16_nn_sub(m, n, kpad, reinterpret_cast<const __bf16 *>(lp), kpad,
-                         reinterpret_cast<const __bf16 *>(up), kpad, c, lda, s);
+                         reinterpret_cast<const __bf16 *>(up), kpad, c, lda, s, st);
     else
         gemm_fp8_nn_sub(m, n, kpad, reinterpret_cast<const uint8_t *>(lp), kpad,
-                        reinterpret_cast<const uint8_t *>(up), kpad, scales, c, lda, s);
+                        reinterpret_cast<const uint8_t *>(up), kpad, scales, c, lda, s, st);
     h.timing_end(2.0 * m * n * k, s);
 }
 

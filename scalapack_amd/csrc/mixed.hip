@@ -272,6 +272,7 @@ struct Mixed {
     int ldx;
     float *S = nullptr;  // FP32 factors
     int lds = 0;
+    const double *dinv = nullptr;  // inverses of their diagonal blocks (null: substitution)
     int *ipiv_s = nullptr;
     double anorm = 0.0;
 
@@ -333,12 +334,12 @@ struct Mixed {
         SLA_LAUNCH_CHECK();
         h.timing_end(12.0 * n * nr);
         TrsvWork tw = h.trsv_work(n);
-        h.timing_begin(CAT_TRSV);
-        trsv_persistent<float, float>(TrsvOp::L, n, nr, S, lds, W, n, tw, st());
-        h.timing_end(2.0 * (double)n * n);
-        h.timing_begin(CAT_TRSV);
-        trsv_persistent<float, float>(TrsvOp::U, n, nr, S, lds, W, n, tw, st());
-        h.timing_end(2.0 * (double)n * n);
+        for (const bool lower : {true, false}) {
+            h.timing_begin(CAT_TRSV);
+            if (dinv) trsv_blockinv<float>(lower, n, nr, S, lds, dinv, W, n, tw, st());
+            else trsv_persistent<float, float>(lower ? TrsvOp::L : TrsvOp::U, n, nr, S, lds, W, n, tw, st());
+            h.timing_end(2.0 * (double)n * n);
+        }
         h.trsv_epoch = tw.epoch;
     }
 
@@ -350,12 +351,12 @@ struct Mixed {
         gather_rows<double>(v, n, z, n, h.perm, n, 1, st());
         h.timing_end(16.0 * n);
         TrsvWork tw = h.trsv_work(n);
-        h.timing_begin(CAT_TRSV);
-        trsv_persistent<float, double>(TrsvOp::L, n, 1, S, lds, z, n, tw, st());
-        h.timing_end(2.0 * (double)n * n);
-        h.timing_begin(CAT_TRSV);
-        trsv_persistent<float, double>(TrsvOp::U, n, 1, S, lds, z, n, tw, st());
-        h.timing_end(2.0 * (double)n * n);
+        for (const bool lower : {true, false}) {
+            h.timing_begin(CAT_TRSV);
+            if (dinv) trsv_blockinv<double>(lower, n, 1, S, lds, dinv, z, n, tw, st());
+            else trsv_persistent<float, double>(lower ? TrsvOp::L : TrsvOp::U, n, 1, S, lds, z, n, tw, st());
+            h.timing_end(2.0 * (double)n * n);
+        }
         h.trsv_epoch = tw.epoch;
     }
 
@@ -519,13 +520,23 @@ void sgetrs_device(Handle &h, int n, int nrhs, const float *dA, int lda, const i
     if (!(h.perm_key_a == dA && h.perm_key_ipiv == dipiv && h.perm_m == n))
         perm_from_ipiv(h, n, dipiv);
     float *W = static_cast<float *>(h.buffer(BUF_VEC, (size_t)n * 16 * 4 + 256));
+    double *inv = nullptr;
+    if (h.trsv_blockinv) {
+        inv = static_cast<double *>(h.buffer(BUF_DINV, trsv_inverse_doubles(n) * 8));
+        trsv_block_inverses(n, dA, lda, inv, h.stream);
+    }
     TrsvWork tw = h.trsv_work(n);
     for (int j = 0; j < nrhs; j += 16) {
         const int nr = std::min(16, nrhs - j);
         float *Bj = dB + (size_t)j * ldb;
         gather_rows<float>(Bj, ldb, W, n, h.perm, n, nr, h.stream);
-        trsv_persistent<float, float>(TrsvOp::L, n, nr, dA, lda, W, n, tw, h.stream);
-        trsv_persistent<float, float>(TrsvOp::U, n, nr, dA, lda, W, n, tw, h.stream);
+        if (inv) {
+            trsv_blockinv<float>(true, n, nr, dA, lda, inv, W, n, tw, h.stream);
+            trsv_blockinv<float>(false, n, nr, dA, lda, inv, W, n, tw, h.stream);
+        } else {
+            trsv_persistent<float, float>(TrsvOp::L, n, nr, dA, lda, W, n, tw, h.stream);
+            trsv_persistent<float, float>(TrsvOp::U, n, nr, dA, lda, W, n, tw, h.stream);
+        }
         SLA_HIP(hipMemcpy2DAsync(Bj, (size_t)ldb * 4, W, (size_t)n * 4, (size_t)n * 4, nr,
                                  hipMemcpyDeviceToDevice, h.stream));
     }
@@ -546,6 +557,13 @@ int dxgesv_device(Handle &h, int prec, int refine, int n, int nrhs, double *dA, 
     const double cte = std::sqrt((double)n) * EPS64;
 
     auto fallback = [&](int iter) {
+        if (iter != -1 && !h.ir_fallback) {
+            // the caller measures the attempt alone: report the failure
+            info.stop_rule = STOP_FAILED;
+            info.iter = iter;
+            h.timing_collect();
+            return 0;
+        }
         // FP64 factorization in place and solve of X = A^{-1} B; A then holds
         // the factors, so the errors of this X are not measured (NaN)
         info.fell_back = 1;
@@ -591,6 +609,13 @@ int dxgesv_device(Handle &h, int prec, int refine, int n, int nrhs, double *dA, 
     }
     const int schur = schur_of_prec(prec);
     if (sgetrf_device(h, n, n, mx.S, mx.lds, mx.ipiv_s, schur) > 0) return fallback(-3);
+    if (h.trsv_blockinv) {
+        double *inv = static_cast<double *>(h.buffer(BUF_DINV, trsv_inverse_doubles(n) * 8));
+        h.timing_begin(CAT_TRSV);
+        trsv_block_inverses(n, mx.S, mx.lds, inv, h.stream);
+        h.timing_end(0.0);
+        mx.dinv = inv;
+    }
 
     // x0 = solve(b): X = 0, R = B
     hipLaunchKernelGGL(fill_f64_kernel, dim3(blocks_for((size_t)n * nrhs)), dim3(256), 0, h.stream,

@@ -1,11 +1,11 @@
-// mlp.hip -- the precision selector's network: 12 -> 64 -> 64 -> 4, tanh
-// hidden layers, softmax over the four choices (FP8, BF16, FP32 factors with
-// FP64 refinement; plain FP64 LU), FP32 weights.  Forward, backward
+// mlp.hip -- the precision selector's network: 12 -> 64 -> 64 -> 7, tanh
+// hidden layers, softmax over the seven choices (FP8, split-BF16 or FP32
+// factors, each with classical IR or GMRES-IR; plain FP64 LU), FP32 weights.  Forward, backward
 // (softmax cross-entropy) and Adam run as HIP kernels; every reduction has a
 // fixed order (no atomics), so training is reproducible bit for bit.
 //
 // Parameter vector (floats): W1 [64][12], b1 [64], W2 [64][64], b2 [64],
-// W3 [4][64], b3 [4], then the input normalisation mu [12], sigma [12]
+// W3 [7][64], b3 [7], then the input normalisation mu [12], sigma [12]
 // (x_in = (f - mu) / sigma).  Samples are rows of X [B][12].
 #include "sla_common.h"
 #include "sla_handle.h"

@@ -130,6 +130,23 @@ __device__ __forceinline__ bool grid_sync(unsigned *ctr, unsigned target, int *a
     return s_ok != 0;
 }
 
+// Execution interval of one launch measured inside it: the earliest
+// workgroup start (stamp[0], atomic min) and the latest workgroup end
+// (stamp[1], atomic max) on the device's constant-rate wall clock.  Unlike an
+// event pair on the stream, it excludes the time the launch waits for CUs
+// held by kernels of another stream.  stamp == nullptr: nothing is recorded.
+__device__ __forceinline__ void launch_stamp_begin(unsigned long long *stamp)
+{
+    if (stamp && threadIdx.x == 0) atomicMin(stamp, (unsigned long long)wall_clock64());
+}
+__device__ __forceinline__ void launch_stamp_end(unsigned long long *stamp)
+{
+    if (stamp) {
+        __syncthreads();
+        if (threadIdx.x == 0) atomicMax(stamp + 1, (unsigned long long)wall_clock64());
+    }
+}
+
 // (|v|, row) ordering of the pivot search: larger magnitude wins, the lower
 // row wins a tie -- IDAMAX's "first index of the maximum" on the current row
 // order (PBLAS/SRC/pdamax_.c:421-425).

@@ -85,7 +85,8 @@ struct MixedInfo {
 enum StopRule {
     STOP_CONVERGED = 0,       // DSGESV's test: ||r|| <= ||x|| ||A|| sqrt(N) eps
     STOP_STALL_ACCEPTED = 1,  // a stall (PDGERFS's rule) with scaled residual < 16
-    STOP_FALLBACK = 2         // X came from the FP64 factorization
+    STOP_FALLBACK = 2,        // X came from the FP64 factorization
+    STOP_FAILED = 3           // refinement failed, FP64 fallback disabled: X is the last iterate
 };
 
 // Reusable device buffers, grown on demand.
@@ -104,6 +105,7 @@ enum BufId {
     BUF_SELECT,      // sampled submatrices of the selector's features
     BUF_MODEL,       // the selector's parameters (sla_selector_set_model)
     BUF_TRSV_MAIL,   // epoch-tagged hand-off words of the persistent TRSV
+    BUF_DINV,        // inverses of the diagonal blocks of the low-precision factors
     BUF_COUNT
 };
 
@@ -189,8 +191,12 @@ struct Handle {
     // mixed-precision refinement
     int ir_max_iter = 30;        // refinement steps before the FP64 fallback (DSGESV's ITERMAX)
     int ir_gmres_on_stall = 1;   // classical refinement that stalls continues as GMRES-IR
+    int ir_fallback = 1;         // a failed refinement is redone by the FP64 factorization
+                                 // (0: report the failure, ITER < 0, and leave A as it is)
     int ir_accept_stall = 1;     // a stalled refinement below scaled residual 16 is a result
                                  // (0: only DSGESV's convergence test ends it)
+    int trsv_blockinv = 1;       // solves with low-precision factors use the inverses of
+                                 // their 128 x 128 diagonal blocks (0: substitution)
     int gmres_m = 50;            // GMRES steps per correction
     double gmres_tol = 1e-6;     // relative residual reduction per correction
 
@@ -208,6 +214,18 @@ struct Handle {
     };
     std::vector<Pending> pending;
     std::vector<hipEvent_t> ev_pool, ev_free;
+    // execution intervals of the outer trailing-update launches, measured
+    // inside the kernels while profiling: device pairs (first workgroup
+    // start, last workgroup end) in wall-clock ticks, summed into `exec`
+    static constexpr int KSTAMP_CAP = 4096;
+    unsigned long long *kstamps = nullptr;
+    int kstamp_used = 0;
+    std::vector<double> kstamp_work;  // algorithmic flops of each stamped launch
+    double wall_khz = 100000.0;
+    Stat exec{};
+    // the next stamp pair for a trailing-update launch of `work` flops, or
+    // nullptr when not profiling (or the pairs of this collection are used up)
+    unsigned long long *next_kstamp(double work);
     void timing_begin(int cat, hipStream_t s = nullptr);
     void timing_end(double work_units, hipStream_t s = nullptr);
     void timing_collect();

@@ -9,8 +9,11 @@ namespace sla {
 // trailing: the launch is PDGETRF's outer trailing update (a kernel symbol of
 // its own in traces); max_wgs > 0 caps its workgroups (a persistent grid that
 // leaves CUs to the panel running beside it).
+// stamp: launch_stamp pair of the interior (128 x 128-tiled) launch of a
+// trailing update, or null.
 void dgemm_nn_sub(int m, int n, int k, const double *A, int lda, const double *B, int ldb,
-                  double *C, int ldc, hipStream_t s, bool trailing = false, int max_wgs = 0);
+                  double *C, int ldc, hipStream_t s, bool trailing = false, int max_wgs = 0,
+                  unsigned long long *stamp = nullptr);
 
 // ---- thin GEMMs of the recursive panel and TRSM (gemm_small.hip) -----------
 // C -= A * B when n, k <= 64 with m >= 4n (tall-narrow) or m, k <= 64
@@ -26,15 +29,16 @@ bool gemm_small(int m, int n, int k, const T *A, int lda, const T *B, int ldb, T
 // (limits the workgroups sharing a CU; the kernel itself does not use it).
 void sgemm_nn_sub(int m, int n, int k, const float *A, int lda, const float *B, int ldb, float *C,
                   int ldc, hipStream_t s, bool trailing = false, int max_wgs = 0,
-                  int reserve_lds = 0);
+                  int reserve_lds = 0, unsigned long long *stamp = nullptr);
 // Same with BF16 operands packed k-contiguous: Ap row-major m x k (ld ldak),
 // Bp column-major k x n (ld ldbk); FP32 accumulation into C.
 void gemm_bf16_nn_sub(int m, int n, int k, const __bf16 *Ap, int ldak, const __bf16 *Bp, int ldbk,
-                      float *C, int ldc, hipStream_t s);
+                      float *C, int ldc, hipStream_t s, unsigned long long *stamp = nullptr);
 // FP8-E4M3 (OCP) operands packed like the BF16 ones; the products are scaled
 // back by sa * sb (the per-panel scales the operands were divided by).
 void gemm_fp8_nn_sub(int m, int n, int k, const uint8_t *Ap, int ldak, const uint8_t *Bp,
-                     int ldbk, const float *scales, float *C, int ldc, hipStream_t s);
+                     int ldbk, const float *scales, float *C, int ldc, hipStream_t s,
+                     unsigned long long *stamp = nullptr);
 // Packing: A (m x k, column-major) -> row-major with k padded by zeros to
 // kpad; B (k x n, column-major) -> column-major with ld kpad, zero padded.
 void pack_rows_bf16(int m, int k, int kpad, const float *A, int lda, __bf16 *P, hipStream_t s);
@@ -65,7 +69,7 @@ void pack_split(int split, bool rows, int d0, int k, int kpad, const float *A, i
 // A and B packings were scaled by (FP16 split).
 void gemm_split_nn_sub(int split, int m, int n, int k, const void *Ap, int ldak, size_t aplane,
                        const void *Bp, int ldbk, size_t bplane, const float *absmax, float *C,
-                       int ldc, hipStream_t s);
+                       int ldc, hipStream_t s, unsigned long long *stamp = nullptr);
 
 // ---- precision conversions -------------------------------------------------
 // B = (float)A; sets *overflow when some |A(i,j)| exceeds the FP32 range.
@@ -162,8 +166,25 @@ template <typename TA, typename T>
 void trsv_persistent(TrsvOp op, int n, int nrhs, const TA *A, int lda, T *X, int ldx,
                      TrsvWork &tw, hipStream_t s);
 
+// Block-inverse solves with FP32 factors (refinement corrections, GMRES-IR's
+// preconditioner): trsv_block_inverses writes the FP64 inverses of the
+// 128 x 128 diagonal blocks of L (plane 0) and U (plane 1) into inv
+// (2 * ceil(n / 128) * 128 * 128 doubles); trsv_blockinv then solves
+// X <- L^{-1} X (lower) or U^{-1} X with them, one launch per triangle.
+constexpr int TRSV_INV_BS = 128;
+inline size_t trsv_inverse_doubles(int n)
+{
+    return (size_t)2 * ((n + TRSV_INV_BS - 1) / TRSV_INV_BS) * TRSV_INV_BS * TRSV_INV_BS;
+}
+void trsv_block_inverses(int n, const float *A, int lda, double *inv, hipStream_t s);
+template <typename T>
+void trsv_blockinv(bool lower, int n, int nrhs, const float *A, int lda, const double *inv, T *X,
+                   int ldx, TrsvWork &tw, hipStream_t s);
+
 // ---- small helpers ---------------------------------------------------------
 void fill_int(int *p, int n, int v, hipStream_t s);
+// n launch-stamp pairs to (UINT64_MAX, 0): no start seen, no end seen
+void reset_kstamps(unsigned long long *p, int n, hipStream_t s);
 void fill_u32(unsigned *p, int n, unsigned v, hipStream_t s);
 void iota_int(int *p, int n, int base, hipStream_t s);
 // B[i, :] = Bsrc[perm[i], :] (row gather), n x nrhs

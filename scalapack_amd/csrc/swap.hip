@@ -246,6 +246,14 @@ __global__ void fill_u32_kernel(unsigned *p, int n, unsigned v)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
+__global__ void reset_kstamps_kernel(unsigned long long *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        p[2 * i] = ~0ull;
+        p[2 * i + 1] = 0ull;
+    }
+}
 __global__ void iota_int_kernel(int *p, int n, int base)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -339,6 +347,12 @@ void fill_u32(unsigned *p, int n, unsigned v, hipStream_t s)
 {
     if (n <= 0) return;
     hipLaunchKernelGGL(fill_u32_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, p, n, v);
+    SLA_LAUNCH_CHECK();
+}
+void reset_kstamps(unsigned long long *p, int n, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(reset_kstamps_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, p, n);
     SLA_LAUNCH_CHECK();
 }
 void iota_int(int *p, int n, int base, hipStream_t s)

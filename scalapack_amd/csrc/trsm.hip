@@ -285,7 +285,245 @@ __global__ __launch_bounds__(256) void trsv_persistent_kernel(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Block-inverse solves with the factors of a low-precision factorization
+// (the corrections of iterative refinement and GMRES-IR's preconditioner).
+// The inverses of the 128 x 128 diagonal blocks of L and U are formed once
+// per factorization, so a step of the dependency chain is two products with
+// blocks already in registers instead of 128 dependent substitution steps:
+//   x_b = D_b^{-1} (rhs_b - sum_{c before b} T[b, c] x_c).
+constexpr int TB_BS = 128, TB_KH = TB_BS / 2, TB_INV_COLS = 64;
+
+// inv holds plane 0 = the inverses of L's unit-lower diagonal blocks and
+// plane 1 = those of U's upper diagonal blocks, FP64, block b at
+// inv + (plane * nblk + b) * 128 * 128, column-major (element (r, k) at
+// k * 128 + r), zero outside the triangle; rows past n are the identity's.
+// Workgroup (b, half, plane): one wave, lane j forms column 64 half + j by
+// substitution against e_j in FP64, the block staged in LDS (all lanes read
+// the same element: a broadcast) and the column in LDS (each lane its own).
+template <typename TA>
+__global__ __launch_bounds__(TB_INV_COLS) void trsv_diag_inverse_kernel(int n, const TA *__restrict__ A,
+                                                                       int lda, double *__restrict__ inv)
+{
+    __shared__ TA Ds[TB_BS][TB_BS + 1];
+    __shared__ double Xs[TB_BS][TB_INV_COLS];
+    const int b = blockIdx.x, half = blockIdx.y, plane = blockIdx.z;
+    const int nblk = gridDim.x, rb = b * TB_BS, rows = min(TB_BS, n - rb);
+    const int lane = threadIdx.x, j = half * TB_INV_COLS + lane;
+    for (int idx = lane; idx < TB_BS * TB_BS; idx += TB_INV_COLS) {
+        const int r = idx % TB_BS, k = idx / TB_BS;
+        TA v = (r == k) ? TA(1) : TA(0);
+        if (r < rows && k < rows) v = A[(size_t)(rb + r) + (size_t)(rb + k) * lda];
+        Ds[r][k] = v;
+    }
+    __syncthreads();
+    if (plane == 0) {
+        // unit lower: x_i = e_j(i) - sum_{k < i} L(i, k) x_k
+        for (int i = 0; i < TB_BS; i++) {
+            double s = (i == j) ? 1.0 : 0.0;
+            for (int k = 0; k < i; k++) s = fma(-(double)Ds[i][k], Xs[k][lane], s);
+            Xs[i][lane] = s;
+        }
+    } else {
+        // upper: x_i = (e_j(i) - sum_{k > i} U(i, k) x_k) / U(i, i)
+        for (int i = TB_BS - 1; i >= 0; i--) {
+            double s = (i == j) ? 1.0 : 0.0;
+            for (int k = i + 1; k < TB_BS; k++) s = fma(-(double)Ds[i][k], Xs[k][lane], s);
+            Xs[i][lane] = s / (double)Ds[i][i];
+        }
+    }
+    double *out = inv + ((size_t)plane * nblk + b) * TB_BS * TB_BS + (size_t)j * TB_BS;
+    for (int i = 0; i < TB_BS; i++) out[i] = Xs[i][lane];
+}
+
+// X <- L^{-1} X (FORWARD) or U^{-1} X for the FP32 factors at A with the
+// block inverses at inv (plane 0 / 1), nrhs <= NR <= 16, T the arithmetic.
+// Same ticket / epoch-tagged mailbox protocol as trsv_persistent_kernel, with
+// 128-row blocks: thread (r, h) = (tid % 128, tid / 128) owns row r and the
+// k-half h of every block it multiplies; waves 0 and 1 poll the 128 rows of
+// the block they wait for.
+template <typename T, bool FORWARD, int NR>
+__global__ __launch_bounds__(256) void trsv_blockinv_kernel(
+    int n, int nrhs, const float *__restrict__ A, int lda, const double *__restrict__ inv, T *X,
+    int ldx, int *ticket, uint64_t *mail, unsigned epoch, int *abort_flag, uint64_t spin_limit)
+{
+    constexpr int WPV = sizeof(T) / 4;
+    __shared__ int s_b, s_abort;
+    __shared__ T Xs[TB_BS][NR];
+    __shared__ T Ph[2][TB_BS][NR + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r = tid % TB_BS, h = tid / TB_BS;
+    const int nblk = ceil_div(n, TB_BS);
+    if (tid == 0) {
+        s_b = atomicAdd(ticket, 1);
+        s_abort = 0;
+    }
+    __syncthreads();
+    const int t = s_b;
+    if (t >= nblk) return;
+    const int b = FORWARD ? t : nblk - 1 - t;
+    const int rb = b * TB_BS, rows = min(TB_BS, n - rb);
+    auto mbox = [&](int c, int j, int w, int row) -> uint64_t * {
+        return mail + ((((size_t)c * NR + j) * WPV + w) * TB_BS + row);
+    };
+
+    // the diagonal block's inverse, row r, columns of half h (before any wait)
+    T dv[TB_KH];
+    {
+        const double *Di = inv + ((size_t)(FORWARD ? 0 : 1) * nblk + b) * TB_BS * TB_BS;
+#pragma unroll
+        for (int kk = 0; kk < TB_KH; kk++) dv[kk] = (T)Di[(size_t)(h * TB_KH + kk) * TB_BS + r];
+    }
+    T rhs[NR], part[NR];
+#pragma unroll
+    for (int j = 0; j < NR; j++) {
+        rhs[j] = (h == 0 && r < rows && j < nrhs) ? X[(size_t)(rb + r) + (size_t)j * ldx] : T(0);
+        part[j] = T(0);
+    }
+    float a[TB_KH];
+    auto load_block = [&](int s) {
+        const int c = FORWARD ? s : nblk - 1 - s;
+        const int cb = c * TB_BS, crows = min(TB_BS, n - cb);
+        const float *Ar = A + (size_t)(rb + r) + (size_t)(cb + h * TB_KH) * lda;
+#pragma unroll
+        for (int kk = 0; kk < TB_KH; kk++)
+            a[kk] = (r < rows && h * TB_KH + kk < crows) ? Ar[(size_t)kk * lda] : 0.f;
+    };
+    if (t > 0) load_block(0);
+
+    for (int s = 0; s < t; s++) {
+        const int c = FORWARD ? s : nblk - 1 - s;
+        const int crows = min(TB_BS, n - c * TB_BS);
+        if (wave < 2) {
+            const int row = wave * 64 + lane;
+            uint32_t got[NR][WPV];
+            int ab = 0;
+            const uint64_t t0 = wall_clock64();
+            for (;;) {
+                bool ok = true;
+                if (row < crows) {
+#pragma unroll
+                    for (int j = 0; j < NR; j++) {
+                        if (j < nrhs) {
+#pragma unroll
+                            for (int w = 0; w < WPV; w++) {
+                                const uint64_t u = __hip_atomic_load(
+                                    mbox(c, j, w, row), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                                got[j][w] = (uint32_t)u;
+                                ok = ok && (uint32_t)(u >> 32) == epoch;
+                            }
+                        }
+                    }
+                }
+                if (__all(ok)) break;
+                if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    ab = 1;
+                    break;
+                }
+                if (wall_clock64() - t0 > spin_limit) {
+                    __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ab = 1;
+                    break;
+                }
+                if (t - s > 4) __builtin_amdgcn_s_sleep(16);
+                else __builtin_amdgcn_s_sleep(1);
+            }
+#pragma unroll
+            for (int j = 0; j < NR; j++)
+                Xs[row][j] = (row < crows && j < nrhs) ? tv_join<T>(got[j][0], got[j][WPV - 1]) : T(0);
+            if (ab) s_abort = 1;
+        }
+        __syncthreads();
+        if (s_abort) return;  // the whole workgroup leaves together
+#pragma unroll
+        for (int kk = 0; kk < TB_KH; kk++) {
+            const T av = (T)a[kk];
+#pragma unroll
+            for (int j = 0; j < NR; j++) part[j] = fma(av, Xs[h * TB_KH + kk][j], part[j]);
+        }
+        if (s + 1 < t) load_block(s + 1);
+        __syncthreads();  // Xs is reused by the next block
+    }
+    // y = rhs - (part_0 + part_1), then x = D^{-1} y as two half products
+#pragma unroll
+    for (int j = 0; j < NR; j++) Ph[h][r][j] = part[j];
+    __syncthreads();
+    if (h == 0) {
+#pragma unroll
+        for (int j = 0; j < NR; j++) Xs[r][j] = r < rows ? rhs[j] - (Ph[0][r][j] + Ph[1][r][j]) : T(0);
+    }
+    __syncthreads();
+    T q[NR];
+#pragma unroll
+    for (int j = 0; j < NR; j++) q[j] = T(0);
+#pragma unroll
+    for (int kk = 0; kk < TB_KH; kk++) {
+        const T d = dv[kk];
+#pragma unroll
+        for (int j = 0; j < NR; j++) q[j] = fma(d, Xs[h * TB_KH + kk][j], q[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < NR; j++) Ph[h][r][j] = q[j];
+    __syncthreads();
+    if (h == 0 && r < rows) {
+        const uint64_t tag = (uint64_t)epoch << 32;
+#pragma unroll
+        for (int j = 0; j < NR; j++) {
+            if (j < nrhs) {
+                const T x = Ph[0][r][j] + Ph[1][r][j];
+#pragma unroll
+                for (int w = 0; w < WPV; w++)
+                    __hip_atomic_store(mbox(b, j, w, r), tag | tv_half(x, w), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                X[(size_t)(rb + r) + (size_t)j * ldx] = x;
+            }
+        }
+    }
+}
+
 }  // namespace
+
+void trsv_block_inverses(int n, const float *A, int lda, double *inv, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(trsv_diag_inverse_kernel<float>, dim3(ceil_div(n, TB_BS), 2, 2),
+                       dim3(TB_INV_COLS), 0, s, n, A, lda, inv);
+    SLA_LAUNCH_CHECK();
+}
+
+template <typename T>
+void trsv_blockinv(bool lower, int n, int nrhs, const float *A, int lda, const double *inv, T *X,
+                   int ldx, TrsvWork &tw, hipStream_t s)
+{
+    if (n <= 0 || nrhs <= 0) return;
+    if (nrhs > TV_NR) throw std::logic_error("trsv_blockinv: at most 16 right-hand sides per call");
+    const int nblk = ceil_div(n, TB_BS);
+    if ((size_t)nblk * TB_BS * TV_NR * 2 > tw.mail_words)
+        throw std::logic_error("trsv_blockinv: mailbox too small for this order");
+    fill_int(tw.ticket, 1, 0, s);
+    const int epoch = ++tw.epoch;
+#define SLA_TBI(FW, NRV)                                                                          \
+    hipLaunchKernelGGL((trsv_blockinv_kernel<T, FW, NRV>), dim3(nblk), dim3(256), 0, s, n, nrhs, \
+                       A, lda, inv, X, ldx, tw.ticket, tw.mail, (unsigned)epoch, tw.abort_flag,  \
+                       tw.spin_limit)
+#define SLA_TBI_NR(FW)                          \
+    do {                                        \
+        if (nrhs <= 1) SLA_TBI(FW, 1);          \
+        else if (nrhs <= 2) SLA_TBI(FW, 2);     \
+        else if (nrhs <= 4) SLA_TBI(FW, 4);     \
+        else if (nrhs <= 8) SLA_TBI(FW, 8);     \
+        else SLA_TBI(FW, 16);                   \
+    } while (0)
+    if (lower) SLA_TBI_NR(true);
+    else SLA_TBI_NR(false);
+#undef SLA_TBI_NR
+#undef SLA_TBI
+    SLA_LAUNCH_CHECK();
+}
+template void trsv_blockinv<float>(bool, int, int, const float *, int, const double *, float *,
+                                   int, TrsvWork &, hipStream_t);
+template void trsv_blockinv<double>(bool, int, int, const float *, int, const double *, double *,
+                                    int, TrsvWork &, hipStream_t);
 
 template <typename T>
 void trsm_llnu(int m, int n, const T *L, int ldl, T *B, int ldb, hipStream_t s)

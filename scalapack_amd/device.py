@@ -17,7 +17,7 @@ STAT_CLASSES = ("panel", "laswp", "trsm", "gemm", "trsv", "other", "gemm_panel")
 PRECISIONS = {"fp64": 0, "fp32": 1, "bf16": 2, "fp8": 3, "bf16x3": 4, "bf16x6": 5, "fp16x3": 6}
 REFINEMENTS = {"ir": 0, "gmres": 1}
 # what ended a mixed-precision refinement (SLA_STOP_*)
-STOP_RULES = ("converged", "stall_accepted", "fallback")
+STOP_RULES = ("converged", "stall_accepted", "fallback", "failed")
 
 
 class SlaError(RuntimeError):
@@ -91,6 +91,14 @@ class Handle:
                                            ctypes.byref(n)))
             out[name] = {"ms": ms.value, "work": work.value, "launches": n.value}
         return out
+
+    def update_exec(self) -> dict[str, float]:
+        """Trailing-update launches as the kernels measure their own
+        execution (first workgroup start to last workgroup end)."""
+        ms, work, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_longlong()
+        _check(self._lib.sla_get_update_exec(self._h, ctypes.byref(ms), ctypes.byref(work),
+                                             ctypes.byref(n)))
+        return {"ms": ms.value, "work": work.value, "launches": n.value}
 
     def reset_stats(self) -> None:
         _check(self._lib.sla_reset_stats(self._h))

@@ -138,7 +138,8 @@ def split_slices(a32: np.ndarray, prec: str) -> tuple[list[np.ndarray], int]:
 
 @pytest.mark.parametrize("prec", ["bf16x3", "bf16x6", "fp16x3"])
 @pytest.mark.parametrize("m,n,k", [(128, 128, 128), (256, 384, 512), (1024, 1024, 512),
-                                   (300, 200, 77), (130, 260, 32)])
+                                   (300, 200, 77), (130, 260, 32), (520, 390, 64),
+                                   (768, 640, 512)])
 def test_gemm_split_parity(dev, prec, m, n, k):
     """Split 16-bit products: against the float64 sum of the slice products
     the kernel forms (FP32 accumulation bound), and against the exact product
@@ -409,3 +410,42 @@ def test_dxgesv_large_fp32_ir(dev, pivoting):
     _note(f"  iter={it} history={st['history']} sresid={s:.3e}")
     assert info == 0 and it >= 0
     assert s < 16.0
+
+
+@pytest.mark.parametrize("blockinv", [0, 1])
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (100, 3), (128, 1), (129, 2), (1000, 16),
+                                    (4100, 1), (4100, 5)])
+def test_sgetrs_matches_float64_solve_with_the_same_factors(dev, n, nrhs, blockinv):
+    """sla_sgetrs (FP32 arithmetic) against P^T-applied float64 triangular
+    solves with the same FP32 factors of a diagonally dominant A: relative
+    error within 64 n 2^-24 (the FP32 solves' rounding; the factors are
+    well conditioned).  Both solvers: substitution (blockinv 0) and the
+    inverses of the 128 x 128 diagonal blocks (blockinv 1, block edges at
+    128 and 129, several block rows at 4100)."""
+    _note(f"sgetrs n={n} nrhs={nrhs} blockinv={blockinv}")
+    rng = np.random.default_rng(n + 17 * nrhs)
+    A = rng.uniform(-1, 1, (n, n)) + n * np.eye(n)
+    B = rng.uniform(-1, 1, (n, nrhs))
+    dA = to_dev(A, torch.float32)
+    ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
+    assert dev.sgetrf(n, n, dA, n, ipiv, prec="fp32") == 0
+    dB = to_dev(B, torch.float32)
+    b0 = dev.get("trsv_blockinv")
+    dev.set("trsv_blockinv", blockinv)
+    try:
+        dev.sgetrs(n, nrhs, dA, n, ipiv, dB, n)
+    finally:
+        dev.set("trsv_blockinv", b0)
+    torch.cuda.synchronize()
+    LU = from_dev(dA).astype(np.float64)
+    L = np.tril(LU, -1) + np.eye(n)
+    U = np.triu(LU)
+    pb = B.copy()
+    for i, p in enumerate(ipiv.cpu().numpy()):  # PDLAPIV: interchanges in order
+        pb[[i, p - 1]] = pb[[p - 1, i]]
+    import scipy.linalg as sl
+    x_ref = sl.solve_triangular(U, sl.solve_triangular(L, pb, lower=True, unit_diagonal=True))
+    X = from_dev(dB).astype(np.float64)
+    err = np.abs(X - x_ref).max() / np.abs(x_ref).max()
+    _note(f"  max rel error {err:.3e}")
+    assert err <= 64 * n * 2.0 ** -24

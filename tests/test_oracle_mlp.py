@@ -2,7 +2,7 @@
 the mathematics: its analytic gradient against central finite differences
 in float64 (a dropped tanh derivative, a transposed weight or a missing
 class weight fails), the softmax probabilities summing to one, and the
-loss of a zero network being log 4."""
+loss of a zero network being log 7 (seven classes)."""
 import sys
 from pathlib import Path
 
@@ -21,12 +21,12 @@ def _params(seed):
     return p
 
 
-@pytest.mark.parametrize("cost", [None, [1.0, 2.0, 0.5, 3.0]])
+@pytest.mark.parametrize("cost", [None, [1.0, 2.0, 0.5, 3.0, 1.5, 0.7, 2.5]])
 def test_gradient_matches_finite_differences(cost):
     rng = np.random.default_rng(3)
     p = _params(4)
     X = rng.normal(0, 2, (9, 12))
-    y = rng.integers(0, 4, 9)
+    y = rng.integers(0, 7, 9)
     loss, g = mo.loss_grad(p, X, y, cost)
     assert np.isclose(loss, mo.loss_only(p, X, y, cost), rtol=1e-14, atol=0)
     idx = rng.choice(mo.NTRAIN, 60, replace=False)
@@ -46,5 +46,5 @@ def test_softmax_and_zero_network():
     z = np.zeros(mo.NPARAM)
     z[-12:] = 1.0
     loss, g = mo.loss_grad(z, X, np.array([0, 1, 2, 3, 0]))
-    assert np.isclose(loss, np.log(4.0), rtol=1e-15, atol=0)
-    assert mo.NPARAM == 5276 and mo.NTRAIN == 5252
+    assert np.isclose(loss, np.log(7.0), rtol=1e-15, atol=0)
+    assert mo.NPARAM == 5471 and mo.NTRAIN == 5447
