@@ -165,8 +165,8 @@ def svd_matrices(torch, n: int, kappas, seed: int = 100):
 
 
 # the selector's classes (scalapack_amd.selector.CHOICES) as "precision+refinement"
-CHOICE_NAMES = ("fp8+ir", "fp8+gmres", "bf16x3+ir", "bf16x3+gmres", "fp32+ir", "fp32+gmres",
-                "fp64")
+CHOICE_NAMES = ("fp8+ir", "fp8+gmres", "bf16x3+ir", "bf16x3+gmres", "bf16x6+ir",
+                "bf16x6+gmres", "fp64")
 
 KERNEL_OF = {  # kernel symbol behind each timing class (for matching kernel traces)
     ("gemm", "fp64"): "dgemm_nn_sub_direct_kernel<1> (trailing update A22 -= L21 U12)",

@@ -345,8 +345,9 @@ int sla_gesv_batched_tune(sla_handle_t h, int prec, int batch, const int *n, con
 /* ------------------------------------------------------------------------ */
 /* A network 12 -> 64 -> 64 -> 7 (tanh, softmax; FP32 weights) picks the
  * factorization precision and the refinement of sla_dxgesv -- class 0 FP8 +
- * classical IR, 1 FP8 + GMRES-IR, 2 split BF16 (SLA_PREC_BF16X3) + IR, 3 split
- * BF16 + GMRES-IR, 4 FP32 + IR, 5 FP32 + GMRES-IR, 6 plain FP64 LU -- from 12
+ * classical IR, 1 FP8 + GMRES-IR, 2 SLA_PREC_BF16X3 + IR, 3 BF16X3 + GMRES-IR,
+ * 4 SLA_PREC_BF16X6 (FP32-accurate factors) + IR, 5 BF16X6 + GMRES-IR, 6 plain
+ * FP64 LU -- from 12
  * features of A computed on the GPU (norms, row / column scaling spread,
  * sampled pivot growth and a PDGECON-style condition estimate of sampled
  * principal submatrices, and the stable-rank fraction ||A||_F^2 / (n

@@ -1361,12 +1361,12 @@ int sla_dxgesv(sla_handle_t h, int prec, int refine, int n, int nrhs, double *dA
 static_assert(sla::MLP_NPARAM == SLA_MLP_NPARAM, "parameter count of the selector network");
 
 // The selector's class -> factorization precision and refinement of
-// sla_dxgesv: {FP8, split BF16, FP32} x {classical IR, GMRES-IR}, then FP64.
+// sla_dxgesv: {FP8, BF16x3, BF16x6} x {classical IR, GMRES-IR}, then FP64.
 static_assert(sla::MLP_OUT == SLA_SELECT_CLASSES, "classes of the selector network");
 static int selector_precision(int choice)
 {
-    static const int prec[SLA_SELECT_CLASSES] = {SLA_PREC_FP8,   SLA_PREC_FP8,  SLA_PREC_BF16X3,
-                                                 SLA_PREC_BF16X3, SLA_PREC_FP32, SLA_PREC_FP32,
+    static const int prec[SLA_SELECT_CLASSES] = {SLA_PREC_FP8,    SLA_PREC_FP8,    SLA_PREC_BF16X3,
+                                                 SLA_PREC_BF16X3, SLA_PREC_BF16X6, SLA_PREC_BF16X6,
                                                  SLA_PREC_FP64};
     return (choice >= 0 && choice < SLA_SELECT_CLASSES) ? prec[choice] : SLA_PREC_FP64;
 }

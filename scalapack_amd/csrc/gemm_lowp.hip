@@ -460,19 +460,21 @@ __global__ __launch_bounds__(256, S == 2 ? 2 : 1) void gemm_split_nn_sub_direct_
     launch_stamp_end(stamp);
 }
 
-// The split product for large tiles: a 256 x 128 tile of C per 256-thread
-// workgroup, the operand slices of each 32-deep K step staged once in LDS
-// and shared by the four waves (wave w: rows 128 (w & 1) .., columns
-// 64 (w >> 1) ..: 8 x 4 accumulators of 16 x 16), the next K step's slices
-// loaded into registers while the MFMAs of this one run, LDS double
-// buffered (one workgroup barrier per K step).  LDS rows hold 32 k values
-// plus 8 of padding (80 bytes), so the 16 lanes that read one 16-byte
-// fragment each cover distinct banks.  Two slices (S = 2) only: the
-// double-buffered tiles take 120 KB of LDS.
-constexpr int SL_M = 256, SL_N = 128, SL_K = 32, SL_LD = SL_K + 8;
+// The split product for large tiles: a TM x 128 tile of C per 256-thread
+// workgroup (TM = 256 for two slices, 128 for three), the operand slices of
+// each 32-deep K step staged once in LDS and shared by the four waves (wave
+// w: rows (TM / 2) (w & 1) .., columns 64 (w >> 1) ..: TM / 32 x 4
+// accumulators of 16 x 16), the next K step's slices loaded into registers
+// while the MFMAs of this one run, LDS double buffered (one workgroup
+// barrier per K step).  LDS rows hold 32 k values plus 8 of padding (80
+// bytes), so the 16 lanes that read one 16-byte fragment each cover distinct
+// banks.  Either slice count takes 120 KB of LDS; the products run in the
+// direct kernel's order, so both kernels give the same bits.
+constexpr int SL_N = 128, SL_K = 32, SL_LD = SL_K + 8;
+constexpr int split_tile_m(int S) { return S == 2 ? 256 : 128; }
 constexpr size_t split_lds_bytes(int S)
 {
-    return (size_t)2 * S * (SL_M + SL_N) * SL_LD * 2;
+    return (size_t)2 * S * (split_tile_m(S) + SL_N) * SL_LD * 2;
 }
 
 template <typename T16, int S>
@@ -481,33 +483,33 @@ __global__ __launch_bounds__(256, 1) void gemm_split_lds_kernel(
     const T16 *__restrict__ Bp, int ldbk, size_t bplane, const float *__restrict__ absmax,
     float *__restrict__ C, int ldc, unsigned long long *stamp)
 {
-    static_assert(S == 2, "the LDS tiles are sized for two slices");
+    constexpr int TM = split_tile_m(S), MB = TM / 32, AR = TM / 64;
     extern __shared__ __align__(16) unsigned char sl_lds[];
     // [buf][slice][row][SL_LD] of A, then [buf][slice][col][SL_LD] of B
     T16 *As = reinterpret_cast<T16 *>(sl_lds);
-    T16 *Bs = As + (size_t)2 * S * SL_M * SL_LD;
-    auto a_at = [&](int buf, int sl, int row) { return As + (((size_t)buf * S + sl) * SL_M + row) * SL_LD; };
+    T16 *Bs = As + (size_t)2 * S * TM * SL_LD;
+    auto a_at = [&](int buf, int sl, int row) { return As + (((size_t)buf * S + sl) * TM + row) * SL_LD; };
     auto b_at = [&](int buf, int sl, int col) { return Bs + (((size_t)buf * S + sl) * SL_N + col) * SL_LD; };
 
     launch_stamp_begin(stamp);
-    const int tiles_m = m / SL_M, tiles_n = n / SL_N;
+    const int tiles_m = m / TM, tiles_n = n / SL_N;
     int tmi, tni;
     tile_coords_lp(blockIdx.x, tiles_m, tiles_n, tmi, tni);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int row0 = tmi * SL_M, col0 = tni * SL_N;
-    const int wm = (wave & 1) * 128, wn = (wave >> 1) * 64;
+    const int row0 = tmi * TM, col0 = tni * SL_N;
+    const int wm = (wave & 1) * (TM / 2), wn = (wave >> 1) * 64;
     const int lr = lane & 15, lk = lane >> 4;
 
-    // global -> register staging: A rows tid / 4 + 64 i (i < 4), B columns
-    // tid / 4 + 64 i (i < 2), 16-byte chunk tid % 4 of the 32 k values
+    // global -> register staging: A rows tid / 4 + 64 i (i < TM / 64), B
+    // columns tid / 4 + 64 i (i < 2), 16-byte chunk tid % 4 of the 32 k values
     const int srow = tid >> 2, sch = (tid & 3) * 8;
-    s8_t ra[S][4], rb[S][2];
+    s8_t ra[S][AR], rb[S][2];
     auto gload = [&](int t) {
         const size_t k0 = (size_t)SL_K * t + sch;
 #pragma unroll
         for (int sl = 0; sl < S; sl++) {
 #pragma unroll
-            for (int i = 0; i < 4; i++)
+            for (int i = 0; i < AR; i++)
                 ra[sl][i] = *(const s8_t *)(Ap + sl * aplane + (size_t)(row0 + srow + 64 * i) * ldak + k0);
 #pragma unroll
             for (int i = 0; i < 2; i++)
@@ -518,15 +520,15 @@ __global__ __launch_bounds__(256, 1) void gemm_split_lds_kernel(
 #pragma unroll
         for (int sl = 0; sl < S; sl++) {
 #pragma unroll
-            for (int i = 0; i < 4; i++) *(s8_t *)(a_at(buf, sl, srow + 64 * i) + sch) = ra[sl][i];
+            for (int i = 0; i < AR; i++) *(s8_t *)(a_at(buf, sl, srow + 64 * i) + sch) = ra[sl][i];
 #pragma unroll
             for (int i = 0; i < 2; i++) *(s8_t *)(b_at(buf, sl, srow + 64 * i) + sch) = rb[sl][i];
         }
     };
 
-    f4_t acc[8][4];
+    f4_t acc[MB][4];
 #pragma unroll
-    for (int i = 0; i < 8; i++)
+    for (int i = 0; i < MB; i++)
 #pragma unroll
         for (int j = 0; j < 4; j++) acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
 
@@ -543,13 +545,18 @@ __global__ __launch_bounds__(256, 1) void gemm_split_lds_kernel(
 #pragma unroll
             for (int j = 0; j < 4; j++) fb[sl][j] = *(const s8_t *)(b_at(buf, sl, wn + 16 * j + lr) + 8 * lk);
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
+        for (int i = 0; i < MB; i++) {
             s8_t fa[S];
 #pragma unroll
             for (int sl = 0; sl < S; sl++) fa[sl] = *(const s8_t *)(a_at(buf, sl, wm + 16 * i + lr) + 8 * lk);
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                 f4_t c = acc[i][j];
+                if constexpr (S == 3) {
+                    c = mfma16<T16>(fa[2], fb[0][j], c);
+                    c = mfma16<T16>(fa[1], fb[1][j], c);
+                    c = mfma16<T16>(fa[0], fb[2][j], c);
+                }
                 c = mfma16<T16>(fa[1], fb[0][j], c);
                 c = mfma16<T16>(fa[0], fb[1][j], c);
                 acc[i][j] = mfma16<T16>(fa[0], fb[0][j], c);
@@ -564,7 +571,7 @@ __global__ __launch_bounds__(256, 1) void gemm_split_lds_kernel(
     for (int j = 0; j < 4; j++) {
         float *Cc = C + (size_t)(col0 + wn + 16 * j + lr) * ldc + row0 + wm;
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
+        for (int i = 0; i < MB; i++) {
             f4_t *cp = (f4_t *)(Cc + 16 * i + 4 * lk);
             f4_t c = *cp;
             c[0] -= back * acc[i][j][0];
@@ -901,18 +908,19 @@ static void gemm_split_launch(int m, int n, int k, const T16 *Ap, int ldak, size
     const bool vec = k % 32 == 0 && ldak % 8 == 0 && ldbk % 8 == 0 && ldc % 4 == 0 &&
                      aplane % 8 == 0 && bplane % 8 == 0 && aligned16(Ap) && aligned16(Bp) &&
                      aligned16(C);
-    const int ml = (m / SL_M) * SL_M, nl = (n / SL_N) * SL_N;
-    if (S == 2 && split_lds_on() && vec && ml > 0 && nl > 0) {
-        // 256 x 128 tiles through LDS; the borders by the kernels below
+    constexpr int TM = split_tile_m(S);
+    const int ml = (m / TM) * TM, nl = (n / SL_N) * SL_N;
+    if (split_lds_on() && vec && ml > 0 && nl > 0) {
+        // TM x 128 tiles through LDS; the borders by the kernels below
         static bool attr = false;
         if (!attr) {
             SLA_HIP(hipFuncSetAttribute(
-                reinterpret_cast<const void *>(&gemm_split_lds_kernel<T16, 2>),
-                hipFuncAttributeMaxDynamicSharedMemorySize, (int)split_lds_bytes(2)));
+                reinterpret_cast<const void *>(&gemm_split_lds_kernel<T16, S>),
+                hipFuncAttributeMaxDynamicSharedMemorySize, (int)split_lds_bytes(S)));
             attr = true;
         }
-        hipLaunchKernelGGL((gemm_split_lds_kernel<T16, 2>), dim3((ml / SL_M) * (nl / SL_N)),
-                           dim3(256), split_lds_bytes(2), s, ml, nl, k, Ap, ldak, aplane, Bp, ldbk,
+        hipLaunchKernelGGL((gemm_split_lds_kernel<T16, S>), dim3((ml / TM) * (nl / SL_N)),
+                           dim3(256), split_lds_bytes(S), s, ml, nl, k, Ap, ldak, aplane, Bp, ldbk,
                            bplane, absmax, C, ldc, stamp);
         SLA_LAUNCH_CHECK();
         if (m > ml)

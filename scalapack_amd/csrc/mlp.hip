@@ -1,5 +1,5 @@
 // mlp.hip -- the precision selector's network: 12 -> 64 -> 64 -> 7, tanh
-// hidden layers, softmax over the seven choices (FP8, split-BF16 or FP32
+// hidden layers, softmax over the seven choices (FP8, BF16x3 or BF16x6
 // factors, each with classical IR or GMRES-IR; plain FP64 LU), FP32 weights.  Forward, backward
 // (softmax cross-entropy) and Adam run as HIP kernels; every reduction has a
 // fixed order (no atomics), so training is reproducible bit for bit.
