@@ -259,6 +259,7 @@ __global__ __launch_bounds__(256) void combine_kernel(int n, int nj, const doubl
 }
 
 unsigned blocks_for(size_t n) { return (unsigned)((n + 255) / 256); }
+size_t round_up(size_t n, size_t q) { return (n + q - 1) / q * q; }
 
 // ---------------------------------------------------------------------------
 struct Mixed {
@@ -288,10 +289,15 @@ struct Mixed {
     void alloc()
     {
         const size_t nn = (size_t)n;
-        char *v = static_cast<char *>(h.buffer(BUF_VEC, nn * nrhs * 8 + nn * nrhs * 4 + 4096));
+        // R, then W, then the norm words; every part starts on a 256-byte
+        // boundary (the norms take 64-bit atomics, which must be naturally
+        // aligned: 12 n nrhs bytes alone is not a multiple of 8 for odd n nrhs)
+        const size_t offw = round_up(nn * nrhs * 8, 256);
+        const size_t offn = offw + round_up(nn * nrhs * 4, 256);
+        char *v = static_cast<char *>(h.buffer(BUF_VEC, offn + 4096));
         R = reinterpret_cast<double *>(v);
-        W = reinterpret_cast<float *>(v + nn * nrhs * 8);
-        norms = reinterpret_cast<unsigned long long *>(v + nn * nrhs * 8 + nn * nrhs * 4);
+        W = reinterpret_cast<float *>(v + offw);
+        norms = reinterpret_cast<unsigned long long *>(v + offn);
         slices = ceil_div(n, RES_KC);
         const int nrp = nrhs <= 1 ? 1 : nrhs <= 4 ? 4 : 16;
         part = static_cast<double *>(h.buffer(BUF_PARTIAL, (size_t)slices * nrp * nn * 8 + 256));

@@ -235,8 +235,12 @@ def _oracle_x(oracle, A, B, nb=64):
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16"])
-@pytest.mark.parametrize("n,nrhs", [(2048, 1), (1500, 3), (1024, 16), (4096, 1)])
+@pytest.mark.parametrize("n,nrhs", [(2048, 1), (1500, 3), (1024, 16), (4096, 1), (1001, 1),
+                                    (777, 3)])
 def test_dxgesv_uniform(dev, oracle, prec, n, nrhs):
+    """(1001, 1) and (777, 3): n nrhs odd, so 12 n nrhs bytes of R and W are
+    not a multiple of 8 -- the norm words after them must still be aligned
+    for their 64-bit atomics (the refinement buffer's layout)."""
     _note(f"dxgesv {prec} IR uniform n={n} nrhs={nrhs}")
     rng = np.random.default_rng(n + nrhs)
     A = np.asfortranarray(rng.uniform(-1, 1, (n, n)))
