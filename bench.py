@@ -10,14 +10,18 @@ for every timed solve.  The solve never reads a cached result: every step
 starts from the pristine A (restored outside the timed region where the
 solve overwrites it) and does the whole factorization + solve (+ refinement).
 
---mode mixed (default; the headline): sla_dxgesv -- an FP32-storage LU whose
-    trailing updates run on FP32 MFMA, refined with FP64 residuals to FP64
-    backward error (classical refinement, continuing as GMRES-IR on a stall;
-    GMRES-IR from the start for kappa >= 1e6).  Matrices: A = U diag(sigma)
+--mode mixed (default; the headline): the learned selector (--policy auto,
+    sla_dxgesv_auto: features + network + solve inside the timed region)
+    picks the factorization precision (FP8 / BF16x3 / BF16x6 trailing
+    updates of an FP32-storage LU, or plain FP64 LU) and the refinement
+    (classical IR or GMRES-IR) per matrix; refined with FP64 residuals to
+    FP64 backward error, FP64 fallback as the judge.  --policy fixed runs
+    --prec with IR (GMRES-IR from --gmres-from).  Matrices: A = U diag(sigma)
     V^T, U and V the Q factors of seeded Gaussian N x N matrices, sigma
     geometric from 1 to 1/kappa, one per kappa; b = A x_true, x_true seeded
     Gaussian.  value = total 2/3 N^3 over the kappa set / total solve time.
-    The plain FP64 LU (sla_dgesv) is timed on the first matrix beside it.
+    The plain FP64 LU (sla_dgesv) is timed on the first matrix beside it,
+    and every matrix's forward error is reported beside the FP64 LU's.
 --mode fp64: the plain FP64 PDGETRF + PDGETRS drop-in path (sla_dgesv) on A
     i.i.d. uniform[-1, 1), b = A * ones.
 --mode batched: 10,000 independent systems in one persistent launch
@@ -30,8 +34,9 @@ solve overwrites it) and does the whole factorization + solve (+ refinement).
 The line also carries
   roofline      the dominant kernel of the timed solves (most summed time):
                 algorithmic work of all its launches / their summed duration
-                (HIP events around each launch on the stream it runs on)
-                against the MI355X peak of its unit;
+                (for the trailing update the launches' own execution
+                intervals, measured inside the kernels; HIP events around
+                each launch otherwise) against the MI355X peak of its unit;
   cpu_baseline  the reference ScaLAPACK PDGETRF + PDGETRS (oracle/_ref,
                 built from the reference sources, MKL BLAS) on the host cores
                 for a bounded N = 16384 sample, same rate definition.
@@ -69,7 +74,7 @@ def parse():
     ap.add_argument("--nb", type=int, default=512)
     ap.add_argument("--mode", choices=["mixed", "fp64", "batched"], default="mixed")
     ap.add_argument("--batch", type=int, default=10000, help="batched mode: number of systems")
-    ap.add_argument("--policy", choices=["auto", "fixed"], default="fixed",
+    ap.add_argument("--policy", choices=["auto", "fixed"], default="auto",
                     help="mixed mode: auto = the learned selector (sla_dxgesv_auto) picks "
                          "precision and refinement per matrix inside the timed region; fixed = "
                          "--prec with classical IR (GMRES-IR from --gmres-from)")

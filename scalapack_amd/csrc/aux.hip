@@ -364,19 +364,6 @@ __global__ __launch_bounds__(256) void rfs_reduce_kernel(int n, int slices, cons
 }
 
 // w = |r| + nz eps w (+ safe1 where w <= safe2), pdgerfs.f:561-572.
-// y[i] = sum over slices of part[slice][0][i] (fixed order): the product of
-// rfs_partial_n_kernel / rfs_partial_t_kernel without a right-hand side.
-__global__ __launch_bounds__(256) void slice_sum_kernel(int n, int slices,
-                                                        const double *__restrict__ part,
-                                                        double *__restrict__ y)
-{
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    double s = 0.0;
-    for (int c = 0; c < slices; c++) s += part[(size_t)c * 2 * n + i];
-    y[i] = s;
-}
-
 // x[i] in [-1, 1): an integer hash of i (the power steps' fixed start).
 __global__ void hash_vector_kernel(int n, double *x)
 {
@@ -409,6 +396,145 @@ __global__ void scal_kernel(int n, double alpha, double *x)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < n) x[i] *= alpha;
+}
+
+// ---- the selector's fused feature passes -----------------------------------
+// Row pass (the block layout of rfs_partial_n_kernel: 256 rows x a 512-column
+// slice per block): per row and slice the sum of |a| and the product A v
+// (part[slice][0 / 1][row]), per block the largest |a| (bmax) and the sum of
+// squares as a (scale, sum) pair of LAPACK DLASSQ's kind (bssq[2 block]).
+__global__ __launch_bounds__(256) void feat_rows_kernel(int n, const double *__restrict__ A, int lda,
+                                                        const double *__restrict__ x,
+                                                        double *__restrict__ part,
+                                                        double *__restrict__ bmax,
+                                                        double *__restrict__ bssq)
+{
+    __shared__ double xs[RF_KC];
+    __shared__ double ws[3][4];
+    const int c0 = blockIdx.y * RF_KC, kc = min(RF_KC, n - c0);
+    for (int k = threadIdx.x; k < kc; k += 256) xs[k] = x[c0 + k];
+    __syncthreads();
+    const int i = blockIdx.x * RF_ROWS + threadIdx.x;
+    double asum = 0.0, acc = 0.0, amax = 0.0, scale = 0.0, ssq = 1.0;
+    if (i < n) {
+        const double *a = A + i + (size_t)c0 * lda;
+#pragma unroll 8
+        for (int k = 0; k < kc; k++) {
+            const double v = a[(size_t)k * lda], av = fabs(v);
+            asum += av;
+            acc = fma(v, xs[k], acc);
+            amax = fmax(amax, av);
+            if (av > 0.0) {
+                if (scale < av) {
+                    const double r = scale / av;
+                    ssq = 1.0 + ssq * r * r;
+                    scale = av;
+                } else {
+                    const double r = av / scale;
+                    ssq = fma(r, r, ssq);
+                }
+            }
+        }
+        double *P = part + (size_t)blockIdx.y * 2 * n;
+        P[i] = asum;
+        P[n + i] = acc;
+    }
+    // block maximum and (scale, ssq) pair, folded in a fixed order
+    for (int o = 32; o > 0; o >>= 1) {
+        const double m2 = __shfl_xor(amax, o), s2 = __shfl_xor(scale, o), q2 = __shfl_xor(ssq, o);
+        amax = fmax(amax, m2);
+        const double sm = fmax(scale, s2);
+        if (sm > 0.0) {
+            const double r1 = scale / sm, r2 = s2 / sm;
+            ssq = (scale > 0.0 ? ssq * r1 * r1 : 0.0) + (s2 > 0.0 ? q2 * r2 * r2 : 0.0);
+        }
+        scale = sm;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        ws[0][wave] = amax;
+        ws[1][wave] = scale;
+        ws[2][wave] = ssq;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = ws[0][0], sc = ws[1][0], q = ws[2][0];
+        for (int w = 1; w < 4; w++) {
+            m = fmax(m, ws[0][w]);
+            const double s2 = ws[1][w], sm = fmax(sc, s2);
+            if (sm > 0.0) {
+                const double r1 = sc / sm, r2 = s2 / sm;
+                q = (sc > 0.0 ? q * r1 * r1 : 0.0) + (s2 > 0.0 ? ws[2][w] * r2 * r2 : 0.0);
+            }
+            sc = sm;
+        }
+        const int b = blockIdx.y * gridDim.x + blockIdx.x;
+        bmax[b] = m;
+        bssq[2 * b] = sc;
+        bssq[2 * b + 1] = q;
+    }
+}
+
+// Column pass (the layout of rfs_partial_t_kernel: 64 columns x a 1024-row
+// slice per block, a wave reads down 16 columns): per column and slice the
+// sum of |a| and A^T y (part[slice][0 / 1][col]).
+__global__ __launch_bounds__(256) void feat_cols_kernel(int n, const double *__restrict__ A, int lda,
+                                                        const double *__restrict__ y,
+                                                        double *__restrict__ part)
+{
+    __shared__ double ys[RF_TROWS];
+    const int r0 = blockIdx.y * RF_TROWS, rs = min(RF_TROWS, n - r0);
+    for (int k = threadIdx.x; k < rs; k += 256) ys[k] = y[r0 + k];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double *P = part + (size_t)blockIdx.y * 2 * n;
+    for (int cc = 0; cc < RF_TCOLS / 4; cc++) {
+        const int col = blockIdx.x * RF_TCOLS + wave * (RF_TCOLS / 4) + cc;
+        if (col >= n) break;
+        const double *a = A + (size_t)col * lda + r0;
+        double asum = 0.0, acc = 0.0;
+#pragma unroll 4
+        for (int r = lane; r < rs; r += 64) {
+            const double v = a[r];
+            asum += fabs(v);
+            acc = fma(v, ys[r], acc);
+        }
+        asum = wave_sum(asum);
+        acc = wave_sum(acc);
+        if (lane == 0) {
+            P[col] = asum;
+            P[n + col] = acc;
+        }
+    }
+}
+
+// y[i] = sum over slices of part[slice][half][i] (fixed order).
+__global__ __launch_bounds__(256) void slice_sum2_kernel(int n, int slices, int half,
+                                                         const double *__restrict__ part,
+                                                         double *__restrict__ y)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int c = 0; c < slices; c++) s += part[(size_t)c * 2 * n + (size_t)half * n + i];
+    y[i] = s;
+}
+
+// Fold of the per-block (scale, ssq) pairs: res = scale sqrt(ssq) (one block).
+__global__ __launch_bounds__(64) void ssq_fold_kernel(int nb, const double *__restrict__ bssq,
+                                                      double *res)
+{
+    if (threadIdx.x != 0) return;
+    double sc = 0.0, q = 1.0;
+    for (int b = 0; b < nb; b++) {
+        const double s2 = bssq[2 * b], sm = fmax(sc, s2);
+        if (sm > 0.0) {
+            const double r1 = sc / sm, r2 = s2 / sm;
+            q = (sc > 0.0 ? q * r1 * r1 : 0.0) + (s2 > 0.0 ? bssq[2 * b + 1] * r2 * r2 : 0.0);
+        }
+        sc = sm;
+    }
+    *res = sc * sqrt(q);
 }
 
 __global__ void ferr_weights_kernel(int n, const double *__restrict__ r, double *__restrict__ w,
@@ -1093,78 +1219,85 @@ namespace {
 double lg(double x) { return std::log10(std::min(std::max(x, 1e-300), 1e300)); }
 }  // namespace
 
-// Rayleigh estimate of ||A||_2^2 from two power steps on A^T A started at the
-// hash vector v: z1 = A^T A v scaled to max |z1| = 1, z2 = A^T A z1,
-// lambda = (z1 . z2) / (z1 . z1).  Four passes over A.
-static double power_lambda(Handle &h, int n, const double *dA, int lda)
+// The selector's global features in four passes over A: a row pass (row
+// sums of |A|, max |a|, the sum of squares, y1 = A v), a column pass (column
+// sums of |A|, z1 = A^T y1), then y2 = A z1 and z2 = A^T y2 with z1 scaled
+// to max |z1| = 1; lambda = (z1 . z2) / (z1 . z1) is the Rayleigh estimate
+// of ||A||_2^2 of two power steps on A^T A started at the hash vector v.
+struct FusedNorms {
+    double n1, ni, nm, nf, rmin, cmin, lam;
+};
+
+static FusedNorms fused_norms(Handle &h, int n, const double *dA, int lda)
 {
-    const int sn = ceil_div(n, RF_KC), stn = ceil_div(n, RF_TROWS);
-    AuxWork a(h, n, (size_t)std::max(sn, stn) * 2 * n);
+    const int sn = ceil_div(n, RF_KC), stn = ceil_div(n, RF_TROWS), rb = ceil_div(n, RF_ROWS);
+    const int nblk = rb * sn;
+    const size_t pd = (size_t)std::max(sn, stn) * 2 * n;
+    AuxWork a(h, n, pd + 3 * (size_t)nblk);
+    double *bmax = a.part + pd, *bssq = bmax + nblk;
     hipStream_t s = h.stream;
-    auto mv = [&](bool trans, const double *in, double *out) {
-        if (!trans) {
-            hipLaunchKernelGGL(rfs_partial_n_kernel, dim3(ceil_div(n, RF_ROWS), sn), dim3(256), 0, s, n,
-                               dA, lda, in, a.part);
-            hipLaunchKernelGGL(slice_sum_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, sn, a.part,
-                               out);
-        } else {
-            hipLaunchKernelGGL(rfs_partial_t_kernel, dim3(ceil_div(n, RF_TCOLS), stn), dim3(256), 0, s,
-                               n, dA, lda, in, a.part);
-            hipLaunchKernelGGL(slice_sum_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, stn,
-                               a.part, out);
-        }
-        SLA_LAUNCH_CHECK();
+    const dim3 gs(ceil_div(n, 256));
+    auto sum2 = [&](int slices, int half, double *out) {
+        hipLaunchKernelGGL(slice_sum2_kernel, gs, dim3(256), 0, s, n, slices, half, a.part, out);
     };
     auto dot = [&](const double *x, const double *y, int slot) {
         const int nb = a.red_blocks(n);
         hipLaunchKernelGGL(dot1_kernel, dim3(nb), dim3(256), 0, s, n, x, y, a.red_v);
         hipLaunchKernelGGL(reduce2_kernel<RED_SUM>, dim3(1), dim3(256), 0, s, nb, a.red_v, 1.0,
                            a.res + slot);
-        SLA_LAUNCH_CHECK();
     };
-    hipLaunchKernelGGL(hash_vector_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, a.x);
-    mv(false, a.x, a.v);
-    mv(true, a.v, a.r);  // z1
-    a.maxabs(a.r, n, 0);
-    double zmax = 0.0;
-    a.fetch(&zmax, a.res, 8);
-    if (!(zmax > 0.0) || !std::isfinite(zmax)) return 0.0;
-    hipLaunchKernelGGL(scal_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, 1.0 / zmax, a.r);
-    mv(false, a.r, a.w);
-    mv(true, a.w, a.x);  // z2
+    hipLaunchKernelGGL(hash_vector_kernel, gs, dim3(256), 0, s, n, a.x);
+    // pass 1: rows
+    hipLaunchKernelGGL(feat_rows_kernel, dim3(rb, sn), dim3(256), 0, s, n, dA, lda, a.x, a.part, bmax,
+                       bssq);
+    sum2(sn, 0, a.w);  // row sums of |A|
+    sum2(sn, 1, a.v);  // y1
+    a.maxabs(a.w, n, 0);
+    a.minabs(a.w, n, 1);
+    a.maxabs(bmax, nblk, 2);
+    hipLaunchKernelGGL(ssq_fold_kernel, dim3(1), dim3(64), 0, s, nblk, bssq, a.res + 3);
+    // pass 2: columns
+    hipLaunchKernelGGL(feat_cols_kernel, dim3(ceil_div(n, RF_TCOLS), stn), dim3(256), 0, s, n, dA,
+                       lda, a.v, a.part);
+    sum2(stn, 0, a.w);  // column sums of |A|
+    sum2(stn, 1, a.r);  // z1
+    a.maxabs(a.w, n, 4);
+    a.minabs(a.w, n, 5);
+    a.maxabs(a.r, n, 6);
+    SLA_LAUNCH_CHECK();
+    double r[7];
+    a.fetch(r, a.res, sizeof r);
+    FusedNorms out{r[4], r[0], r[2], r[3], r[1], r[5], 0.0};
+    const double zmax = r[6];
+    if (!(zmax > 0.0) || !std::isfinite(zmax)) return out;
+    // passes 3 and 4: one more power step from z1 / max |z1|
+    hipLaunchKernelGGL(scal_kernel, gs, dim3(256), 0, s, n, 1.0 / zmax, a.r);
+    hipLaunchKernelGGL(rfs_partial_n_kernel, dim3(rb, sn), dim3(256), 0, s, n, dA, lda, a.r, a.part);
+    sum2(sn, 0, a.w);  // y2
+    hipLaunchKernelGGL(rfs_partial_t_kernel, dim3(ceil_div(n, RF_TCOLS), stn), dim3(256), 0, s, n, dA,
+                       lda, a.w, a.part);
+    sum2(stn, 0, a.x);  // z2
     dot(a.r, a.x, 0);
     dot(a.r, a.r, 1);
+    SLA_LAUNCH_CHECK();
     double d[2];
     a.fetch(d, a.res, sizeof d);
-    return d[1] > 0.0 ? d[0] / d[1] : 0.0;
+    out.lam = d[1] > 0.0 ? d[0] / d[1] : 0.0;
+    return out;
 }
 
 void selector_features_device(Handle &h, int n, const double *dA, int lda, float *f)
 {
     if (n < 1) throw std::invalid_argument("selector features: n >= 1");
     f[0] = (float)std::log10((double)n);
-    const double n1 = dlange_device(h, '1', n, n, dA, lda), ni = dlange_device(h, 'I', n, n, dA, lda);
-    const double nm = dlange_device(h, 'M', n, n, dA, lda), nf = dlange_device(h, 'F', n, n, dA, lda);
-    f[1] = (float)lg(n1);
-    f[2] = (float)lg(ni);
-    f[3] = (float)lg(nm);
+    const FusedNorms fn = fused_norms(h, n, dA, lda);
+    const double nf = fn.nf;
+    f[1] = (float)lg(fn.n1);
+    f[2] = (float)lg(fn.ni);
+    f[3] = (float)lg(fn.nm);
     f[4] = (float)lg(nf / std::sqrt((double)n));
-    {
-        AuxWork a(h, n, 0);
-        hipLaunchKernelGGL(lange_rowsum_kernel, dim3(ceil_div(n, 64)), dim3(64), 0, h.stream, n, n, dA, lda,
-                           a.x);
-        hipLaunchKernelGGL(lange_colsum_kernel, dim3(ceil_div(n, 64)), dim3(64), 0, h.stream, n, n, dA, lda,
-                           a.v);
-        SLA_LAUNCH_CHECK();
-        a.maxabs(a.x, n, 0);
-        a.minabs(a.x, n, 1);
-        a.maxabs(a.v, n, 2);
-        a.minabs(a.v, n, 3);
-        double r[4];
-        a.fetch(r, a.res, sizeof r);
-        f[5] = (float)(lg(r[0]) - lg(r[1]));
-        f[6] = (float)(lg(r[2]) - lg(r[3]));
-    }
+    f[5] = (float)(lg(fn.ni) - lg(fn.rmin));
+    f[6] = (float)(lg(fn.n1) - lg(fn.cmin));
     const int k = std::min(n, 128), S = n > 128 ? 4 : 1;
     const size_t kk = (size_t)k * k;
     const size_t bytes = 2 * S * kk * 8 + S * (size_t)k * 8 + S * (size_t)k * 4 + S * 4 +
@@ -1243,7 +1376,7 @@ void selector_features_device(Handle &h, int n, const double *dA, int lda, float
     // the stable-rank fraction ||A||_F^2 / (n ||A||_2^2): a global view of
     // how the singular values spread, which the sampled submatrices of a
     // large matrix cannot see
-    const double lam = power_lambda(h, n, dA, lda);
+    const double lam = fn.lam;
     f[11] = (float)(lam > 0.0 ? std::min(20.0, std::max(-20.0, lg(nf * nf / ((double)n * lam)))) : -20.0);
 }
 

@@ -14,12 +14,16 @@
 // empty; every workgroup leaves when the ticket passes the batch size.
 // Inside a system the factorization is blocked right-looking with W-column
 // panels:
-//   panel    columns kb .. kb+W-1 unblocked: pivot search as a wave-shuffle
-//            argmax folded across the four waves in LDS, row interchange over
-//            all n columns, reciprocal scaling, rank-1 update of the panel;
+//   panel    columns kb .. kb+W-1: while the panel has at most 256 RPT rows
+//            (RPT = 2 for FP64, 4 for FP32) it is held in registers and a
+//            column costs one workgroup barrier (panel_regs); taller panels
+//            run unblocked in memory (pivot search as a wave-shuffle argmax
+//            folded across the four waves in LDS, row interchange over all n
+//            columns, reciprocal scaling, rank-1 update, two barriers per
+//            column);
 //   U12      L11^{-1} A12, one thread per column of A12;
 //   update   A22 -= L21 U12 in 64 x 64 tiles, L21 / U12 staged through LDS,
-//            4 x 4 outputs per thread.
+//            the products on the 16 x 16 x 4 MFMA of the storage precision.
 // The solves take 64-row blocks: one wave solves the diagonal block in
 // registers (lane = row, values broadcast by lane reads), then all four
 // waves update the rows below (above) with the 64 new values.
@@ -29,11 +33,13 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <numeric>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 namespace sla {
@@ -42,6 +48,17 @@ namespace {
 
 constexpr int BT = 256;       // threads per workgroup
 constexpr int UT = 64;        // update tile edge
+
+// Kernel variants for comparisons: SLA_BATCHED_FLAGS (environment) sets the
+// flags of factor_system (default 3: register panels and MFMA updates).
+int batched_flags()
+{
+    static const int f = [] {
+        const char *e = getenv("SLA_BATCHED_FLAGS");
+        return e ? (atoi(e) & 3) : 3;
+    }();
+    return f;
+}
 
 struct SysDesc {
     int n, nrhs, lda, ldb;
@@ -66,13 +83,260 @@ __device__ __forceinline__ void argmax_fold3(T &v, int &i, T &sv, T v2, int i2, 
     }
 }
 
+// LDS of the register-resident panel: per column parity, the candidate row
+// of every wave (slots 0..3) and the row at the diagonal position (slot 4),
+// the wave maxima and their positions; the panel's pivots.
+template <typename T, int W>
+struct RegPanelLds {
+    T rows[2][5][W];
+    T val[2][4];
+    int pos[2][4];
+    int piv[W];
+};
+
+// Panel columns kb .. kb+w-1 of rows kb .. n-1 held in registers while they
+// are factored: thread t holds positions kb + t + 256 q (q < RPT), so this
+// path serves panels of at most 256 RPT rows.  Per column j: every thread
+// finds its largest |a| among the positions >= j (lowest position on ties),
+// a wave argmax, the owner of each wave's winner posts its whole panel row
+// and the owner of position j its row, ONE workgroup barrier, then every
+// thread folds the four posts in the same order: the owner of the pivot's
+// position takes the row from position j and the owner of position j the
+// pivot row (the interchange within the panel), the rows below j are scaled
+// by the reciprocal of the pivot and updated with the pivot row.  The
+// posts are double-buffered by column parity, so no second barrier is
+// needed.  The operations on every element are those of the unblocked
+// panel below (same interchange, reciprocal scaling, FMA), so both give the
+// same bits.  The interchanges reach the other columns after the panel,
+// replayed in order column by column.
+template <typename T, int W, int RPT>
+__device__ void panel_regs(int n, int kb, int w, T *__restrict__ a, int lda, int *__restrict__ piv,
+                           int &info, RegPanelLds<T, W> &sl)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    T v[RPT][W];
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+        const int r = kb + tid + BT * q;
+#pragma unroll
+        for (int c = 0; c < W; c++)
+            v[q][c] = (r < n && c < w) ? a[(size_t)r + (size_t)(kb + c) * lda] : T(0);
+    }
+#pragma unroll
+    for (int jj = 0; jj < W; jj++) {
+        if (jj < w) {  // uniform
+        const int j = kb + jj, pb = jj & 1;
+        // the largest |a| of column j among this thread's positions >= j
+        T best = T(-1);
+        int bi = INT_MAX;
+#pragma unroll
+        for (int q = 0; q < RPT; q++) {
+            const int r = kb + tid + BT * q;
+            const T x = fabs(v[q][jj]);
+            if (r >= j && r < n && x > best) {  // a NaN never wins
+                best = x;
+                bi = r;
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) argmax_fold(best, bi, __shfl_xor(best, o), __shfl_xor(bi, o));
+        if (bi != INT_MAX && (bi - kb) % BT == tid) {  // this thread holds the wave's winner
+            const int qw = (bi - kb) / BT;
+#pragma unroll
+            for (int q = 0; q < RPT; q++)
+                if (q == qw)
+#pragma unroll
+                    for (int c = 0; c < W; c++) sl.rows[pb][wave][c] = v[q][c];
+        }
+        if (tid == jj)  // position j is (thread jj, q = 0)
+#pragma unroll
+            for (int c = 0; c < W; c++) sl.rows[pb][4][c] = v[0][c];
+        if (lane == 0) {
+            sl.val[pb][wave] = best;
+            sl.pos[pb][wave] = bi;
+        }
+        __syncthreads();
+        T fv = sl.val[pb][0];
+        int fi = sl.pos[pb][0], fw = 0;
+#pragma unroll
+        for (int q = 1; q < 4; q++) {
+            const T v2 = sl.val[pb][q];
+            const int i2 = sl.pos[pb][q];
+            if (v2 > fv || (v2 == fv && i2 < fi)) {
+                fv = v2;
+                fi = i2;
+                fw = q;
+            }
+        }
+        // no comparable entry (all NaN): keep row j
+        const int p = fi < n ? fi : j;
+        const T *prow = fi < n ? sl.rows[pb][fw] : sl.rows[pb][4];
+        const T pv = prow[jj];
+        if (tid == 0) {
+            piv[j] = p + 1;
+            sl.piv[jj] = p;
+        }
+        if (pv != T(0)) {
+            if (p != j) {
+                if ((p - kb) % BT == tid) {
+                    const int qp = (p - kb) / BT;
+#pragma unroll
+                    for (int q = 0; q < RPT; q++)
+                        if (q == qp)
+#pragma unroll
+                            for (int c = 0; c < W; c++) v[q][c] = sl.rows[pb][4][c];
+                }
+                if (tid == jj)
+#pragma unroll
+                    for (int c = 0; c < W; c++) v[0][c] = prow[c];
+            }
+            const T rcp = T(1) / pv;
+#pragma unroll
+            for (int q = 0; q < RPT; q++) {
+                const int r = kb + tid + BT * q;
+                if (r > j && r < n) v[q][jj] = v[q][jj] * rcp;
+            }
+        } else if (info == 0) {
+            info = j + 1;
+        }
+        // rank-1 update of the panel's columns right of j
+#pragma unroll
+        for (int q = 0; q < RPT; q++) {
+            const int r = kb + tid + BT * q;
+            if (r > j && r < n) {
+                const T l = v[q][jj];
+                // constant bounds, so the inner loop unrolls before the outer
+#pragma unroll
+                for (int c = 1; c < W; c++)
+                    if (c > jj && c < w) v[q][c] = fma(-l, prow[c], v[q][c]);
+            }
+        }
+        }  // jj < w
+    }
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+        const int r = kb + tid + BT * q;
+        if (r < n)
+#pragma unroll
+            for (int c = 0; c < W; c++)
+                if (c < w) a[(size_t)r + (size_t)(kb + c) * lda] = v[q][c];
+    }
+    __syncthreads();  // sl.piv complete
+    // the panel's interchanges, in order, on every other column
+    for (int c = tid; c < n; c += BT) {
+        if (c >= kb && c < kb + w) continue;
+        T *cc = a + (size_t)c * lda;
+        for (int jj = 0; jj < w; jj++) {
+            const int p = sl.piv[jj];
+            if (p != kb + jj) {
+                const T t = cc[kb + jj];
+                cc[kb + jj] = cc[p];
+                cc[p] = t;
+            }
+        }
+    }
+}
+
+// C[rows x cols] -= Ls[0:rows, 0:w] Us[0:w, 0:cols] for one 64 x 64 tile on
+// the 16 x 16 x 4 MFMA of the storage precision: wave v owns rows
+// 32 (v & 1) .., columns 32 (v >> 1) .. as 2 x 2 blocks.  w is a multiple of
+// 4 (every update follows a full panel of width W in {8, 16, 32}).
+// D layouts: f64 16x16x4, element r of lane l = D[l/16 + 4 r][l % 16];
+// f32 16x16x4, element r of lane l = D[4 (l/16) + r][l % 16].
+typedef double bd4_t __attribute__((ext_vector_type(4)));
+typedef float bf4_t __attribute__((ext_vector_type(4)));
+
+template <typename T, int W>
+__device__ __forceinline__ void tile_update_mfma(int w, int rows, int cols, const T (*Ls)[W + 1],
+                                                 const T (*Us)[UT + 1], T *__restrict__ C, int ldc)
+{
+    using acc_t = typename std::conditional<sizeof(T) == 8, bd4_t, bf4_t>::type;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wr = (wave & 1) * 32, wc = (wave >> 1) * 32, lr = lane & 15, lk = lane >> 4;
+    acc_t acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = acc_t{0, 0, 0, 0};
+    for (int k0 = 0; k0 < w; k0 += 4) {
+        T af[2], bf[2];
+#pragma unroll
+        for (int i = 0; i < 2; i++) af[i] = Ls[wr + 16 * i + lr][k0 + lk];
+#pragma unroll
+        for (int j = 0; j < 2; j++) bf[j] = Us[k0 + lk][wc + 16 * j + lr];
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                if constexpr (sizeof(T) == 8)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+                else
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i], bf[j], acc[i][j], 0, 0, 0);
+            }
+    }
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const int c = wc + 16 * j + lr;
+        if (c >= cols) continue;
+        T *cc = C + (size_t)c * ldc;
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = wr + 16 * i + (sizeof(T) == 8 ? lk + 4 * r : 4 * lk + r);
+                if (row < rows) cc[row] -= acc[i][j][r];
+            }
+    }
+}
+
+// The same with 4 x 4 outputs per thread on vector FMAs (thread t: rows
+// 4 (t % 16) .., columns 4 (t / 16) ..).
+template <typename T, int W>
+__device__ __forceinline__ void tile_update_valu(int w, int rows, int cols, const T (*Ls)[W + 1],
+                                                 const T (*Us)[UT + 1], T *__restrict__ C, int ldc)
+{
+    const int tr = threadIdx.x & 15, tc = threadIdx.x >> 4;
+    T acc[4][4];
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++) acc[x][y] = T(0);
+    for (int k = 0; k < w; k++) {
+        T l[4], u[4];
+#pragma unroll
+        for (int x = 0; x < 4; x++) l[x] = Ls[4 * tr + x][k];
+#pragma unroll
+        for (int y = 0; y < 4; y++) u[y] = Us[k][4 * tc + y];
+#pragma unroll
+        for (int x = 0; x < 4; x++)
+#pragma unroll
+            for (int y = 0; y < 4; y++) acc[x][y] = fma(l[x], u[y], acc[x][y]);
+    }
+#pragma unroll
+    for (int y = 0; y < 4; y++) {
+        const int c = 4 * tc + y;
+        if (c < cols) {
+            T *cc = C + (size_t)c * ldc;
+#pragma unroll
+            for (int x = 0; x < 4; x++)
+                if (4 * tr + x < rows) cc[4 * tr + x] -= acc[x][y];
+        }
+    }
+}
+
+// flags: bit 0 -- panels of at most 256 RPT rows in registers (panel_regs);
+// bit 1 -- the trailing-update tiles on MFMA (else vector FMAs).
 template <typename T, int W>
 __device__ void factor_system(int n, T *__restrict__ a, int lda, int *__restrict__ piv, int &info,
-                              T (*Ls)[W + 1], T (*Us)[UT + 1], T *s_v, int *s_i)
+                              T (*Ls)[W + 1], T (*Us)[UT + 1], T *s_v, int *s_i,
+                              RegPanelLds<T, W> &rp, int flags)
 {
+    constexpr int RPT = sizeof(T) == 8 ? 2 : 4;  // rows per thread of the register panel
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int kb = 0; kb < n; kb += W) {
         const int w = min(W, n - kb);
+        if ((flags & 1) && n - kb <= BT * RPT) {
+            panel_regs<T, W, RPT>(n, kb, w, a, lda, piv, info, rp);
+        } else {
         // ---- panel (unblocked), columns kb .. kb+w-1.  Thread t owns rows
         // j+1+t, j+1+t+256, ... of every step, so the rank-1 update of
         // column j and the pivot search of column j+1 run in one pass over
@@ -155,6 +419,7 @@ __device__ void factor_system(int n, T *__restrict__ a, int lda, int *__restrict
             // row j+1 is owned by thread 0 in this pass: A(j+1, j+1) is final
             if (tid == 0 && j + 1 < cend && j + 1 < n) s_v[4] = a[(size_t)(j + 1) * (lda + 1)];
         }
+        }  // panel
         __syncthreads();  // the panel is final before U12 reads L11
         const int c0 = kb + w;
         if (c0 >= n) break;
@@ -167,9 +432,9 @@ __device__ void factor_system(int n, T *__restrict__ a, int lda, int *__restrict
             }
         }
         __syncthreads();
-        // ---- A22 -= L21 U12 in 64 x 64 tiles
+        // ---- A22 -= L21 U12 in 64 x 64 tiles (L21 / U12 slices staged in LDS,
+        // the products on MFMA)
         const int m2 = n - c0;
-        const int tr = tid & 15, tc = tid >> 4;  // 4 x 4 outputs at rows 4tr.., cols 4tc..
         for (int t0 = 0; t0 < m2; t0 += UT)
             for (int u0 = 0; u0 < m2; u0 += UT) {
                 const int rows = min(UT, m2 - t0), cols = min(UT, m2 - u0);
@@ -182,32 +447,12 @@ __device__ void factor_system(int n, T *__restrict__ a, int lda, int *__restrict
                     Us[k][c] = c < cols ? a[(size_t)(kb + k) + (size_t)(c0 + u0 + c) * lda] : T(0);
                 }
                 __syncthreads();
-                T acc[4][4];
-#pragma unroll
-                for (int x = 0; x < 4; x++)
-#pragma unroll
-                    for (int y = 0; y < 4; y++) acc[x][y] = T(0);
-                for (int k = 0; k < w; k++) {
-                    T l[4], u[4];
-#pragma unroll
-                    for (int x = 0; x < 4; x++) l[x] = Ls[4 * tr + x][k];
-#pragma unroll
-                    for (int y = 0; y < 4; y++) u[y] = Us[k][4 * tc + y];
-#pragma unroll
-                    for (int x = 0; x < 4; x++)
-#pragma unroll
-                        for (int y = 0; y < 4; y++) acc[x][y] = fma(l[x], u[y], acc[x][y]);
-                }
-#pragma unroll
-                for (int y = 0; y < 4; y++) {
-                    const int c = 4 * tc + y;
-                    if (c < cols) {
-                        T *cc = a + (size_t)(c0 + u0 + c) * lda + c0 + t0;
-#pragma unroll
-                        for (int x = 0; x < 4; x++)
-                            if (4 * tr + x < rows) cc[4 * tr + x] -= acc[x][y];
-                    }
-                }
+                if (flags & 2)
+                    tile_update_mfma<T, W>(w, rows, cols, Ls, Us,
+                                           a + (size_t)(c0 + u0) * lda + c0 + t0, lda);
+                else
+                    tile_update_valu<T, W>(w, rows, cols, Ls, Us,
+                                           a + (size_t)(c0 + u0) * lda + c0 + t0, lda);
                 __syncthreads();  // Ls / Us are reloaded by the next tile
             }
     }
@@ -279,13 +524,15 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int batch, const int *
                                                           T *const *__restrict__ A,
                                                           int *const *__restrict__ ipiv,
                                                           T *const *__restrict__ B,
-                                                          int *__restrict__ info, int *queue)
+                                                          int *__restrict__ info, int *queue,
+                                                          int flags)
 {
     __shared__ T Ls[UT][W + 1];
     __shared__ T Us[W][UT + 1];
     __shared__ T s_v[9];  // [0..3] wave maxima, [4] A(j, j), [5..8] signed values
     __shared__ int s_i[4];
     __shared__ int s_q;
+    __shared__ RegPanelLds<T, W> rp;
     for (;;) {
         if (threadIdx.x == 0) s_q = atomicAdd(queue, 1);
         __syncthreads();
@@ -298,7 +545,7 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int batch, const int *
         int *piv = ipiv[s];
         // n < 0 marks a system whose sizes the allocation-free entry rejected
         int inf = d.n < 0 ? -1 : 0;
-        if (d.n > 0) factor_system<T, W>(d.n, a, d.lda, piv, inf, Ls, Us, s_v, s_i);
+        if (d.n > 0) factor_system<T, W>(d.n, a, d.lda, piv, inf, Ls, Us, s_v, s_i, rp, flags);
         if (threadIdx.x == 0) info[s] = inf;
         if (inf == 0 && d.n > 0)
             for (int r = 0; r < d.nrhs; r++) solve_system<T>(d.n, a, d.lda, piv, B[s] + (size_t)r * d.ldb);
@@ -424,7 +671,7 @@ void launch(const BatchedPlan &p, int w, int wgs, T *const *A, int *const *ipiv,
     hipLaunchKernelGGL(zero_queue_kernel, dim3(1), dim3(64), 0, s, p.d_queue);
 #define SLA_BATCHED(WV)                                                                          \
     hipLaunchKernelGGL((gesv_batched_kernel<T, WV>), dim3(wgs), dim3(BT), 0, s, p.batch,         \
-                       p.d_order, p.d_desc, A, ipiv, B, info, p.d_queue)
+                       p.d_order, p.d_desc, A, ipiv, B, info, p.d_queue, batched_flags())
     switch (w) {
     case 8: SLA_BATCHED(8); break;
     case 16: SLA_BATCHED(16); break;
@@ -539,7 +786,7 @@ void gesv_batched_noalloc_device(Handle &h, int batch, const int *dn, const int 
     const int wgs = std::max(1, h.batched_wgs_per_cu) * h.num_cus;
 #define SLA_BATCHED(WV)                                                                          \
     hipLaunchKernelGGL((gesv_batched_kernel<T, WV>), dim3(wgs), dim3(BT), 0, h.stream, batch,     \
-                       order, desc, dA, dipiv, dB, dinfo, queue)
+                       order, desc, dA, dipiv, dB, dinfo, queue, batched_flags())
     switch (h.batched_w) {
     case 8: SLA_BATCHED(8); break;
     case 16: SLA_BATCHED(16); break;
