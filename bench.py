@@ -178,9 +178,12 @@ KERNEL_OF = {  # kernel symbol behind each timing class (for matching kernel tra
     ("gemm", "fp32"): "sgemm_nn_sub_direct_kernel<1> (trailing update, FP32 MFMA)",
     ("gemm", "bf16"): "gemm_bf16_nn_sub_direct_kernel (trailing update, BF16 MFMA)",
     ("gemm", "fp8"): "gemm_fp8_nn_sub_direct_kernel (trailing update, FP8 MFMA)",
-    ("gemm", "bf16x3"): "gemm_split_nn_sub_direct_kernel<__bf16, 2> (trailing update, BF16x3)",
-    ("gemm", "bf16x6"): "gemm_split_nn_sub_direct_kernel<__bf16, 3> (trailing update, BF16x6)",
-    ("gemm", "fp16x3"): "gemm_split_nn_sub_direct_kernel<_Float16, 2> (trailing update, FP16x3)",
+    ("gemm", "bf16x3"): "gemm_split_lds_kernel<__bf16, 2> (trailing update, BF16x3; "
+                        "tile borders by gemm_split_nn_sub_direct_kernel)",
+    ("gemm", "bf16x6"): "gemm_split_lds_kernel<__bf16, 3> (trailing update, BF16x6; "
+                        "tile borders by gemm_split_nn_sub_direct_kernel)",
+    ("gemm", "fp16x3"): "gemm_split_lds_kernel<_Float16, 2> (trailing update, FP16x3; "
+                        "tile borders by gemm_split_nn_sub_direct_kernel)",
     ("panel", "fp64"): "panel_leaf_kernel<double, 32, RPT>",
 }
 
@@ -495,7 +498,7 @@ def main() -> int:
             "scaled_residual": max(sres), "forward_error": max(fwd),
             "kernel_ms_per_step": {k: round(v["ms"] / args.steps, 2) for k, v in stats.items()},
             "roofline": roofline_from(stats, args.steps, "fp64",
-                                      {"mode": "fp64", "n": n, "nb": nb}),
+                                      {"n": n, "nb": nb, "prec": "fp64"}),
         })
         ok = max(sres) < 16.0
     else:
@@ -564,7 +567,7 @@ def main() -> int:
                                                     for k, v in fstats.items()},
                              "gemm_roofline": roofline_from({"gemm": fstats["gemm"]},
                                                             args.steps, "fp64",
-                                                            {"mode": "fp64", "n": n, "nb": nb})}
+                                                            {"n": n, "nb": nb, "prec": "fp64"})}
             del AT, B0
         ms = 1e3 * total_t / total_steps
         value = flops / (ms / 1e3) / 1e12
@@ -596,7 +599,11 @@ def main() -> int:
             "speedup_vs_fp64_lu": (round(value / fp64_line["tflops_equiv"], 3)
                                    if fp64_line else None),
             "roofline": roofline_from(all_stats[dom_prec], steps_by_prec[dom_prec], dom_prec,
-                                      {"mode": "mixed", "n": n, "nb": nb, "prec": dom_prec}),
+                                      {"n": n, "nb": nb, "prec": dom_prec}),
+            # the same for the solves of each factorization precision
+            "roofline_by_precision": {gp: roofline_from(st, steps_by_prec[gp], gp,
+                                                        {"n": n, "nb": nb, "prec": gp})
+                                      for gp, st in all_stats.items()},
         })
     line["cpu_baseline"] = None if args.no_cpu else cpu_baseline(args.cpu_n)
     print(json.dumps(line), flush=True)
