@@ -144,7 +144,10 @@ int sla_set_stream(sla_handle_t h, void *stream);
  * for callers timing the low-precision attempt alone), "trsv_blockinv" (0/1,
  * default 1: solves with low-precision factors multiply by the inverses of
  * their 128 x 128 diagonal blocks, formed once per factorization; 0:
- * substitution), "gmres_m" (GMRES steps per correction, default 50),
+ * substitution), "trsm_mfma" (0/1, default 1: the block row U12 <- L11^{-1} A12
+ * of every block column is one launch with the 32 x 32 diagonal blocks
+ * applied by their inverses and the products on MFMA; 0: recursive 32-row
+ * substitution strips and GEMMs), "gmres_m" (GMRES steps per correction, default 50),
  * "gmres_tol_exp" (e: relative reduction 10^-e per correction, default 6),
  * "pivoting" (FP32-storage factorizations: 0 partial pivoting, 1 tournament
  * pivoting of the panel leaves, default 1; the FP64 path always pivots as

@@ -983,7 +983,7 @@ int sla_set_param(sla_handle_t h, const char *name, int value)
                                   "pivoting", "lookahead", "update_cus", "lacon_lapack",
                                   "batched_w", "batched_wgs_per_cu", "batched_graph",
                                   "update_lds_kb", "ir_accept_stall", "trsv_blockinv",
-                                  "ir_fallback"};
+                                  "ir_fallback", "trsm_mfma"};
     if (std::find_if(std::begin(known), std::end(known), [&](const char *x) { return k == x; }) ==
         std::end(known)) {
         g_last_error = "sla_set_param: unknown parameter " + k;
@@ -1001,6 +1001,7 @@ int sla_set_param(sla_handle_t h, const char *name, int value)
     else if (k == "ir_gmres_on_stall") H->ir_gmres_on_stall = value != 0;
     else if (k == "ir_accept_stall") H->ir_accept_stall = value != 0;
     else if (k == "trsv_blockinv" && (value == 0 || value == 1)) H->trsv_blockinv = value;
+    else if (k == "trsm_mfma" && (value == 0 || value == 1)) H->trsm_mfma = value;
     else if (k == "ir_fallback" && (value == 0 || value == 1)) H->ir_fallback = value;
     else if (k == "pivoting" && (value == 0 || value == 1)) H->pivoting = value;
     else if (k == "lookahead" && (value == 0 || value == 1)) H->lookahead = value;
@@ -1032,6 +1033,7 @@ int sla_get_param(sla_handle_t h, const char *name, int *value)
     else if (k == "ir_gmres_on_stall") *value = H->ir_gmres_on_stall;
     else if (k == "ir_accept_stall") *value = H->ir_accept_stall;
     else if (k == "trsv_blockinv") *value = H->trsv_blockinv;
+    else if (k == "trsm_mfma") *value = H->trsm_mfma;
     else if (k == "ir_fallback") *value = H->ir_fallback;
     else if (k == "pivoting") *value = H->pivoting;
     else if (k == "lookahead") *value = H->lookahead;

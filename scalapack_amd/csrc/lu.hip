@@ -70,12 +70,8 @@ Handle::Handle()
     pair_count = ctl + 4;
     bad_pivot = ctl + 5;
 
-    constexpr int GMAX = 256;
-    slot_cand_v = alloc_dev<double>(2 * GMAX);
-    slot_cand_r = alloc_dev<int>(2 * GMAX);
-    slot_rows = alloc_dev<double>(2 * GMAX * 32);
-    slot_rowj = alloc_dev<double>(2 * 32);
-    slot_gmax = GMAX;
+    panel_mail = alloc_dev<uint64_t>(PANEL_MAIL_WORDS);
+    fill_u32(reinterpret_cast<unsigned *>(panel_mail), (int)(2 * PANEL_MAIL_WORDS), ~0u, stream);
 
     pairs = alloc_dev<int>(2 * 2048);
     pair_counts = alloc_dev<int>(2);
@@ -90,8 +86,7 @@ Handle::~Handle()
 {
     (void)hipStreamSynchronize(stream);
     batched_plan_free(batched);
-    for (void *p : {(void *)ctl, (void *)slot_cand_v, (void *)slot_cand_r, (void *)slot_rows,
-                    (void *)slot_rowj, (void *)pairs, (void *)pair_counts,
+    for (void *p : {(void *)ctl, (void *)panel_mail, (void *)pairs, (void *)pair_counts,
                     (void *)perm, (void *)work, (void *)kstamps})
         if (p) hipFree(p);
     for (auto &b : bufs)
@@ -109,11 +104,8 @@ template <typename T>
 PanelWork<T> Handle::panel_work()
 {
     PanelWork<T> pw;
-    pw.slots.cand_v = slot_cand_v;
-    pw.slots.cand_r = slot_cand_r;
-    pw.slots.rows = reinterpret_cast<T *>(slot_rows);
-    pw.slots.rowj = reinterpret_cast<T *>(slot_rowj);
-    pw.slots.gmax = slot_gmax;
+    pw.mail = panel_mail;
+    pw.mail_state = &panel_mail_state;
     pw.ctr = ctr;
     pw.abort_flag = abort_flag;
     pw.zero_col = zero_col;
@@ -267,6 +259,8 @@ struct Lu {
     int schur = SCHUR_NATIVE;  // precision of the outer trailing update
     bool l_packed = false;     // L21 of the current block column is packed
     hipStream_t s_main = nullptr, s_panel = nullptr;  // update stream, panel stream
+    const T *inv_of[2] = {nullptr, nullptr};  // L11 whose inverses each stream's area holds
+    int inv_m[2] = {0, 0};
 
     T *at(int i, int j) const { return A + (size_t)i + (size_t)j * lda; }
 
@@ -299,13 +293,31 @@ struct Lu {
     // MFMA update on operands packed once per block column.
     void update(int m, int n, int k, const T *l21, const T *u12, T *c, hipStream_t s);
 
-    // B[m x n] <- L^{-1} B for the unit lower m x m L at l: recursive halving
-    // down to <= 32-row diagonal solves (or one strip launch for a small block
-    // inside the panel), the off-diagonal products on the GEMMs (the 32-row
-    // blocking mirrors PB_CptrsmAB.c:359-412).
+    // B[m x n] <- L^{-1} B for the unit lower m x m L at l.  With trsm_mfma
+    // (default) every block of 33 .. 512 rows is one blocked launch (32-row
+    // diagonal blocks by their inverses, the products on MFMA); otherwise
+    // recursive halving down to <= 32-row diagonal solves (or one strip
+    // launch for a small block inside the panel), the off-diagonal products
+    // on the GEMMs (the 32-row blocking mirrors PB_CptrsmAB.c:359-412).
     void trsm(int m, int n, const T *l, T *b, hipStream_t s)
     {
         if (m <= 0 || n <= 0) return;
+        if (h.trsm_mfma && m > 32 && m <= 512) {
+            // one inverse area per stream (the panel stream's solves overlap
+            // the main stream's)
+            T *inv = static_cast<T *>(h.buffer(BUF_TRSM_INV, 2 * trsm_inv_elems(512) * sizeof(T)));
+            const int slot = s != s_main;
+            if (slot) inv += trsm_inv_elems(512);
+            // the block row's two solves (next block column, then the rest)
+            // share L11: the second reuses the inverses
+            const bool ready = inv_of[slot] == l && inv_m[slot] == m;
+            inv_of[slot] = l;
+            inv_m[slot] = m;
+            h.timing_begin(CAT_TRSM, s);
+            trsm_llnu_blocked<T>(m, n, l, lda, b, lda, inv, ready, s);
+            h.timing_end((double)m * m * n, s);
+            return;
+        }
         // <= 32 rows, or a small block inside the panel: one strip launch;
         // for FP32 storage also up to 1024 columns (the next block column's
         // U12 then costs 3 launches instead of 31 -- it is on the critical
@@ -422,7 +434,7 @@ template <>
 unsigned Lu<double>::leaf(double *a, int m, int w, int lcols, int rcols, int row0, int col0,
                           hipStream_t s)
 {
-    return panel_leaf<double>(a, lda, m, w, lcols, rcols, row0, col0, ipiv, pw, h.ctr_base, s);
+    return panel_leaf<double>(a, lda, m, w, lcols, rcols, row0, col0, ipiv, pw, s);
 }
 
 template <>
@@ -438,7 +450,7 @@ unsigned Lu<float>::leaf(float *a, int m, int w, int lcols, int rcols, int row0,
         return panel_tslu<float>(a, lda, m, w, lcols, rcols, row0, col0, ipiv, pw, slots,
                                  h.ctr_base, s);
     }
-    return panel_leaf<float>(a, lda, m, w, lcols, rcols, row0, col0, ipiv, pw, h.ctr_base, s);
+    return panel_leaf<float>(a, lda, m, w, lcols, rcols, row0, col0, ipiv, pw, s);
 }
 
 template <>
@@ -538,6 +550,7 @@ static int getrf_device(Handle &h, int m, int n, T *dA, int lda, int *dipiv, int
     if (m <= 0 || n <= 0) return 0;
     const int mn = std::min(m, n);
     h.ensure_perm(m);
+    if (h.trsm_mfma) h.buffer(BUF_TRSM_INV, 2 * trsm_inv_elems(512) * sizeof(T));
     h.ctr_base = 0;
     fill_int(h.ctl, 6, 0, h.stream);         // counter, flags
     fill_int(h.zero_col, 1, INT_MAX, h.stream);

@@ -106,6 +106,7 @@ enum BufId {
     BUF_MODEL,       // the selector's parameters (sla_selector_set_model)
     BUF_TRSV_MAIL,   // epoch-tagged hand-off words of the persistent TRSV
     BUF_DINV,        // inverses of the diagonal blocks of the low-precision factors
+    BUF_TRSM_INV,    // inverses of the 32 x 32 diagonal blocks of L11 (two streams)
     BUF_COUNT
 };
 
@@ -141,10 +142,8 @@ struct Handle {
     int *bad_pivot = nullptr;
     unsigned ctr_base = 0;
 
-    double *slot_cand_v = nullptr;
-    int *slot_cand_r = nullptr;
-    double *slot_rows = nullptr, *slot_rowj = nullptr;  // sized for doubles, reused for floats
-    int slot_gmax = 0;
+    uint64_t *panel_mail = nullptr;     // the panel leaf's mailbox (PANEL_MAIL_WORDS)
+    PanelMailState panel_mail_state;    // its last launch
 
     int *pairs = nullptr;        // two pair lists of 2048 ints
     int *pair_counts = nullptr;  // their counts
@@ -197,6 +196,7 @@ struct Handle {
                                  // (0: only DSGESV's convergence test ends it)
     int trsv_blockinv = 1;       // solves with low-precision factors use the inverses of
                                  // their 128 x 128 diagonal blocks (0: substitution)
+    int trsm_mfma = 1;           // U12 solves: one blocked MFMA launch per block (0: strips)
     int gmres_m = 50;            // GMRES steps per correction
     double gmres_tol = 1e-6;     // relative residual reduction per correction
 

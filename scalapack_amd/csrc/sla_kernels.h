@@ -77,20 +77,29 @@ void convert_f64_f32(int m, int n, const double *A, int lda, float *B, int ldb, 
                      hipStream_t s);
 
 // ---- panel leaf (PDGETF2 on <= 32 columns) --------------------------------
-template <typename T>
-struct PanelSlots {
-    double *cand_v;  // [2][gmax] candidate |pivot|
-    int *cand_r;     // [2][gmax] candidate row (leaf-relative)
-    T *rows;         // [2][gmax][32] candidate rows
-    T *rowj;         // [2][32] row j
-    int gmax;
+// Mailbox of the leaf's per-column exchange: 2 regions x 32 column blocks of
+// PANEL_MAIL_COL 64-bit words -- PANEL_MAIL_GMAX candidates of up to 33 words
+// (row index, 32 values), then the 32 words of row j at PANEL_MAIL_ROWJ.
+// Every word holds the empty pattern (all ones) until a launch writes it.
+constexpr int PANEL_MAIL_GMAX = 128;
+constexpr int PANEL_MAIL_ROWJ = PANEL_MAIL_GMAX * 33;
+constexpr int PANEL_MAIL_COL = PANEL_MAIL_ROWJ + 32;
+constexpr size_t PANEL_MAIL_REGION = 32 * (size_t)PANEL_MAIL_COL;
+constexpr size_t PANEL_MAIL_WORDS = 2 * PANEL_MAIL_REGION;
+
+// Host-side record of the mailbox's last launch (whose words the next
+// launch empties) and the launch count (whose parity picks the region).
+struct PanelMailState {
+    unsigned launches = 0;
+    int prev_wa = 0, prev_g = 0, prev_cs = 0;
 };
 
 template <typename T>
 struct PanelWork {
-    PanelSlots<T> slots;
+    uint64_t *mail;             // PANEL_MAIL_WORDS words, see above
+    PanelMailState *mail_state; // host record shared by every leaf of the handle
     unsigned *ctr;       // monotonic arrival counter of the device-wide barrier
-    int *abort_flag;     // set by a barrier that waited past spin_limit
+    int *abort_flag;     // set by a wait that lasted past spin_limit
     int *zero_col;       // min over zero pivots of the 1-based column (INT_MAX: none)
     uint64_t spin_limit; // wall-clock ticks a barrier may wait
     int target_wgs;      // preferred number of workgroups per leaf launch
@@ -98,13 +107,13 @@ struct PanelWork {
 
 // Factors the m x wa leaf at A (global rows row0.., columns col0.. relative to
 // the factored matrix) in place; writes ipiv[row0 .. row0+wa-1] as 1-based
-// global rows and applies each interchange also to the lcols columns left
+// global rows and applies the interchanges also to the lcols columns left
 // and rcols columns right of the leaf (the rest of its block column).
-// Returns the number of barrier arrivals the launch adds to *ctr (the
-// caller's next `base`).
+// Launches on one handle are ordered (one stream at a time).  Returns the
+// barrier arrivals the launch adds to *ctr: none.
 template <typename T>
 unsigned panel_leaf(T *A, int lda, int m, int wa, int lcols, int rcols, int row0, int col0,
-                    int *ipiv, const PanelWork<T> &pw, unsigned base, hipStream_t s);
+                    int *ipiv, const PanelWork<T> &pw, hipStream_t s);
 
 // ---- panel leaf with tournament pivoting (low-precision factorization) -------
 template <typename T>
@@ -148,6 +157,13 @@ void laswp_cols(double *A, int lda, int nrows, int k1, int k2, const int *ipiv, 
 // (PDTRSM 'L','L','N','U').
 template <typename T>
 void trsm_llnu(int m, int n, const T *L, int ldl, T *B, int ldb, hipStream_t s);
+// The same for m <= 512 with the products on MFMA (inverses of the 32 x 32
+// diagonal blocks, then block substitution); inv: trsm_inv_elems(m) of T,
+// already holding the inverses of this L when inv_ready.
+constexpr size_t trsm_inv_elems(int m) { return (size_t)((m + 31) / 32) * 1024; }
+template <typename T>
+void trsm_llnu_blocked(int m, int n, const T *L, int ldl, T *B, int ldb, T *inv, bool inv_ready,
+                       hipStream_t s);
 // One launch per triangle for n x nrhs right-hand sides (nrhs <= 16), with L
 // (unit lower) and U (upper, non-unit) stored together as PDGETRF leaves them:
 // X <- L^{-1} X, U^{-1} X, U^{-T} X or L^{-T} X.

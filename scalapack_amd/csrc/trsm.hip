@@ -23,6 +23,8 @@
 #include "sla_common.h"
 #include "sla_kernels.h"
 
+#include <type_traits>
+
 namespace sla {
 
 namespace {
@@ -536,6 +538,170 @@ void trsm_llnu(int m, int n, const T *L, int ldl, T *B, int ldb, hipStream_t s)
 }
 template void trsm_llnu<double>(int, int, const double *, int, double *, int, hipStream_t);
 template void trsm_llnu<float>(int, int, const float *, int, float *, int, hipStream_t);
+
+// ---------------------------------------------------------------------------
+// trsm_llnu_blocked: the same solve for m <= 512 rows in one launch (plus one
+// for the inverses, which a second solve with the same L11 reuses), the
+// products on MFMA.  L11 is taken in 32-row blocks; the inverse of each unit
+// lower 32 x 32 diagonal block is formed by trsm_diag_inv_kernel (thread =
+// element, forward substitution against the identity, one barrier per row;
+// a block cut by m is completed with the identity).  Each workgroup (8
+// waves) owns 16 columns of B, held in LDS, and for block k
+//   X_k = inv(L_kk) B_k                     (waves 0, 1: a 16-row tile each)
+//   B_i -= L_ik X_k  for the rows below     (16-row tiles, two per wave at a
+//                                            time so 16 loads are in flight)
+// with v_mfma_f64_16x16x4f64 (FP32: v_mfma_f32_16x16x4f32), L read from
+// global memory (L2-resident across the workgroups), X_k from LDS; the next
+// block's inverse operands are loaded during the current block's update.
+namespace {
+
+constexpr int TB_M = 512, TB_NC = 16, TB_LD = TB_NC + 1, TB_T = 512;
+typedef double bd4_t __attribute__((ext_vector_type(4)));
+typedef float bf4_t __attribute__((ext_vector_type(4)));
+
+template <typename T>
+__global__ __launch_bounds__(1024) void trsm_diag_inv_kernel(int m, const T *__restrict__ L,
+                                                             int ldl, T *__restrict__ inv)
+{
+    // thread (i, j) holds element (i, j) of the inverse; step k subtracts
+    // D_ik x_kj from the rows below k (forward substitution, column-parallel)
+    __shared__ T Ds[32][33], Xs[32][33];
+    const int b = blockIdx.x, tid = threadIdx.x, i = tid % 32, j = tid / 32, r0 = 32 * b;
+    Ds[i][j] = (r0 + i < m && r0 + j < m && i > j) ? L[(size_t)(r0 + i) + (size_t)(r0 + j) * ldl]
+                                                  : T(0);
+    T x = (i == j) ? T(1) : T(0);
+    Xs[i][j] = x;
+    for (int k = 0; k < 31; k++) {
+        __syncthreads();
+        if (i > k) {
+            x = fma(-Ds[i][k], Xs[k][j], x);
+            if (i == k + 1) Xs[i][j] = x;  // row k + 1 is final
+        }
+    }
+    inv[(size_t)b * 1024 + (size_t)j * 32 + i] = x;
+}
+
+template <typename T>
+__device__ __forceinline__ auto mfma16x4(T a, T b, typename std::conditional<sizeof(T) == 8, bd4_t,
+                                                                            bf4_t>::type c)
+{
+    if constexpr (sizeof(T) == 8) return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    else return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// row of accumulator element r of lane (lr, lk) within a 16 x 16 tile
+template <typename T>
+__device__ __forceinline__ int mfma_row(int lk, int r)
+{
+    return sizeof(T) == 8 ? lk + 4 * r : 4 * lk + r;
+}
+
+template <typename T>
+__global__ __launch_bounds__(TB_T) void trsm_llnu_blocked_kernel(int m, int n,
+                                                                 const T *__restrict__ L, int ldl,
+                                                                 const T *__restrict__ inv,
+                                                                 T *__restrict__ B, int ldb)
+{
+    using acc_t = typename std::conditional<sizeof(T) == 8, bd4_t, bf4_t>::type;
+    constexpr int NW = TB_T / 64;
+    __shared__ T Xs[TB_M][TB_LD];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int c0 = blockIdx.x * TB_NC;
+    const int nblk = (m + 31) / 32, m32 = 32 * nblk;
+    // B strip into LDS (rows >= m and columns >= n as zeros)
+    for (int idx = tid; idx < m32 * TB_NC; idx += TB_T) {
+        const int r = idx % m32, c = idx / m32;
+        Xs[r][c] = (r < m && c0 + c < n) ? B[(size_t)r + (size_t)(c0 + c) * ldb] : T(0);
+    }
+    // inverse operands of block 0 (waves 0, 1)
+    T iva[8];
+    if (wave < 2) {
+#pragma unroll
+        for (int s = 0; s < 8; s++) iva[s] = inv[(size_t)(4 * s + lk) * 32 + 16 * wave + lr];
+    }
+    __syncthreads();
+    for (int k = 0; k < nblk; k++) {
+        const int kr = 32 * k;
+        // X_k = inv(L_kk) B_k: waves 0, 1 take rows kr + 16 wave ..
+        acc_t d = acc_t{0, 0, 0, 0};
+        if (wave < 2) {
+#pragma unroll
+            for (int s = 0; s < 8; s++) d = mfma16x4<T>(iva[s], Xs[kr + 4 * s + lk][lr], d);
+            if (k + 1 < nblk) {  // next block's inverse, in flight during this block's update
+                const T *iv = inv + (size_t)(k + 1) * 1024;
+#pragma unroll
+                for (int s = 0; s < 8; s++) iva[s] = iv[(size_t)(4 * s + lk) * 32 + 16 * wave + lr];
+            }
+        }
+        __syncthreads();  // every read of B_k is done
+        if (wave < 2) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) Xs[kr + 16 * wave + mfma_row<T>(lk, r)][lr] = d[r];
+        }
+        __syncthreads();  // X_k is final
+        const int ntile = (m32 - kr - 32) / 16;  // 16-row tiles below block k
+        if (ntile <= 0) break;
+        // -X_k as the B operand of every tile of this block
+        T xb[8];
+#pragma unroll
+        for (int s = 0; s < 8; s++) xb[s] = -Xs[kr + 4 * s + lk][lr];
+        // tiles t and t + NW together: 16 loads of L in flight per lane
+        for (int t = wave; t < ntile; t += 2 * NW) {
+            T a[2][8];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int row = kr + 32 + 16 * (t + h * NW) + lr;
+                const T *lp = L + (size_t)row + (size_t)kr * ldl;
+                const bool in = t + h * NW < ntile && row < m;
+#pragma unroll
+                for (int s = 0; s < 8; s++) {
+                    const int col = 4 * s + lk;
+                    a[h][s] = (in && kr + col < m) ? lp[(size_t)col * ldl] : T(0);
+                }
+            }
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                if (t + h * NW >= ntile) break;
+                const int r0 = kr + 32 + 16 * (t + h * NW);
+                acc_t c;
+#pragma unroll
+                for (int r = 0; r < 4; r++) c[r] = Xs[r0 + mfma_row<T>(lk, r)][lr];
+#pragma unroll
+                for (int s = 0; s < 8; s++) c = mfma16x4<T>(a[h][s], xb[s], c);
+#pragma unroll
+                for (int r = 0; r < 4; r++) Xs[r0 + mfma_row<T>(lk, r)][lr] = c[r];
+            }
+        }
+        __syncthreads();  // block k + 1 is updated
+    }
+    for (int idx = tid; idx < m * TB_NC; idx += TB_T) {
+        const int r = idx % m, c = idx / m;
+        if (c0 + c < n) B[(size_t)r + (size_t)(c0 + c) * ldb] = Xs[r][c];
+    }
+}
+
+}  // namespace
+
+template <typename T>
+void trsm_llnu_blocked(int m, int n, const T *L, int ldl, T *B, int ldb, T *inv, bool inv_ready,
+                       hipStream_t s)
+{
+    if (m <= 0 || n <= 0) return;
+    if (m > TB_M) throw std::logic_error("trsm_llnu_blocked: at most 512 rows per call");
+    const int nblk = (m + 31) / 32;
+    if (!inv_ready) {
+        hipLaunchKernelGGL(trsm_diag_inv_kernel<T>, dim3(nblk), dim3(1024), 0, s, m, L, ldl, inv);
+        SLA_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(trsm_llnu_blocked_kernel<T>, dim3(ceil_div(n, TB_NC)), dim3(TB_T), 0, s, m,
+                       n, L, ldl, (const T *)inv, B, ldb);
+    SLA_LAUNCH_CHECK();
+}
+template void trsm_llnu_blocked<double>(int, int, const double *, int, double *, int, double *,
+                                        bool, hipStream_t);
+template void trsm_llnu_blocked<float>(int, int, const float *, int, float *, int, float *, bool,
+                                       hipStream_t);
 
 template <typename TA, typename T>
 void trsv_persistent(TrsvOp op, int n, int nrhs, const TA *A, int lda, T *X, int ldx,

@@ -245,6 +245,37 @@ def test_panel_leaf_heights(oracle, dev, M):
     assert np.allclose(lu, ao, rtol=1e-12, atol=1e-12)
 
 
+@pytest.mark.parametrize("M,N,NB", [(700, 700, 100), (1000, 1300, 512), (2048, 2048, 256),
+                                    (611, 450, 77)])
+def test_blocked_trsm_matches_strips_and_oracle(oracle, dev, M, N, NB):
+    """The U12 solves as one blocked MFMA launch (trsm_mfma = 1, the default)
+    against the recursive 32-row strips (trsm_mfma = 0) and the oracle: the
+    same IPIV, factors within FP64 rounding of each other.  NB values that
+    are not multiples of 32 exercise the identity-completed last diagonal
+    block."""
+    _note(f"trsm_mfma M={M} N={N} NB={NB}")
+    a = oracle.matgen(M, N, 17)
+    ao = a.copy(order="F")
+    piv_o, _ = oracle.pdgetrf(ao, NB)
+    mn = min(M, N)
+    out = {}
+    dev.set("nb", NB)
+    try:
+        for mode in (0, 1):
+            dev.set("trsm_mfma", mode)
+            dA = to_dev(a)
+            dp = torch.zeros(mn, dtype=torch.int32, device="cuda")
+            assert dev.dgetrf(M, N, dA, M, dp) == 0
+            out[mode] = (dp.cpu().numpy(), from_dev(dA))
+    finally:
+        dev.set("trsm_mfma", 1)
+        dev.set("nb", 256)
+    assert np.array_equal(out[1][0], piv_o) and np.array_equal(out[0][0], piv_o)
+    scale = np.abs(ao).max()
+    assert np.abs(out[1][1] - out[0][1]).max() <= 1e-11 * scale
+    assert np.abs(out[1][1] - ao).max() <= 1e-11 * scale
+
+
 def test_pdlaswp_rows_and_columns(oracle, ctx):
     from scalapack_amd import api
     rng = np.random.default_rng(4)
