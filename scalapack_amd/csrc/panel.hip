@@ -36,6 +36,10 @@
 
 #include <climits>
 
+// wavefront-wide maximum of the ROCm device library
+extern "C" __device__ __attribute__((const)) unsigned long long __ockl_wfred_max_u64(
+    unsigned long long);
+
 namespace sla {
 
 namespace {
@@ -72,6 +76,30 @@ __device__ __forceinline__ void static_for(F &&f)
         f(std::integral_constant<int, I>{});
         static_for<I + 1, N>(f);
     }
+}
+
+// Pivot key of a candidate: the bits of |v| plus one, monotone in |v|; 0 for
+// no candidate and for a NaN, which IDAMAX's comparisons never select.
+__device__ __forceinline__ uint64_t mag_key(double v, bool valid)
+{
+    const double a = fabs(v);
+    return (valid && a == a) ? __builtin_bit_cast(uint64_t, a) + 1ull : 0ull;
+}
+
+// Wave-wide first maximum: the largest key and, among the lanes holding it,
+// the lowest row (INT_MAX when no lane holds a key), in every lane.
+__device__ __forceinline__ void wave_argmax(uint64_t &key, int &row)
+{
+    const uint64_t km = __ockl_wfred_max_u64(key);
+    uint64_t tied = __ballot(km != 0 && key == km);
+    int best = INT_MAX;
+    while (tied) {  // uniform; one lane unless magnitudes tie
+        const int l = __ffsll((unsigned long long)tied) - 1;
+        best = min(best, __builtin_amdgcn_readlane(row, l));
+        tied &= tied - 1;
+    }
+    key = km;
+    row = best;
 }
 
 // Gathered word idx of a column into LDS: candidate idx / (W + 1) (row
@@ -121,12 +149,12 @@ __global__ __launch_bounds__(PT) void panel_leaf_kernel(
     // gathered words per thread: G * CS candidate words + W words of row j
     constexpr int KW = (LEAF_GMAX * CS + W + PT - 1) / PT;
 
-    __shared__ double s_wv[NWV];
+    __shared__ uint64_t s_wk[NWV];
     __shared__ int s_wr[NWV];
     __shared__ int s_cr[LEAF_GMAX];
     __shared__ T s_rows[LEAF_GMAX][W];
     __shared__ T s_rowj[W];
-    __shared__ int s_p, s_abort;
+    __shared__ int s_abort;
     __shared__ int s_pv[W];              // interchange of each column (p; j when none)
     __shared__ int s_dst[2 * W], s_src[2 * W], s_npair;
 
@@ -164,31 +192,28 @@ __global__ __launch_bounds__(PT) void panel_leaf_kernel(
         if (!alive || j >= wa) return;  // wa and the abort flag are uniform across the grid
         uint64_t *cb = box + (size_t)j * PANEL_MAIL_COL;  // this column's words
 
-        // 1. candidate of this workgroup: per thread, per wave (shuffles),
-        //    then every thread folds the NWV wave results itself
-        double bv = -1.0;
+        // 1. candidate of this workgroup: per thread, per wave (packed
+        //    magnitude keys, one wave reduction), then every thread folds
+        //    the NWV wave results itself
+        uint64_t bk = 0;
         int br = INT_MAX;
 #pragma unroll
         for (int q = 0; q < RPT; q++) {
             const int r = g * R + q * PT + tid;
-            if (r < m && r >= j) {
-                const double v = fabs((double)a[q][j]);
-                if (v > bv) { bv = v; br = r; }
-            }
+            const uint64_t k = mag_key((double)a[q][j], r < m && r >= j);
+            if (k > bk) { bk = k; br = r; }
         }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            const double ov = __shfl_xor(bv, off);
-            const int orr = __shfl_xor(br, off);
-            if (pivot_better(ov, orr, bv, br)) { bv = ov; br = orr; }
-        }
-        if (lane == 0) { s_wv[wave] = bv; s_wr[wave] = br; }
+        wave_argmax(bk, br);
+        if (lane == 0) { s_wk[wave] = bk; s_wr[wave] = br; }
         __syncthreads();
-        double cv = s_wv[0];
+        uint64_t ck = s_wk[0];
         int cr = s_wr[0];
 #pragma unroll
-        for (int w = 1; w < NWV; w++)
-            if (pivot_better(s_wv[w], s_wr[w], cv, cr)) { cv = s_wv[w]; cr = s_wr[w]; }
+        for (int w = 1; w < NWV; w++) {
+            const uint64_t k = s_wk[w];
+            const int r = s_wr[w];
+            if (k > ck || (k == ck && r < cr)) { ck = k; cr = r; }
+        }
 
         // 2. publish the candidate (row index and the row's W values; a
         //    workgroup without rows >= j publishes INT_MAX and zeros) and,
@@ -247,26 +272,21 @@ __global__ __launch_bounds__(PT) void panel_leaf_kernel(
             return;
         }
 
-        // 4. global pivot: the same reduction order in every workgroup; a
-        //    candidate's value is |its row's entry in column j|
-        if (wave == 0) {
-            double v = -1.0;
+        // 4. global pivot: every wave reduces the G candidates itself (the
+        //    same result everywhere, no further barrier); a candidate's value
+        //    is |its row's entry in column j|
+        int p;
+        {
+            uint64_t k = 0;
             int r = INT_MAX;
             for (int x = lane; x < G; x += 64) {
                 const int xr = s_cr[x];
-                const double xv = xr == INT_MAX ? -1.0 : fabs((double)s_rows[x][j]);
-                if (pivot_better(xv, xr, v, r)) { v = xv; r = xr; }
+                const uint64_t xk = mag_key((double)s_rows[x][j], xr != INT_MAX);
+                if (xk > k || (xk == k && xr < r)) { k = xk; r = xr; }
             }
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                const double ov = __shfl_xor(v, off);
-                const int orr = __shfl_xor(r, off);
-                if (pivot_better(ov, orr, v, r)) { v = ov; r = orr; }
-            }
-            if (lane == 0) s_p = (r == INT_MAX) ? j : r;  // an all-NaN column keeps row j
+            wave_argmax(k, r);
+            p = (r == INT_MAX) ? j : r;  // an all-NaN column keeps row j
         }
-        __syncthreads();
-        const int p = s_p;
         const int gw = p / R;
         const T piv = s_rows[gw][j];
         const bool nonzero = (piv != T(0));

@@ -74,11 +74,13 @@ __device__ __forceinline__ uint64_t piv_key(float v, int row)
 // holds matrix column j, and the update writes column c+1's new value into
 // register column c, so every register index is a compile-time constant while
 // the step loop stays rolled.
-struct GeppLds {
+struct alignas(16) GeppLds {
     uint64_t wkey[2][TNW];
     float wrow[2][TNW][TW];
     int wpid[2][TNW];
 };
+typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef unsigned long long u2_t __attribute__((ext_vector_type(2)));
 
 template <typename T, int NR, bool FULL>
 __device__ __forceinline__ void wg_gepp(T (&v)[NR][TW], const int (&key)[NR], const int (&pid)[NR],
@@ -107,11 +109,16 @@ __device__ __forceinline__ void wg_gepp(T (&v)[NR][TW], const int (&key)[NR], co
         const uint64_t wmax = __ockl_wfred_max_u64(bk);
         if (wmax != 0 && bk == wmax) {  // the one lane holding the wave's winner
 #pragma unroll
-            for (int c = 0; c < TW; c++) {
-                T sel = v[0][c];
+            for (int c4 = 0; c4 < TW; c4 += 4) {
+                f4_t sel4;
 #pragma unroll
-                for (int q = 1; q < NR; q++) sel = (bq == q) ? v[q][c] : sel;
-                sl.wrow[pb][wave][c] = sel;
+                for (int e = 0; e < 4; e++) {
+                    T sel = v[0][c4 + e];
+#pragma unroll
+                    for (int q = 1; q < NR; q++) sel = (bq == q) ? v[q][c4 + e] : sel;
+                    sel4[e] = sel;
+                }
+                *reinterpret_cast<f4_t *>(&sl.wrow[pb][wave][c4]) = sel4;
             }
             int sp = pid[0];
 #pragma unroll
@@ -120,11 +127,14 @@ __device__ __forceinline__ void wg_gepp(T (&v)[NR][TW], const int (&key)[NR], co
         }
         if (lane == 0) sl.wkey[pb][wave] = wmax;
         __syncthreads();
-        uint64_t fk = sl.wkey[pb][0];
+        u2_t kk[TNW / 2];
+#pragma unroll
+        for (int w = 0; w < TNW / 2; w++) kk[w] = reinterpret_cast<const u2_t *>(sl.wkey[pb])[w];
+        uint64_t fk = kk[0][0];
         int fw = 0;
 #pragma unroll
         for (int w = 1; w < TNW; w++) {
-            const uint64_t k = sl.wkey[pb][w];
+            const uint64_t k = kk[w / 2][w % 2];
             if (k > fk) {
                 fk = k;
                 fw = w;
@@ -144,7 +154,14 @@ __device__ __forceinline__ void wg_gepp(T (&v)[NR][TW], const int (&key)[NR], co
             out_pid[j] = any ? sl.wpid[pb][fw] : -1;
         }
         if (any && s_u != nullptr && tid < TW - j) s_u[j][j + tid] = prow[tid];
-        const T pj = prow[0];
+        T pr[TW];
+#pragma unroll
+        for (int c4 = 0; c4 < TW; c4 += 4) {
+            const f4_t x = reinterpret_cast<const f4_t *>(prow)[c4 / 4];
+#pragma unroll
+            for (int e = 0; e < 4; e++) pr[c4 + e] = x[e];
+        }
+        const T pj = pr[0];
         const bool nz = pj != T(0);
         const T rcp = nz ? T(1) / pj : T(0);
         // every row rotates (a retired row with multiplier 0: its values are
@@ -155,7 +172,7 @@ __device__ __forceinline__ void wg_gepp(T (&v)[NR][TW], const int (&key)[NR], co
             if (nz) l = l * rcp;
             l = act[q] ? l : T(0);
 #pragma unroll
-            for (int c = 1; c < TW; c++) v[q][c - 1] = fma(-l, prow[c], v[q][c]);
+            for (int c = 1; c < TW; c++) v[q][c - 1] = fma(-l, pr[c], v[q][c]);
             v[q][TW - 1] = T(0);
         }
     }
