@@ -327,7 +327,7 @@ __device__ __forceinline__ void tile_update_valu(int w, int rows, int cols, cons
 // bit 1 -- the trailing-update tiles on MFMA (else vector FMAs).
 template <typename T, int W>
 __device__ void factor_system(int n, T *__restrict__ a, int lda, int *__restrict__ piv, int &info,
-                              T (*Ls)[W + 1], T (*Us)[UT + 1], T *s_v, int *s_i,
+                              T (*Ls)[W + 1], T (*Us)[UT + 1], T (*L11s)[W + 1], T *s_v, int *s_i,
                               RegPanelLds<T, W> &rp, int flags)
 {
     constexpr int RPT = sizeof(T) == 8 ? 2 : 4;  // rows per thread of the register panel
@@ -423,28 +423,40 @@ __device__ void factor_system(int n, T *__restrict__ a, int lda, int *__restrict
         __syncthreads();  // the panel is final before U12 reads L11
         const int c0 = kb + w;
         if (c0 >= n) break;
-        // ---- U12 = L11^{-1} A12 (unit lower L11), one thread per column
-        for (int c = c0 + tid; c < n; c += BT) {
-            T *cc = a + (size_t)c * lda;
-            for (int r = kb; r < kb + w; r++) {
-                const T x = cc[r];
-                for (int r2 = r + 1; r2 < kb + w; r2++) cc[r2] = fma(-a[r2 + (size_t)r * lda], x, cc[r2]);
-            }
-        }
-        __syncthreads();
-        // ---- A22 -= L21 U12 in 64 x 64 tiles (L21 / U12 slices staged in LDS,
-        // the products on MFMA)
+        // ---- U12 = L11^{-1} A12 and A22 -= L21 U12, by 64-column chunks of
+        // the trailing columns: the chunk of A12 is staged in LDS once,
+        // solved there against L11 (one thread per column, the FMA sequence
+        // of a substitution) and written back as U12, then every 64-row tile
+        // of A22 below it is updated with it (L21 tiles staged in LDS, the
+        // products on MFMA or vector FMAs)
         const int m2 = n - c0;
-        for (int t0 = 0; t0 < m2; t0 += UT)
-            for (int u0 = 0; u0 < m2; u0 += UT) {
-                const int rows = min(UT, m2 - t0), cols = min(UT, m2 - u0);
+        for (int idx = tid; idx < w * w; idx += BT) {
+            const int r = idx % w, k = idx / w;
+            L11s[r][k] = a[(size_t)(kb + r) + (size_t)(kb + k) * lda];
+        }
+        for (int u0 = 0; u0 < m2; u0 += UT) {
+            const int cols = min(UT, m2 - u0);
+            for (int idx = tid; idx < UT * w; idx += BT) {
+                const int k = idx % w, c = idx / w;
+                Us[k][c] = c < cols ? a[(size_t)(kb + k) + (size_t)(c0 + u0 + c) * lda] : T(0);
+            }
+            __syncthreads();
+            if (tid < cols) {
+                for (int r = 0; r < w; r++) {
+                    const T x = Us[r][tid];
+                    for (int r2 = r + 1; r2 < w; r2++) Us[r2][tid] = fma(-L11s[r2][r], x, Us[r2][tid]);
+                }
+            }
+            __syncthreads();
+            for (int idx = tid; idx < UT * w; idx += BT) {
+                const int k = idx % w, c = idx / w;
+                if (c < cols) a[(size_t)(kb + k) + (size_t)(c0 + u0 + c) * lda] = Us[k][c];
+            }
+            for (int t0 = 0; t0 < m2; t0 += UT) {
+                const int rows = min(UT, m2 - t0);
                 for (int idx = tid; idx < UT * w; idx += BT) {
                     const int i = idx % UT, k = idx / UT;
                     Ls[i][k] = i < rows ? a[(size_t)(c0 + t0 + i) + (size_t)(kb + k) * lda] : T(0);
-                }
-                for (int idx = tid; idx < UT * w; idx += BT) {
-                    const int k = idx % w, c = idx / w;
-                    Us[k][c] = c < cols ? a[(size_t)(kb + k) + (size_t)(c0 + u0 + c) * lda] : T(0);
                 }
                 __syncthreads();
                 if (flags & 2)
@@ -453,63 +465,105 @@ __device__ void factor_system(int n, T *__restrict__ a, int lda, int *__restrict
                 else
                     tile_update_valu<T, W>(w, rows, cols, Ls, Us,
                                            a + (size_t)(c0 + u0) * lda + c0 + t0, lda);
-                __syncthreads();  // Ls / Us are reloaded by the next tile
+                __syncthreads();  // Ls is reloaded by the next tile, Us by the next chunk
             }
+        }
     }
 }
 
-// X = U^{-1} L^{-1} P B for one right-hand side column x (length n).
+// X = U^{-1} L^{-1} P B for all nrhs columns of B at once.
+//   P: the interchanges composed once into a permutation in LDS (thread 0,
+//      pdlapv2.f:208-242's order), then every column gathered through LDS;
+//   L (unit lower) top down and U bottom up in 16-row blocks: the diagonal
+//      block by substitution in registers, lane = row of one of four
+//      columns per wave (four waves: sixteen columns at a time), the block's
+//      row of the triangle loaded before the substitution; then every thread
+//      updates rows outside the block for one column (dot product over the
+//      block).
 template <typename T>
-__device__ void solve_system(int n, const T *__restrict__ a, int lda, const int *__restrict__ piv,
-                             T *__restrict__ x)
+__device__ void solve_all(int n, const T *__restrict__ a, int lda, const int *__restrict__ piv,
+                          T *__restrict__ B, int ldb, int nrhs, int *s_perm, int *s_piv, T *s_x)
 {
+    constexpr int SB = 16;  // rows per diagonal block
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    // P: the interchanges in order (pdlapv2.f:208-242)
+    const int lr = lane & (SB - 1), grp = lane / SB;  // row in the block, column of the wave
+    for (int i = tid; i < n; i += BT) {
+        s_perm[i] = i;
+        s_piv[i] = piv[i] - 1;
+    }
+    __syncthreads();
     if (tid == 0)
         for (int j = 0; j < n; j++) {
-            const int p = piv[j] - 1;
+            const int p = s_piv[j];
             if (p != j) {
-                const T t = x[j];
-                x[j] = x[p];
-                x[p] = t;
+                const int t = s_perm[j];
+                s_perm[j] = s_perm[p];
+                s_perm[p] = t;
             }
         }
     __syncthreads();
-    // L (unit lower), 64-row blocks top down
-    for (int b0 = 0; b0 < n; b0 += 64) {
-        const int rows = min(64, n - b0);
-        if (wave == 0) {
-            T v = lane < rows ? x[b0 + lane] : T(0);
-            for (int i = 0; i < rows; i++) {
-                const T xi = lane_read(v, i);
-                if (lane > i && lane < rows) v = fma(-a[(size_t)(b0 + lane) + (size_t)(b0 + i) * lda], xi, v);
+    for (int c = 0; c < nrhs; c++) {
+        T *x = B + (size_t)c * ldb;
+        for (int i = tid; i < n; i += BT) s_x[i] = x[s_perm[i]];
+        __syncthreads();
+        for (int i = tid; i < n; i += BT) x[i] = s_x[i];
+        __syncthreads();
+    }
+    // L, 16-row blocks top down
+    for (int b0 = 0; b0 < n; b0 += SB) {
+        const int rows = min(SB, n - b0);
+        for (int c = 4 * wave + grp; c - grp < nrhs; c += 16) {
+            const bool on = c < nrhs && lr < rows;
+            T *x = B + (size_t)(on ? c : 0) * ldb;
+            const T *ar = a + (size_t)(b0 + lr) + (size_t)b0 * lda;
+            T lrow[SB];
+#pragma unroll
+            for (int i = 0; i < SB; i++) lrow[i] = (on && i < lr) ? ar[(size_t)i * lda] : T(0);
+            T v = on ? x[b0 + lr] : T(0);
+#pragma unroll
+            for (int i = 0; i < SB; i++) {
+                const T xi = __shfl(v, i + SB * grp);
+                if (lr > i) v = fma(-lrow[i], xi, v);
             }
-            if (lane < rows) x[b0 + lane] = v;
+            if (on) x[b0 + lr] = v;
         }
         __syncthreads();
-        for (int i = b0 + rows + tid; i < n; i += BT) {
+        const int below = n - b0 - rows;
+        for (int idx = tid; idx < below * nrhs; idx += BT) {
+            const int i = b0 + rows + idx % below, c = idx / below;
+            T *x = B + (size_t)c * ldb;
             T acc = T(0);
             for (int k = 0; k < rows; k++) acc = fma(a[(size_t)i + (size_t)(b0 + k) * lda], x[b0 + k], acc);
             x[i] -= acc;
         }
         __syncthreads();
     }
-    // U (upper, non-unit), 64-row blocks bottom up
-    const int nb = (n + 63) / 64;
+    // U (non-unit), 16-row blocks bottom up
+    const int nb = (n + SB - 1) / SB;
     for (int bb = nb - 1; bb >= 0; bb--) {
-        const int b0 = bb * 64, rows = min(64, n - b0);
-        if (wave == 0) {
-            T v = lane < rows ? x[b0 + lane] : T(0);
-            const T d = lane < rows ? a[(size_t)(b0 + lane) * (lda + 1)] : T(1);
-            for (int i = rows - 1; i >= 0; i--) {
-                if (lane == i) v = v / d;
-                const T xi = lane_read(v, i);
-                if (lane < i) v = fma(-a[(size_t)(b0 + lane) + (size_t)(b0 + i) * lda], xi, v);
+        const int b0 = bb * SB, rows = min(SB, n - b0);
+        for (int c = 4 * wave + grp; c - grp < nrhs; c += 16) {
+            const bool on = c < nrhs && lr < rows;
+            T *x = B + (size_t)(on ? c : 0) * ldb;
+            const T *ar = a + (size_t)(b0 + lr) + (size_t)b0 * lda;
+            T urow[SB];
+#pragma unroll
+            for (int i = 0; i < SB; i++)
+                urow[i] = (on && i > lr && i < rows) ? ar[(size_t)i * lda] : T(0);
+            const T d = on ? ar[(size_t)lr * lda] : T(1);
+            T v = on ? x[b0 + lr] : T(0);
+#pragma unroll
+            for (int i = SB - 1; i >= 0; i--) {
+                if (lr == i && i < rows) v = v / d;
+                const T xi = __shfl(v, i + SB * grp);
+                if (lr < i && i < rows) v = fma(-urow[i], xi, v);
             }
-            if (lane < rows) x[b0 + lane] = v;
+            if (on) x[b0 + lr] = v;
         }
         __syncthreads();
-        for (int i = tid; i < b0; i += BT) {
+        for (int idx = tid; idx < b0 * nrhs; idx += BT) {
+            const int i = idx % b0, c = idx / b0;
+            T *x = B + (size_t)c * ldb;
             T acc = T(0);
             for (int k = 0; k < rows; k++) acc = fma(a[(size_t)i + (size_t)(b0 + k) * lda], x[b0 + k], acc);
             x[i] -= acc;
@@ -533,6 +587,7 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int batch, const int *
     __shared__ int s_i[4];
     __shared__ int s_q;
     __shared__ RegPanelLds<T, W> rp;
+    __shared__ T L11s[W][W + 1];
     for (;;) {
         if (threadIdx.x == 0) s_q = atomicAdd(queue, 1);
         __syncthreads();
@@ -545,17 +600,43 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int batch, const int *
         int *piv = ipiv[s];
         // n < 0 marks a system whose sizes the allocation-free entry rejected
         int inf = d.n < 0 ? -1 : 0;
-        if (d.n > 0) factor_system<T, W>(d.n, a, d.lda, piv, inf, Ls, Us, s_v, s_i, rp, flags);
+        if (d.n > 0) factor_system<T, W>(d.n, a, d.lda, piv, inf, Ls, Us, L11s, s_v, s_i, rp, flags);
         if (threadIdx.x == 0) info[s] = inf;
-        if (inf == 0 && d.n > 0)
-            for (int r = 0; r < d.nrhs; r++) solve_system<T>(d.n, a, d.lda, piv, B[s] + (size_t)r * d.ldb);
+        __syncthreads();
+    }
+}
+
+// The solves of every factored system (INFO = 0), by the same kind of
+// persistent queue (queue[1]), after the factorization launch: a kernel of
+// its own keeps its registers out of the factorization's.
+template <typename T>
+__global__ __launch_bounds__(BT) void getrs_batched_kernel(int batch, const int *__restrict__ order,
+                                                           const SysDesc *__restrict__ desc,
+                                                           T *const *__restrict__ A,
+                                                           int *const *__restrict__ ipiv,
+                                                           T *const *__restrict__ B,
+                                                           const int *__restrict__ info, int *queue)
+{
+    __shared__ int s_q;
+    __shared__ int s_perm[BATCHED_NMAX], s_piv[BATCHED_NMAX];
+    __shared__ T s_x[BATCHED_NMAX];
+    for (;;) {
+        if (threadIdx.x == 0) s_q = atomicAdd(queue + 1, 1);
+        __syncthreads();
+        const int q = s_q;
+        __syncthreads();
+        if (q >= batch) break;
+        const int s = order[q];
+        const SysDesc d = desc[s];
+        if (d.n > 0 && d.nrhs > 0 && info[s] == 0)
+            solve_all<T>(d.n, A[s], d.lda, ipiv[s], B[s], d.ldb, d.nrhs, s_perm, s_piv, s_x);
         __syncthreads();
     }
 }
 
 __global__ void zero_queue_kernel(int *q)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *q = 0;
+    if (threadIdx.x < 2 && blockIdx.x == 0) q[threadIdx.x] = 0;
 }
 
 // Set-up of the allocation-free entry, one workgroup: validates the device
@@ -573,7 +654,7 @@ __global__ __launch_bounds__(1024) void noalloc_prep_kernel(int batch, const int
 {
     __shared__ int cnt[PREP_BUCKETS];
     for (int b = threadIdx.x; b < PREP_BUCKETS; b += blockDim.x) cnt[b] = 0;
-    if (threadIdx.x == 0) *queue = 0;
+    if (threadIdx.x < 2) queue[threadIdx.x] = 0;
     __syncthreads();
     for (int s = threadIdx.x; s < batch; s += blockDim.x) {
         const int ns = n[s], rs = nrhs[s], la = lda[s], lb = ldb[s];
@@ -665,8 +746,8 @@ bool same_sizes(const BatchedPlan &p, int batch, const int *n, const int *nrhs, 
 }
 
 template <typename T>
-void launch(const BatchedPlan &p, int w, int wgs, T *const *A, int *const *ipiv, T *const *B,
-            int *info, hipStream_t s)
+void launch(const BatchedPlan &p, int w, int wgs, int solve_wgs, T *const *A, int *const *ipiv,
+            T *const *B, int *info, hipStream_t s)
 {
     hipLaunchKernelGGL(zero_queue_kernel, dim3(1), dim3(64), 0, s, p.d_queue);
 #define SLA_BATCHED(WV)                                                                          \
@@ -679,6 +760,9 @@ void launch(const BatchedPlan &p, int w, int wgs, T *const *A, int *const *ipiv,
     default: throw std::invalid_argument("batched: panel width must be 8, 16 or 32");
     }
 #undef SLA_BATCHED
+    SLA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(getrs_batched_kernel<T>, dim3(solve_wgs), dim3(BT), 0, s, p.batch,
+                       p.d_order, p.d_desc, (T *const *)A, ipiv, B, info, p.d_queue);
     SLA_LAUNCH_CHECK();
 }
 
@@ -795,6 +879,9 @@ void gesv_batched_noalloc_device(Handle &h, int batch, const int *dn, const int 
     }
 #undef SLA_BATCHED
     SLA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(getrs_batched_kernel<T>, dim3(4 * h.num_cus), dim3(BT), 0, h.stream, batch,
+                       order, desc, (T *const *)dA, (int *const *)dipiv, dB, dinfo, queue);
+    SLA_LAUNCH_CHECK();
 }
 
 template void gesv_batched_noalloc_device<double>(Handle &, int, const int *, const int *,
@@ -814,7 +901,7 @@ void gesv_batched_device(Handle &h, int batch, const int *n, const int *nrhs, T 
     const int wgs = std::max(1, h.batched_wgs_per_cu) * h.num_cus;
     const int prec = sizeof(T) == 8 ? 0 : 1;
     if (!h.batched_graph) {
-        launch<T>(p, w, wgs, dA, dipiv, dB, dinfo, h.stream);
+        launch<T>(p, w, wgs, 4 * h.num_cus, dA, dipiv, dB, dinfo, h.stream);
         SLA_HIP(hipStreamSynchronize(h.stream));
         return;
     }
@@ -834,7 +921,7 @@ void gesv_batched_device(Handle &h, int batch, const int *n, const int *nrhs, T 
         hipGraph_t g = nullptr;
         SLA_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
         try {
-            launch<T>(p, w, wgs, dA, dipiv, dB, dinfo, cs);
+            launch<T>(p, w, wgs, 4 * h.num_cus, dA, dipiv, dB, dinfo, cs);
         } catch (...) {
             (void)hipStreamEndCapture(cs, &g);
             if (g) (void)hipGraphDestroy(g);
@@ -910,7 +997,7 @@ BatchedTune tune_batched_device(Handle &h, int batch, const int *n, const int *n
                 hipLaunchKernelGGL(fill_synthetic_kernel<T>, dim3(64, std::min(batch, 65535)), dim3(256), 0, h.stream,
                                    batch, p.d_desc, dA, dB);
                 SLA_HIP(hipEventRecord(e0, h.stream));
-                launch<T>(p, w, k * h.num_cus, dA, dP, dB, info, h.stream);
+                launch<T>(p, w, k * h.num_cus, 4 * h.num_cus, dA, dP, dB, info, h.stream);
                 SLA_HIP(hipEventRecord(e1, h.stream));
                 SLA_HIP(hipEventSynchronize(e1));
                 float ms = 0.f;
