@@ -20,6 +20,8 @@
 #include "sla_common.h"
 #include "sla_kernels.h"
 
+#include <cstdlib>
+
 namespace sla {
 
 namespace {
@@ -249,6 +251,105 @@ __global__ __launch_bounds__(256, 1) void dgemm_nn_sub_direct_kernel(
     launch_stamp_end(stamp);
 }
 
+// The wide interior kernel: a 128 x 256 tile of C per 512-thread workgroup
+// (one per CU, two waves per SIMD), each of the 8 waves a 64 x 64 sub-tile
+// as 4 x 4 accumulators of v_mfma_f64_16x16x4f64.  A (128 x 16) and B
+// (16 x 256) of each 16-deep K step are staged once in double-buffered LDS
+// and shared by the eight waves; the next step's global loads are in flight
+// during this step's MFMAs.  Per output and K step the workgroup moves 1.5
+// bytes from global memory, against 4 for the 128 x 128 kernel that loads
+// every wave's operands itself.  The A rows are stored permuted (LDS
+// position 16 i + l of a wave's 64 rows holds its row 4 l + i), so MFMA block
+// i's operand row l is C row 4 l + i and a lane's four outputs of element r
+// are four consecutive rows: one 32-byte access of C each.  Requires m a
+// multiple of 128, n of 256, k of 16, and 32-byte aligned columns.
+constexpr int XM = 128, XN = 256, XK = 16, XT = 512;
+
+template <int TRAILING>
+__global__ __launch_bounds__(XT, 1) void dgemm_nn_sub_wide_kernel(
+    int m, int n, int k, const double *__restrict__ A, int lda,
+    const double *__restrict__ B, int ldb, double *__restrict__ C, int ldc,
+    unsigned long long *stamp)
+{
+    __shared__ __align__(16) double As[2][XK][XM];
+    __shared__ __align__(16) double Bs[2][XK][XN];
+    launch_stamp_begin(stamp);
+    const int tiles_m = m / XM, tiles_n = n / XN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+    // global -> register: A rows a_r .. a_r + 3 at k = a_k; B column b_c,
+    // k = b_k .. b_k + 7
+    const int a_r = 4 * (tid & 31), a_k = tid >> 5;
+    const int a_pos = (a_r / 64) * 64 + (a_r % 64) / 4;  // + 16 e for row a_r + e
+    const int b_c = tid & 255, b_k = (tid >> 8) * 8;
+    const int nk = k / XK;
+    for (int tile = blockIdx.x; tile < tiles_m * tiles_n; tile += gridDim.x) {
+        int tmi, tni;
+        tile_coords(tile, tiles_m, tiles_n, tmi, tni);
+        const int bm = tmi * XM, bn = tni * XN;
+        const double *Ap = A + bm + a_r + (size_t)a_k * lda;
+        const double *Bp = B + (size_t)(bn + b_c) * ldb + b_k;
+        d4_t ra, rb[2];
+        auto gload = [&](int kt) {
+            const size_t k0 = (size_t)kt * XK;
+            ra = *(const d4_t *)(Ap + k0 * lda);
+            rb[0] = *(const d4_t *)(Bp + k0);
+            rb[1] = *(const d4_t *)(Bp + k0 + 4);
+        };
+        auto lstore = [&](int buf) {
+#pragma unroll
+            for (int e = 0; e < 4; e++) As[buf][a_k][a_pos + 16 * e] = ra[e];
+#pragma unroll
+            for (int q = 0; q < 2; q++)
+#pragma unroll
+                for (int e = 0; e < 4; e++) Bs[buf][b_k + 4 * q + e][b_c] = rb[q][e];
+        };
+        d4_t acc[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+        gload(0);
+        lstore(0);
+        __syncthreads();
+        for (int kt = 0; kt < nk; kt++) {
+            const int buf = kt & 1;
+            if (kt + 1 < nk) gload(kt + 1);
+#pragma unroll
+            for (int s4 = 0; s4 < 4; s4++) {
+                double a[4], b[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++) a[i] = As[buf][4 * s4 + lk][wm + 16 * i + lr];
+#pragma unroll
+                for (int j = 0; j < 4; j++) b[j] = Bs[buf][4 * s4 + lk][wn + 16 * j + lr];
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+            }
+            if (kt + 1 < nk) lstore(buf ^ 1);
+            __syncthreads();
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            double *Cc = C + (size_t)(bn + wn + 16 * j + lr) * ldc + bm + wm;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                d4_t *cp = (d4_t *)(Cc + 4 * (lk + 4 * r));
+                d4_t c = *cp;
+                c[0] -= acc[0][j][r];
+                c[1] -= acc[1][j][r];
+                c[2] -= acc[2][j][r];
+                c[3] -= acc[3][j][r];
+                *cp = c;
+            }
+        }
+    }
+    launch_stamp_end(stamp);
+}
+
 // C[m x n] -= A[m x k] * B[k x n] for n <= 16 (the right-hand sides of the
 // triangular solves).  HBM-bound: each thread streams rows of A once.
 template <int NMAX>
@@ -281,6 +382,19 @@ __global__ __launch_bounds__(256) void dgemm_nn_sub_skinny_kernel(
 
 }  // namespace
 
+// The wide interior kernel serves the outer trailing update (SLA_DGEMM_WIDE=0
+// in the environment selects the 128 x 128 kernel instead, for comparisons);
+// the panel's GEMMs keep the 128 x 128 kernels, whose smaller footprint
+// leaves room beside the concurrently running trailing update.
+static bool dgemm_wide_on()
+{
+    static const bool on = [] {
+        const char *e = getenv("SLA_DGEMM_WIDE");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
 void dgemm_nn_sub(int m, int n, int k, const double *A, int lda, const double *B, int ldb,
                   double *C, int ldc, hipStream_t s, bool trailing, int max_wgs,
                   unsigned long long *stamp)
@@ -300,6 +414,23 @@ void dgemm_nn_sub(int m, int n, int k, const double *A, int lda, const double *B
         return;
     }
     auto aligned = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 31) == 0; };
+    const bool aligned_all = k % 16 == 0 && lda % 4 == 0 && ldb % 4 == 0 && ldc % 4 == 0 &&
+                             aligned(A) && aligned(B) && aligned(C);
+    const int mw = (m / XM) * XM, nw = (n / XN) * XN;
+    if (trailing && dgemm_wide_on() && mw > 0 && nw > 0 && aligned_all) {
+        const int tiles = (mw / XM) * (nw / XN);
+        const int grid = (max_wgs >= 8 && tiles > max_wgs) ? (max_wgs / 8) * 8 : tiles;
+        hipLaunchKernelGGL(dgemm_nn_sub_wide_kernel<1>, dim3(grid), dim3(XT), 0, s, mw, nw, k, A,
+                           lda, B, ldb, C, ldc, stamp);
+        SLA_LAUNCH_CHECK();
+        // the rows below (all columns) and the columns right of the interior
+        if (m > mw) dgemm_nn_sub(m - mw, n, k, A + mw, lda, B, ldb, C + mw, ldc, s, trailing,
+                                 max_wgs, stamp);
+        if (n > nw)
+            dgemm_nn_sub(mw, n - nw, k, A, lda, B + (size_t)nw * ldb, ldb, C + (size_t)nw * ldc,
+                         ldc, s, trailing, max_wgs, stamp);
+        return;
+    }
     const int mf = (m / 128) * 128, nf = (n / 128) * 128;
     if (mf > 0 && nf > 0 && k % 16 == 0 && lda % 4 == 0 && ldb % 4 == 0 && ldc % 4 == 0 &&
         aligned(A) && aligned(B) && aligned(C)) {
