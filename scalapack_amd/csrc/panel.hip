@@ -78,6 +78,9 @@ __device__ __forceinline__ void static_for(F &&f)
     }
 }
 
+typedef unsigned long long u64x2_t __attribute__((ext_vector_type(2)));
+typedef int i32x4_t __attribute__((ext_vector_type(4)));
+
 // Pivot key of a candidate: the bits of |v| plus one, monotone in |v|; 0 for
 // no candidate and for a NaN, which IDAMAX's comparisons never select.
 __device__ __forceinline__ uint64_t mag_key(double v, bool valid)
@@ -149,8 +152,8 @@ __global__ __launch_bounds__(PT) void panel_leaf_kernel(
     // gathered words per thread: G * CS candidate words + W words of row j
     constexpr int KW = (LEAF_GMAX * CS + W + PT - 1) / PT;
 
-    __shared__ uint64_t s_wk[NWV];
-    __shared__ int s_wr[NWV];
+    __shared__ __align__(16) uint64_t s_wk[NWV];
+    __shared__ __align__(16) int s_wr[NWV];
     __shared__ int s_cr[LEAF_GMAX];
     __shared__ T s_rows[LEAF_GMAX][W];
     __shared__ T s_rowj[W];
@@ -206,13 +209,29 @@ __global__ __launch_bounds__(PT) void panel_leaf_kernel(
         wave_argmax(bk, br);
         if (lane == 0) { s_wk[wave] = bk; s_wr[wave] = br; }
         __syncthreads();
-        uint64_t ck = s_wk[0];
-        int cr = s_wr[0];
+        // all NWV posts in one round of 16-byte LDS reads, then a
+        // branch-free fold in registers
+        uint64_t wk[NWV];
+        int wr[NWV];
+#pragma unroll
+        for (int w = 0; w < NWV; w += 2) {
+            const u64x2_t k2 = *reinterpret_cast<const u64x2_t *>(&s_wk[w]);
+            wk[w] = k2[0];
+            wk[w + 1] = k2[1];
+        }
+#pragma unroll
+        for (int w = 0; w < NWV; w += 4) {
+            const i32x4_t r4 = *reinterpret_cast<const i32x4_t *>(&s_wr[w]);
+#pragma unroll
+            for (int e = 0; e < 4; e++) wr[w + e] = r4[e];
+        }
+        uint64_t ck = wk[0];
+        int cr = wr[0];
 #pragma unroll
         for (int w = 1; w < NWV; w++) {
-            const uint64_t k = s_wk[w];
-            const int r = s_wr[w];
-            if (k > ck || (k == ck && r < cr)) { ck = k; cr = r; }
+            const bool take = wk[w] > ck || (wk[w] == ck && wr[w] < cr);
+            ck = take ? wk[w] : ck;
+            cr = take ? wr[w] : cr;
         }
 
         // 2. publish the candidate (row index and the row's W values; a
