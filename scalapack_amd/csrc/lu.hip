@@ -380,12 +380,18 @@ struct Lu {
         }
         for (int j = 0, p = 0; j < mn; j += nb, p++) {
             const int jb = std::min(nb, mn - j), jn = j + jb, q = p & 1;
-            swap_block(j, jn, q, jprev, j - jprev, s0);
-            swap_block(j, jn, q, jn, N - jn, s0);
-            laswp_apply<int>(h.perm, M, 0, 1, pairs(q), pair_count(q), 2 * jb, s0);
-            if (jn >= N) break;
             const bool has_next = jn < mn;
             const int jb2 = has_next ? std::min(nb, mn - jn) : 0;
+            // block p's interchanges: before the hand-off only on the next
+            // block column and on the previous one (whose columns the panel
+            // stream swaps next, with block p+1's interchanges), so that the
+            // next panel starts as early as possible; the rest of the
+            // trailing columns after the hand-off (the panel stream never
+            // touches them)
+            swap_block(j, jn, q, jn, std::min(jb2, N - jn), s0);
+            swap_block(j, jn, q, jprev, j - jprev, s0);
+            laswp_apply<int>(h.perm, M, 0, 1, pairs(q), pair_count(q), 2 * jb, s0);
+            if (jn >= N) break;
             l_packed = false;
             if (has_next) {
                 trsm(jb, jb2, at(j, j), at(j, jn), s0);
@@ -398,6 +404,7 @@ struct Lu {
                 SLA_HIP(hipEventRecord(h.ev_panel, s1));
             }
             const int c = jn + jb2, rest = N - c;
+            swap_block(j, jn, q, c, rest, s0);
             if (rest > 0) {
                 trsm(jb, rest, at(j, j), at(j, c), s0);
                 if (jn < M) update(M - jn, rest, jb, at(jn, j), at(j, c), at(jn, c), s0);
