@@ -174,7 +174,8 @@ CHOICE_NAMES = ("fp8+ir", "fp8+gmres", "bf16x3+ir", "bf16x3+gmres", "bf16x6+ir",
                 "bf16x6+gmres", "fp64")
 
 KERNEL_OF = {  # kernel symbol behind each timing class (for matching kernel traces)
-    ("gemm", "fp64"): "dgemm_nn_sub_direct_kernel<1> (trailing update A22 -= L21 U12)",
+    ("gemm", "fp64"): "dgemm_nn_sub_wide_kernel<1> (trailing update A22 -= L21 U12; the "
+                      "128-column border by dgemm_nn_sub_direct_kernel<1>)",
     ("gemm", "fp32"): "sgemm_nn_sub_direct_kernel<1> (trailing update, FP32 MFMA)",
     ("gemm", "bf16"): "gemm_bf16_nn_sub_direct_kernel (trailing update, BF16 MFMA)",
     ("gemm", "fp8"): "gemm_fp8_nn_sub_direct_kernel (trailing update, FP8 MFMA)",
