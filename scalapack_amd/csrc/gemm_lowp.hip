@@ -544,24 +544,29 @@ __global__ __launch_bounds__(256, 1) void gemm_split_lds_kernel(
         for (int sl = 0; sl < S; sl++)
 #pragma unroll
             for (int j = 0; j < 4; j++) fb[sl][j] = *(const s8_t *)(b_at(buf, sl, wn + 16 * j + lr) + 8 * lk);
+        s8_t fa[MB][S];
 #pragma unroll
-        for (int i = 0; i < MB; i++) {
-            s8_t fa[S];
+        for (int i = 0; i < MB; i++)
 #pragma unroll
-            for (int sl = 0; sl < S; sl++) fa[sl] = *(const s8_t *)(a_at(buf, sl, wm + 16 * i + lr) + 8 * lk);
+            for (int sl = 0; sl < S; sl++)
+                fa[i][sl] = *(const s8_t *)(a_at(buf, sl, wm + 16 * i + lr) + 8 * lk);
+        // product by product over all accumulators: every accumulator gets
+        // its products in the direct kernel's order, and consecutive MFMAs
+        // never wait for each other's result
+        auto prod = [&](int sa, int sb) {
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                f4_t c = acc[i][j];
-                if constexpr (S == 3) {
-                    c = mfma16<T16>(fa[2], fb[0][j], c);
-                    c = mfma16<T16>(fa[1], fb[1][j], c);
-                    c = mfma16<T16>(fa[0], fb[2][j], c);
-                }
-                c = mfma16<T16>(fa[1], fb[0][j], c);
-                c = mfma16<T16>(fa[0], fb[1][j], c);
-                acc[i][j] = mfma16<T16>(fa[0], fb[0][j], c);
-            }
+            for (int i = 0; i < MB; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) acc[i][j] = mfma16<T16>(fa[i][sa], fb[sb][j], acc[i][j]);
+        };
+        if constexpr (S == 3) {
+            prod(2, 0);
+            prod(1, 1);
+            prod(0, 2);
         }
+        prod(1, 0);
+        prod(0, 1);
+        prod(0, 0);
         if (t + 1 < nk) lstore(buf ^ 1);
         __syncthreads();
     }
