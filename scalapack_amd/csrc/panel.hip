@@ -109,7 +109,7 @@ __device__ __forceinline__ void wave_argmax(uint64_t &key, int &row)
 // index, then W values) for idx < ncand, else row j.
 template <typename T, int W>
 __device__ __forceinline__ void mail_put(int idx, int ncand, uint64_t v, int *s_cr,
-                                         T (*s_rows)[W], T *s_rowj)
+                                         T (*s_rows)[W + 1], T *s_rowj)
 {
     if (idx < ncand) {
         const int gg = idx / (W + 1), e = idx - gg * (W + 1);
@@ -155,7 +155,7 @@ __global__ __launch_bounds__(PT) void panel_leaf_kernel(
     __shared__ __align__(16) uint64_t s_wk[NWV];
     __shared__ __align__(16) int s_wr[NWV];
     __shared__ int s_cr[LEAF_GMAX];
-    __shared__ T s_rows[LEAF_GMAX][W];
+    __shared__ T s_rows[LEAF_GMAX][W + 1];  // odd stride: column j of all rows without bank conflicts
     __shared__ T s_rowj[W];
     __shared__ int s_abort;
     __shared__ int s_pv[W];              // interchange of each column (p; j when none)
