@@ -24,7 +24,7 @@ solve overwrites it) and does the whole factorization + solve (+ refinement).
     and every matrix's forward error is reported beside the FP64 LU's.
 --mode fp64: the plain FP64 PDGETRF + PDGETRS drop-in path (sla_dgesv) on A
     i.i.d. uniform[-1, 1), b = A * ones.
---mode batched: 10,000 independent systems in one persistent launch
+--mode batched: 10,000 independent systems in two persistent launches
     (sla_gesv_batched, hipGraph replay, tuned panel width / grid): N
     log-uniform in [32, 512], NRHS uniform in 1..8, A i.i.d. uniform[-1, 1)
     plus N I on every other system, B uniform; one step = the whole batch;
@@ -424,13 +424,14 @@ def run_batched(torch, h, args) -> dict:
     achieved = flops / (kernel_ms / 1e3) / 1e12
     peak = PEAK[prec]
     return {
-        "metric": "batched uneven solves per second (one persistent launch)",
+        "metric": "batched uneven solves per second (persistent launches)",
         "value": round(nb / sec, 1), "unit": "systems/s", "ms_per_step": round(1e3 * sec, 3),
         "dtype": "f32" if prec == "fp32" else "f64",
         "data": "synthetic: N log-uniform [32,512] (numpy seed 300), NRHS uniform 1..8, A uniform"
                 "[-1,1) (torch seed 301) + N I on even systems, B uniform; in HBM",
-        "config": {"workload": f"{nb} independent systems, getrf+getrs in one persistent launch "
-                               f"(hipGraph replay), panel width {w}, {k} workgroups per CU",
+        "config": {"workload": f"{nb} independent systems, a persistent getrf launch and a "
+                               f"persistent getrs launch (one hipGraph replay), panel width {w}, "
+                               f"{k} workgroups per CU",
                    "batch": nb, "precision": prec, "tuned": {"w": w, "wgs_per_cu": k,
                                                              "tune_ms": round(tune_ms, 3)}},
         "tflops_equiv": round(flops / sec / 1e12, 3),

@@ -302,7 +302,7 @@ int sla_dresidual(sla_handle_t h, int n, int nrhs, const double *dA, int lda, co
                   int ldx, const double *dB, int ldb, double *dR, int ldr, double *rnorm);
 /* ------------------------------------------------------------------------ */
 /* Part 4: batched small systems (no counterpart in the reference: thousands  */
-/* of independent PDGESV-sized-down solves in one persistent launch).         */
+/* of independent PDGESV-sized-down solves in persistent launches).          */
 /* ------------------------------------------------------------------------ */
 /* A_s X_s = B_s for s < batch, each as PDGESV on a 1 x 1 grid (partial
  * pivoting with IDAMAX's first maximum, rows interchanged across the matrix,
@@ -311,8 +311,9 @@ int sla_dresidual(sla_handle_t h, int n, int nrhs, const double *dA, int lda, co
  * matrices) or SLA_PREC_FP32 (float).  n, nrhs, lda, ldb: host int[batch]
  * (n <= 1024, nrhs <= 16); dA, dB: device arrays of device pointers to the
  * column-major matrices; dipiv: device array of device pointers to int[n]
- * (1-based rows); dinfo: device int[batch].  One persistent launch takes the
- * systems largest first.  Handle knobs: "batched_w" (panel width 8 / 16 /
+ * (1-based rows); dinfo: device int[batch].  One persistent launch factors the
+ * systems largest first, a second one solves them (a kernel of its own, so
+ * its registers do not lower the factorization's occupancy).  Handle knobs: "batched_w" (panel width 8 / 16 /
  * 32), "batched_wgs_per_cu" (1..8), "batched_graph" (1: the launch is
  * captured into a hipGraph once per set of pointer arrays and sizes, then
  * replayed). */
