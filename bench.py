@@ -438,7 +438,8 @@ def run_batched(torch, h, args) -> dict:
         "scaled_residual_sample_max": worst,
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
                      "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
-                     "kernel": "gesv_batched_kernel", "avg_launch_ms": round(kernel_ms, 3),
+                     "kernel": "gesv_batched_kernel + getrs_batched_kernel (one graph replay, "
+                               "HIP events around it)", "avg_launch_ms": round(kernel_ms, 3),
                      "avg_launch_work": flops},
     }, worst < 16.0
 
