@@ -388,6 +388,37 @@ def test_dxgesv_fp64_request(dev, oracle):
 
 
 @pytest.mark.parametrize("pivoting", [0, 1])
+def test_kappa_1e8_pivoting_does_not_decide_convergence(dev, oracle, pivoting):
+    """N = 2048, kappa = 1e8, FP32 factors + GMRES-IR: partial pivoting
+    (pivoting = 0, |l| <= 1) and the tournament leaf (pivoting = 1) end the
+    same way -- the refinement does not reach FP64 backward error and the
+    FP64 fallback answers (ITER < 0, scaled residual < 16, forward error
+    within 100x the oracle's FP64 LU).  At this kappa it is the FP32
+    factors' accuracy, not the pivoting's growth, that stops the refinement
+    (at N = 1024 both converge: test_dxgesv_ill_conditioned)."""
+    n = 2048
+    _note(f"dxgesv fp32 gmres kappa=1e8 n={n} pivoting={pivoting}")
+    A = svd_matrix(n, 1e8, 7)
+    rng = np.random.default_rng(8)
+    xt = np.asfortranarray(rng.standard_normal((n, 1)))
+    B = np.asfortranarray(A @ xt)
+    piv0 = dev.get("pivoting")
+    dev.set("pivoting", pivoting)
+    try:
+        X, it, info, st, _, _ = _solve(dev, A, B, "fp32", "gmres")
+    finally:
+        dev.set("pivoting", piv0)
+    s = scaled_residual(A, X, B)
+    x64 = _oracle_x(oracle, A, B)
+    fe = np.abs(X - xt).max() / np.abs(xt).max()
+    fe64 = np.abs(x64 - xt).max() / np.abs(xt).max()
+    _note(f"  iter={it} stop={st.get('stop_rule')} sresid={s:.3e} fwd={fe:.3e} fwd64={fe64:.3e}")
+    assert info == 0 and it < 0
+    assert s < 16.0
+    assert fe <= 100 * fe64 + 1e-14
+
+
+@pytest.mark.parametrize("pivoting", [0, 1])
 def test_dxgesv_large_fp32_ir(dev, pivoting):
     """N = 8192, FP32 factors, classical refinement; checked on the GPU in
     FP64 (no oracle at this size): scaled residual < 16 and the solution of
