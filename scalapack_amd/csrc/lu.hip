@@ -358,21 +358,22 @@ struct Lu {
 
     // Right-looking blocked LU (SRC/pdgetrf.f:227-295) with a look-ahead of
     // one block column.  Main stream S0, step of block column p = [j, jn):
-    //   apply p's interchanges to the previous block column and to the
-    //   columns right of p, and to the kept row permutation;
+    //   apply p's interchanges to the next block column and to the kept row
+    //   permutation;
     //   U12 and the Schur update for the NEXT block column only -> event;
-    //   U12 and the Schur update for the rest of the trailing matrix.
-    // Panel stream S1, once the next block column is updated: factor it,
-    // build its pair list, apply its interchanges to the columns left of the
-    // block column before it (nothing reads those any more).  S0 waits for S1
-    // before its next step.  Interchanges of one column range therefore
-    // always arrive in panel order.
+    //   apply p's interchanges to the columns left of p (nothing but these
+    //   interchanges touches them any more) and to the rest of the trailing
+    //   columns; U12 and the Schur update for the rest.
+    // Panel stream S1, once the next block column is updated: factor it and
+    // build its pair list.  S0 waits for S1 before its next step.  Every
+    // interchange runs on S0, so those of one column range always arrive in
+    // panel order, and the panel stream -- the critical path whenever the
+    // remaining trailing matrix is small -- carries only the panel.
     int run()
     {
         const int mn = std::min(M, N), nb = h.nb;
         if (nb > 512) throw std::logic_error("getrf: NB above 512");
         hipStream_t s0 = s_main, s1 = s_panel;
-        int jprev = 0;
         {
             const int jb = std::min(nb, mn);
             panel(0, 0, jb, 0, jb, s0);
@@ -383,15 +384,15 @@ struct Lu {
             const bool has_next = jn < mn;
             const int jb2 = has_next ? std::min(nb, mn - jn) : 0;
             // block p's interchanges: before the hand-off only on the next
-            // block column and on the previous one (whose columns the panel
-            // stream swaps next, with block p+1's interchanges), so that the
-            // next panel starts as early as possible; the rest of the
-            // trailing columns after the hand-off (the panel stream never
-            // touches them)
+            // block column, so that the next panel starts as early as
+            // possible; on the columns left of p and on the rest of the
+            // trailing columns after it (the panel stream touches neither)
             swap_block(j, jn, q, jn, std::min(jb2, N - jn), s0);
-            swap_block(j, jn, q, jprev, j - jprev, s0);
             laswp_apply<int>(h.perm, M, 0, 1, pairs(q), pair_count(q), 2 * jb, s0);
-            if (jn >= N) break;
+            if (jn >= N) {
+                swap_block(j, jn, q, 0, j, s0);
+                break;
+            }
             l_packed = false;
             if (has_next) {
                 trsm(jb, jb2, at(j, j), at(j, jn), s0);
@@ -400,17 +401,16 @@ struct Lu {
                 SLA_HIP(hipStreamWaitEvent(s1, h.ev_next, 0));
                 panel(jn, jn, jb2, jn, jn + jb2, s1);
                 make_pairs(jn, jn + jb2, q ^ 1, s1);
-                swap_block(jn, jn + jb2, q ^ 1, 0, j, s1);
                 SLA_HIP(hipEventRecord(h.ev_panel, s1));
             }
             const int c = jn + jb2, rest = N - c;
+            swap_block(j, jn, q, 0, j, s0);
             swap_block(j, jn, q, c, rest, s0);
             if (rest > 0) {
                 trsm(jb, rest, at(j, j), at(j, c), s0);
                 if (jn < M) update(M - jn, rest, jb, at(jn, j), at(j, c), at(jn, c), s0);
             }
             if (has_next) SLA_HIP(hipStreamWaitEvent(s0, h.ev_panel, 0));
-            jprev = j;
         }
         return 0;
     }
