@@ -276,6 +276,38 @@ def test_blocked_trsm_matches_strips_and_oracle(oracle, dev, M, N, NB):
     assert np.abs(out[1][1] - ao).max() <= 1e-11 * scale
 
 
+@pytest.mark.parametrize("M,N,NB", [(1500, 1500, 128), (1700, 1100, 128), (900, 1400, 64),
+                                    (2560, 2560, 512)])
+def test_lookahead_matches_serial_order(oracle, dev, M, N, NB):
+    """The look-ahead factorization (panel stream beside the update stream;
+    every row interchange on the update stream, those of the columns left of
+    the panel after the hand-off) against the serial order of
+    SRC/pdgetrf.f:227-295 (lookahead = 0, one stream): the same arithmetic in
+    another order of launches, so IPIV and the factors are bit-identical, and
+    IPIV equals the oracle's.  Tall and wide shapes end the block loop on
+    their last column or row block."""
+    _note(f"lookahead M={M} N={N} NB={NB}")
+    a = oracle.matgen(M, N, 23)
+    ao = a.copy(order="F")
+    piv_o, _ = oracle.pdgetrf(ao, NB, m=M, n=N)
+    mn = min(M, N)
+    out = {}
+    nb0, la0 = dev.get("nb"), dev.get("lookahead")
+    dev.set("nb", NB)
+    try:
+        for la in (0, 1):
+            dev.set("lookahead", la)
+            dA = to_dev(a)
+            dp = torch.zeros(mn, dtype=torch.int32, device="cuda")
+            assert dev.dgetrf(M, N, dA, M, dp) == 0
+            out[la] = (dp.cpu().numpy(), from_dev(dA))
+    finally:
+        dev.set("lookahead", la0)
+        dev.set("nb", nb0)
+    assert np.array_equal(out[1][0], piv_o) and np.array_equal(out[0][0], piv_o)
+    assert np.array_equal(out[1][1], out[0][1])
+
+
 def test_pdlaswp_rows_and_columns(oracle, ctx):
     from scalapack_amd import api
     rng = np.random.default_rng(4)
