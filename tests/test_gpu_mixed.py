@@ -211,6 +211,36 @@ def test_sgetrf_backward_error(dev, prec, n, nb, pivoting):
         assert lmax <= 8.0  # tournament pivoting: bounded multipliers in practice
 
 
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3", "bf16x6", "fp8"])
+@pytest.mark.parametrize("m,n,nb", [(2560, 2560, 512), (1700, 1100, 128), (900, 1400, 128)])
+def test_sgetrf_lookahead_matches_serial_order(dev, prec, m, n, nb):
+    """The FP32-storage factorization with the look-ahead (panel stream
+    beside the update stream) against the serial launch order (lookahead =
+    0): the same arithmetic -- tournament leaves, packed low-precision
+    updates, every row interchange on the update stream -- in another order
+    of launches, so IPIV and the factors are bit-identical."""
+    _note(f"sgetrf lookahead {prec} m={m} n={n} nb={nb}")
+    rng = np.random.default_rng(m + 3 * n + nb)
+    A = np.asfortranarray(rng.uniform(-1, 1, (m, n)).astype(np.float32))
+    mn = min(m, n)
+    out = {}
+    nb0, la0 = dev.get("nb"), dev.get("lookahead")
+    dev.set("nb", nb)
+    try:
+        for la in (0, 1):
+            dev.set("lookahead", la)
+            dA = to_dev(A)
+            ipiv = torch.zeros(mn, dtype=torch.int32, device="cuda")
+            assert dev.sgetrf(m, n, dA, m, ipiv, prec=prec) == 0
+            torch.cuda.synchronize()
+            out[la] = (ipiv.cpu().numpy(), from_dev(dA))
+    finally:
+        dev.set("lookahead", la0)
+        dev.set("nb", nb0)
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+
+
 # ----------------------------------------------------------- refined solve
 def _solve(dev, A, B, prec, refine):
     n, nrhs = B.shape
