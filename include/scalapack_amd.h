@@ -152,7 +152,9 @@ int sla_set_stream(sla_handle_t h, void *stream);
  * "pivoting" (FP32-storage factorizations: 0 partial pivoting, 1 tournament
  * pivoting of the panel leaves, default 1; the FP64 path always pivots as
  * PDGETF2), "lookahead" (0/1, default 1: block column p+1 is factored on a
- * second stream during p's update), "update_cus" (0 = all; else the trailing
+ * second stream during p's update, every row interchange stays on the
+ * caller's stream; 0: one stream -- the same arithmetic in serial launch
+ * order, bit-identical factors and IPIV), "update_cus" (0 = all; else the trailing
  * update runs as a persistent grid sized for this many CUs, leaving the
  * others to the panel), "lacon_lapack" (0/1, default 0: the condition and
  * forward-error estimates of sla_dgecon / sla_dgerfs / pdgecon_ / pdgerfs_
